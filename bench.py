@@ -1,0 +1,217 @@
+#!/usr/bin/env python3
+"""Benchmark of the census-worker hot path on one MI355X.
+
+A step is one pass of the hot path over one block of primes already resident
+in HBM: every admissible order r in 2..9 of every prime gets (M, L, witness,
+K) and every prime its A4 count.  Default workload: the 256 largest primes
+below 2^32 (871 profiles), batched persistent launch; the plain method (one
+launch per prime) is timed beside it on the same GPU.  Output: one JSON line.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload top256|block2_24]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT / "rabung-atlas32_amd"))
+
+METRIC = "exact (p,r) profiles/sec near 2^32, plain vs batched; p50/p99 query latency"
+HBM_PEAK_GBS = 8000.0
+
+
+def top_primes(count):
+    import rabung_amd
+    out = []
+    x = (1 << 32) - 1
+    while len(out) < count:
+        if rabung_amd.is_prime(x):
+            out.append(x)
+        x -= 1
+    return sorted(out)
+
+
+def block_from(start, count):
+    import rabung_amd
+    out = []
+    x = start
+    while len(out) < count:
+        if rabung_amd.is_prime(x):
+            out.append(x)
+        x += 1
+    return out
+
+
+WORKLOADS = {
+    "top256": ("top-of-range: the 256 largest primes below 2^32, all admissible r, "
+               "full profile (M, L, witness, K) + A4, dense scan", lambda: top_primes(256)),
+    "block2_24": ("plain block: 4096 consecutive primes from the first prime >= 2^24, all "
+                  "admissible r, full profile + A4, dense scan", lambda: block_from(1 << 24, 4096)),
+}
+
+
+def mm_products(e):
+    if e < 2:
+        return 0
+    return e.bit_length() + bin(e).count("1") - 2
+
+
+def lcm_mask(mask):
+    from math import gcd
+    m = 1
+    for r in range(2, 10):
+        if mask & (1 << (r - 2)):
+            m = m * r // gcd(m, r)
+    return m
+
+
+def int32_ops(primes):
+    """Algorithmic INT32 work of the dense scan: nh * mm(e) * 8 per prime
+    (one Montgomery product = 4 multiplies + 4 add/compare/select)."""
+    import rabung_amd
+    total = 0
+    for p in primes:
+        if p < 3:
+            continue
+        m = lcm_mask(rabung_amd.mask(p))
+        total += ((p - 1) // 2) * mm_products((p - 1) // m) * 8
+    return total
+
+
+def golden_check(primes, records):
+    """Bit-exact check against the committed sample block 49900 when the
+    workload lies inside it; returns the number of primes checked."""
+    sys.path.insert(0, str(ROOT / "tests"))
+    import golden_data
+    from rabung_amd import codec
+    rows, a4, entry = golden_data.sample_block(49900)
+    index = {r["p"]: (r, v) for r, v in zip(rows, a4)}
+    checked = 0
+    got = codec.records_to_rows(records)
+    for g, rec in zip(got, records):
+        if g["p"] in index:
+            row, v = index[g["p"]]
+            if g != row or rec["A4"] != v:
+                raise SystemExit("bench: result differs from the golden vector at p=%d" % g["p"])
+            checked += 1
+    return checked
+
+
+def cpu_baseline(primes, seconds_budget=30.0):
+    """The CPU oracle (a port of the reference's dense CPU worker) on a bounded
+    sample of the same workload: whole primes from the block, all host threads."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import oracle
+    import random
+    threads = min(16, os.cpu_count() or 1)
+    rng = random.Random(2026)
+    sample = []
+    done_profiles = 0
+    t0 = time.perf_counter()
+    pool = list(primes)
+    rng.shuffle(pool)
+    for p in pool:
+        rec = oracle.profile(p, threads)
+        sample.append(p)
+        done_profiles += bin(rec["mask"]).count("1")
+        if time.perf_counter() - t0 > seconds_budget * 0.5:
+            break
+    dt = time.perf_counter() - t0
+    return {"value": done_profiles / dt, "unit": "profiles/s", "cores": threads, "kind": "port",
+            "sample": "%d prime(s) of the workload (%s), %d profiles, %.1f s, dense "
+                      "discrete-log walk + full-interval scan" % (
+                          len(sample), ",".join(map(str, sample)), done_profiles, dt)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="top256", choices=sorted(WORKLOADS))
+    ap.add_argument("--no-plain", action="store_true", help="skip timing the plain method")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU only (the path shards across primes with "
+              "no exchange; see DESIGN.md); run with --gpus 1", file=sys.stderr)
+        return 2
+    import torch
+    import rabung_amd
+    from rabung_amd import Record, RECORD_SIZE
+
+    desc, make = WORKLOADS[args.workload]
+    primes = make()
+    n = len(primes)
+    profiles = sum(bin(rabung_amd.mask(p)).count("1") for p in primes)
+    dev = torch.device("cuda:0")
+    d_primes = torch.tensor([p - (1 << 32) if p >= (1 << 31) else p for p in primes],
+                            dtype=torch.int32, device=dev)
+    d_out = torch.zeros(n * RECORD_SIZE, dtype=torch.uint8, device=dev)
+    eng = rabung_amd.Engine(0)
+    info = eng.device_info()
+
+    def run(method, steps, warmup):
+        for _ in range(warmup):
+            eng.profile_block_device(d_primes.data_ptr(), n, d_out.data_ptr(), method)
+        torch.cuda.synchronize()
+        scan_ms = []
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            eng.profile_block_device(d_primes.data_ptr(), n, d_out.data_ptr(), method)
+            scan_ms.append(eng.last_timing()["scan_ms"])
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        return dt, scan_ms, eng.last_timing()
+
+    dt, scan_ms, timing = run(rabung_amd.METHOD_BATCHED, args.steps, args.warmup)
+    raw = d_out.cpu().numpy().tobytes()
+    records = [rabung_amd.record_to_dict(Record.from_buffer_copy(raw[i * RECORD_SIZE:(i + 1) * RECORD_SIZE]))
+               for i in range(n)]
+    checked = golden_check(primes, records)
+    ms_step = dt * 1000.0 / args.steps
+    value = profiles / (dt / args.steps)
+
+    plain = None
+    if not args.no_plain:
+        pdt, pscan, ptiming = run(rabung_amd.METHOD_PLAIN, max(1, args.steps // 2), 1)
+        psteps = max(1, args.steps // 2)
+        raw2 = d_out.cpu().numpy().tobytes()
+        if raw2 != raw:
+            raise SystemExit("bench: plain and batched methods disagree")
+        plain = {"profiles_per_s": profiles / (pdt / psteps), "ms_per_step": pdt * 1000.0 / psteps,
+                 "scan_launches_per_step": ptiming["launches"],
+                 "scan_ms_per_step": sum(pscan) / len(pscan)}
+
+    ops = int32_ops(primes)
+    avg_scan_s = (sum(scan_ms) / len(scan_ms)) / 1000.0
+    peak_tops = info["cus"] * 64 * info["clock_khz"] * 1e3 / 1e12
+    achieved = ops / avg_scan_s / 1e12
+    roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": round(peak_tops, 3),
+                "unit": "TOP/s", "frac": round(achieved / peak_tops, 4), "traffic": None,
+                "kernel": "scan_tiles", "algorithmic_ops_per_launch": ops,
+                "avg_launch_ms": round(avg_scan_s * 1000.0, 3),
+                "peak_basis": "%d CUs x 64 lanes x %.0f MHz, INT32 ops" % (
+                    info["cus"], info["clock_khz"] / 1e3)}
+    cpu = None if args.no_cpu else cpu_baseline(primes)
+    line = {
+        "metric": METRIC, "value": round(value, 3), "unit": "profiles/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u32",
+        "data": "synthetic: deterministic prime block (%d primes %d..%d)" % (n, primes[0], primes[-1]),
+        "config": {"workload": desc, "primes": n, "profiles": profiles, "method": "batched",
+                   "positions_per_step": timing["positions"], "golden_checked": checked},
+        "plain": plain,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    eng.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,125 @@
+/*
+ * rabung.h -- C ABI of the MI355X census-worker library (librabung.so).
+ *
+ * One call profiles a block of primes p < 2^32: for every admissible order
+ * r in 2..9 with r | p-1 it returns the longest monochromatic run M, the
+ * initial run L of the identity colour, the canonical reflected witness W and
+ * the threshold K, plus the quadratic four-window count A4 (docs/METHODS.md
+ * of the reference).  Plain pointers and sizes only; no exit() inside the
+ * library; every failure is a nonzero status with nothing valid written.
+ *
+ * Reference interfaces replaced:
+ *   - rbg_profile_block: the per-prime loop of the persistent GPU worker,
+ *     src/census/worker32.cu:20-41 (prepare_gpu_table + gpu_prepared_group,
+ *     src/census/engine32/src/frozen_entry.cu:149-242), and the dense CPU
+ *     worker's emit(), src/census/small_cpu.cpp:15-33, extended to p < 2^32.
+ *     The A4 value replaces the moment worker's per-prime loop,
+ *     src/census/moment_worker.cpp:35-41.
+ *   - baseline_init / baseline_profile: the reference's only in-process C ABI,
+ *     src/census/engine32/src/baseline_bridge.cu:7,21-27 (same names,
+ *     argument meaning and return codes; errors return nonzero instead of
+ *     terminating the process).
+ */
+#ifndef RABUNG_H
+#define RABUNG_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RBG_MAX_BLOCK 4096u /* primes per call: the pack block limit */
+
+/* flags for rbg_profile_block */
+#define RBG_METHOD_BATCHED 0x0u  /* one persistent launch over all tiles */
+#define RBG_METHOD_PLAIN 0x1u    /* one grid launch per prime */
+#define RBG_CERTIFY 0x10u        /* direct powmod check of witness and prefix */
+
+/* status codes */
+#define RBG_OK 0
+#define RBG_EINVAL 1    /* bad argument: not a prime below 2^32, n too large */
+#define RBG_EDEVICE 2   /* HIP runtime error */
+#define RBG_ECERTIFY 3  /* a result failed its direct verification */
+#define RBG_ECAPACITY 4 /* workspace beyond the configured HBM budget */
+
+/* One prime's profile; arrays are indexed by r-2 and only entries whose bit
+ * is set in `mask` are meaningful (others are zero). */
+typedef struct rbg_record {
+    uint32_t p;
+    uint32_t mask; /* bit r-2 set iff r | p-1 (p=2: 0) */
+    uint32_t M[8]; /* longest monochromatic run */
+    uint32_t L[8]; /* initial run of the identity colour from x=1 */
+    uint32_t W[8]; /* least reflected start min(s, p-e) among runs of length M */
+    uint64_t K[8]; /* max(M+1, 2L+2) if r | (p-1)/2 else max(M+1, L+2) */
+    uint32_t A4;   /* #{x <= p-4 : x..x+3 share the quadratic character}; p=2: 0 */
+    uint32_t status;
+} rbg_record;
+
+typedef struct rbg_config {
+    int32_t device;          /* HIP device ordinal */
+    uint32_t tile_chunks;    /* 64-position chunks per wave segment; 0 = tuned */
+    uint32_t blocks_per_cu;  /* persistent blocks per CU; 0 = tuned */
+    uint64_t hbm_budget;     /* bytes of device workspace allowed; 0 = 16 GiB */
+} rbg_config;
+
+typedef struct rbg_ctx rbg_ctx;
+
+/* Create a context bound to cfg->device (cfg may be NULL for defaults).
+ * Returns NULL on failure.  One context per host thread. */
+rbg_ctx *rbg_open(const rbg_config *cfg);
+
+/* Profile n <= RBG_MAX_BLOCK primes.  primes: host array; out: host array of n
+ * records (caller-owned).  All-or-nothing: on a nonzero status no record is
+ * valid.  Blocking. */
+int rbg_profile_block(rbg_ctx *ctx, const uint32_t *primes, uint32_t n,
+                      uint32_t flags, rbg_record *out);
+
+/* Same with device-resident input and output (d_primes: n uint32 in device
+ * memory, d_out: n records in device memory).  Enqueues on the context's
+ * stream and waits for completion.  Host-side preparation reads the primes
+ * back once (4 bytes each). */
+int rbg_profile_block_device(rbg_ctx *ctx, const uint32_t *d_primes, uint32_t n,
+                             uint32_t flags, rbg_record *d_out);
+
+/* Message for the last nonzero status of this context ("" if none). */
+const char *rbg_last_error(const rbg_ctx *ctx);
+
+/* Milliseconds of device time of the last call's dominant kernel (the scan)
+ * and of the whole call, measured with HIP events on the context's stream;
+ * also the number of scan launches and Montgomery products of the call. */
+int rbg_last_timing(const rbg_ctx *ctx, double *scan_ms, double *total_ms,
+                    uint64_t *scan_launches, uint64_t *positions,
+                    uint64_t *products);
+
+/* Compute units, peak engine clock (kHz) and HBM size of the context's device. */
+int rbg_device_info(const rbg_ctx *ctx, uint32_t *cus, uint32_t *clock_khz, uint64_t *hbm_bytes);
+
+void rbg_close(rbg_ctx *ctx);
+
+/* Admissibility mask of p (bit r-2 iff r | p-1); 0 for p < 3. */
+uint32_t rbg_mask(uint32_t p);
+
+/* Deterministic primality for 32-bit inputs. */
+int rbg_is_prime(uint32_t p);
+
+/* Library version string. */
+const char *rbg_version(void);
+
+/* ---- reference bridge (baseline_bridge.cu:7,21-27) ---------------------- */
+
+/* Prepares a process-wide context sized for primes up to `cap`. */
+void baseline_init(unsigned cap);
+
+/* M by order into out[r] and prefixes into prefix[r] (arrays of 10, indexed
+ * by r, entries r in mask written).  `full` nonzero requests every
+ * admissible order of p; zero requests exactly `mask`, which must be a subset
+ * of p's admissible orders.  Returns 0, or 2 for an unsupported mask, or
+ * another nonzero status on error. */
+int baseline_profile(unsigned p, unsigned mask, unsigned *out, unsigned *prefix, int full);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RABUNG_H */

@@ -1,0 +1,467 @@
+// Dense census scan for gfx950 (the "plain method").
+//
+// For each prime p of a block and each admissible order r, the colour of x is
+// cl_r(x) = ind(x) mod r.  With m = lcm of the admissible orders, y = x^((p-1)/m)
+// is an m-th root of unity z^j and j mod r is the colour (any z of exact order
+// m gives a unit relabelling, which preserves colour equality; reference
+// paper/GPU_METHOD_EN.tex:229-244, frozen_entry.cu:15-24).  Only the half
+// interval 1..(p-1)/2 is scanned; reflection cl(p-x) = cl(x) + sigma recovers
+// the full sequence (GPU_METHOD_EN.tex:175-177; original_scan_gpu.cu:380-382).
+//
+// Work decomposition (MI355X-first, not the reference's thread-per-segment):
+//   * one wave = one segment of `chunks` x 64 consecutive positions; lane i
+//     owns position base+i of each 64-position chunk;
+//   * per chunk every lane evaluates its colour code (Montgomery power + LDS
+//     root-table search), one DPP wave shift gives the left neighbour's code,
+//     and one 64-lane ballot per admissible order yields the uint64 change
+//     mask of that order;
+//   * run bookkeeping (prefix, suffix, longest run with least start, A4
+//     windows) is done on those masks with scalar (SALU) bit operations, so
+//     it is wave-uniform and overlaps the vector work of other waves;
+//   * the waves of a block hold consecutive segments and are merged in LDS
+//     into one summary per (tile, order); tiles of all primes of the block
+//     form one flattened space, consumed either by a persistent grid through
+//     a device work counter (batched method) or by one grid per prime
+//     (plain method);
+//   * merge_primes reduces the tile summaries of each prime in order (the
+//     associative run monoid) and finalises M, L, witness, K and A4.
+#include <hip/hip_runtime.h>
+
+#include "rabung.h"
+#include "rbg_types.hpp"
+
+namespace rbg {
+
+__device__ __forceinline__ u32 f4(u32 len) { return len > 3 ? len - 3 : 0; }
+
+__device__ __forceinline__ void take(u32 len, u32 start, u32 &best, u32 &bstart) {
+    if (len > best || (len == best && start < bstart)) {
+        best = len;
+        bstart = start;
+    }
+}
+
+// Montgomery product with 32-bit halves (v_mul_lo_u32 / v_mul_hi_u32); the
+// carry of hi + mh + (lo != 0) is kept because the sum reaches 2p > 2^32 for
+// p > 2^31.
+__device__ __forceinline__ u32 mont_mul(u32 a, u32 b, u32 p, u32 ninv) {
+    u32 lo = a * b;
+    u32 hi = __umulhi(a, b);
+    u32 m = lo * ninv;
+    u32 mh = __umulhi(m, p);
+    u64 s = (u64)hi + mh + (lo != 0u);
+    return (u32)(s >= p ? s - p : s);
+}
+
+__device__ __forceinline__ u32 mont_pow(u32 base, u32 e, u32 p, u32 ninv, u32 one) {
+    if (e == 0) return one;
+    u32 y = base;
+    int top = 31 - __builtin_clz(e);
+    for (int bit = top - 1; bit >= 0; --bit) {
+        y = mont_mul(y, y, p, ninv);
+        if ((e >> bit) & 1u) y = mont_mul(y, base, p, ninv);
+    }
+    return y;
+}
+
+__device__ __forceinline__ u32 add_mod(u32 a, u32 b, u32 p) {
+    u64 s = (u64)a + b;
+    return (u32)(s >= p ? s - p : s);
+}
+
+// Lane i receives v of lane i-1; lane 0 receives `fill`.
+__device__ __forceinline__ u32 shift_up_one(u32 v, u32 fill) {
+    // DPP wave_shr:1 (row/bank masks all enabled, lane 0 keeps `fill`)
+    return (u32)__builtin_amdgcn_update_dpp((int)fill, (int)v, 0x138, 0xF, 0xF, false);
+}
+
+__device__ __forceinline__ u64 ballot64(bool v) { return __builtin_amdgcn_ballot_w64(v); }
+
+// Longest interior run of one 64-position chunk.  C: change mask, N: same
+// mask (valid positions only); f < h are the first and last change bits.
+// Interior runs start at the change bits in [f, h) and close before h.  Only
+// a run strictly longer than `best` can matter (they start after every run
+// already seen).
+__device__ __forceinline__ void interior_best(u64 N, u32 f, u32 h, u32 base, u32 &best,
+                                              u32 &bstart) {
+    if (best > 62) return;
+    u64 between = ((1ull << h) - 1ull) & ~((2ull << f) - 1ull);
+    u64 T = N & between;  // same-as-left bits of interior runs
+    u64 X;
+    u32 l;
+    if (best == 0) {
+        if (T == 0) {  // every interior run has length 1; the earliest starts at f
+            take(1, base + f, best, bstart);
+            return;
+        }
+        X = T;
+        l = 1;
+    } else {
+        // X = positions i with ones at i..i+best-1 (a run of length >= best+1)
+        u32 k = best;
+        X = T;
+        u32 s = 1;
+        while (2 * s <= k) {
+            X &= X >> s;
+            s *= 2;
+        }
+        X &= X >> (k - s);
+        if (X == 0) return;
+        l = k;
+    }
+    for (;;) {
+        u64 Y = X & (T >> l);
+        if (Y == 0) break;
+        X = Y;
+        ++l;
+    }
+    // ones at c..c+l-1, c = ctz(X): the colour run starts at c-1 with length l+1
+    u32 c = (u32)__builtin_ctzll(X);
+    take(l + 1, base + c - 1, best, bstart);
+}
+
+struct OrderState {
+    u32 prefix, cur, best, bstart;
+    u32 popen;
+};
+
+// Scan one segment [seg_start, seg_start+seg_len) with the calling wave.
+// Writes per-order summaries (positions relative to seg_start) into out[r-2].
+__device__ void wave_scan(const PrimeDesc &d, u32 seg_start, u32 seg_len, const u32 *s_val,
+                          const u16 *s_code, RunSum *out, u32 &codes_out, u32 *err) {
+    const u32 lane = threadIdx.x & 63u;
+    const u32 p = d.p, ninv = d.ninv, e = d.e, one = d.one, m = d.m, mask = d.mask;
+    OrderState st[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        st[k].prefix = 0;
+        st[k].cur = 0;
+        st[k].best = 0;
+        st[k].bstart = 0;
+        st[k].popen = 1;
+    }
+    u32 a4 = 0;
+    u64 prevN = 0;
+    u32 carry_code = 0, first_code = 0, last_code = 0;
+    // Montgomery form of this lane's first position
+    u32 xm = mont_mul(seg_start + lane, d.r2, p, ninv);
+    const u32 nchunks = (seg_len + 63u) >> 6;
+    u32 bad = 0;
+    for (u32 c = 0; c < nchunks; ++c) {
+        const u32 base = c << 6;
+        const u32 valid = min(64u, seg_len - base);
+        const u64 vmask = valid == 64u ? ~0ull : ((1ull << valid) - 1ull);
+        // colour code of x: y = x^e, then its index in the sorted root table
+        u32 y = mont_pow(xm, e, p, ninv, one);
+        u32 lo = 0, n = m;
+        while (n > 1) {
+            u32 half = n >> 1;
+            lo = (s_val[lo + half] <= y) ? lo + half : lo;
+            n -= half;
+        }
+        bad |= (s_val[lo] != y) & (lane < valid);
+        u32 code = s_code[lo];
+        u32 left = shift_up_one(code, carry_code);
+        u32 diff = code ^ left;
+        if (c == 0) first_code = __builtin_amdgcn_readfirstlane(code);
+        carry_code = __builtin_amdgcn_readlane(code, 63);
+        last_code = __builtin_amdgcn_readlane(code, valid - 1);
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const u32 r = (u32)k + 2u;
+            if (!(mask & (1u << k))) continue;
+            u64 C = ballot64((diff & plane_bits(r)) != 0u);
+            if (c == 0) C |= 1ull;
+            C &= vmask;
+            const u64 N = ~C & vmask;
+            if (r == 2) {
+                u64 W = N & ((N << 1) | (prevN >> 63)) & ((N << 2) | (prevN >> 62));
+                a4 += (u32)__builtin_popcountll(W);
+                prevN = N;
+            }
+            if (C == 0) continue;
+            OrderState &s = st[k];
+            u32 f = (u32)__builtin_ctzll(C);
+            u32 closed = base + f - s.cur;
+            if (closed) {
+                if (s.popen) {
+                    s.prefix = closed;
+                    s.popen = 0;
+                }
+                take(closed, s.cur, s.best, s.bstart);
+            }
+            s.cur = base + f;
+            u64 rest = C & (C - 1ull);
+            if (rest) {
+                u32 h = 63u - (u32)__builtin_clzll(rest);
+                if (s.popen) {  // head run closes at the second change (chunk 0)
+                    s.prefix = base + (u32)__builtin_ctzll(rest) - s.cur;
+                    s.popen = 0;
+                }
+                interior_best(N, f, h, base, s.best, s.bstart);
+                s.cur = base + h;
+            }
+        }
+        xm = add_mod(xm, d.step, p);
+    }
+    if (__builtin_amdgcn_ballot_w64(bad != 0)) atomicOr(err, 1u);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        if (!(mask & (1u << k))) continue;
+        OrderState &s = st[k];
+        u32 suffix = seg_len - s.cur;
+        if (s.popen) s.prefix = seg_len;
+        if (seg_len) take(suffix, s.cur, s.best, s.bstart);
+        if (lane == 0) {
+            RunSum o;
+            o.prefix = s.prefix;
+            o.suffix = seg_len ? suffix : 0;
+            o.best = s.best;
+            o.bstart = s.bstart;
+            o.a4 = (k == 0) ? a4 : 0;
+            out[k] = o;
+        }
+    }
+    codes_out = first_code | (last_code << 16);
+}
+
+// Ordered merge of two adjacent summaries A (left) and B (right) of one
+// order with plane bits `planes`.  Positions in A/B are absolute.
+struct Agg {
+    u32 first_x, len, fcode, lcode;
+    u32 prefix, suffix, best, bstart, a4;
+};
+
+__device__ __forceinline__ Agg merge(const Agg &A, const Agg &B, u32 planes) {
+    if (A.len == 0) return B;
+    if (B.len == 0) return A;
+    Agg C;
+    C.first_x = A.first_x;
+    C.len = A.len + B.len;
+    C.fcode = A.fcode;
+    C.lcode = B.lcode;
+    C.best = A.best;
+    C.bstart = A.bstart;
+    take(B.best, B.bstart, C.best, C.bstart);
+    bool eq = ((A.lcode ^ B.fcode) & planes) == 0u;
+    if (eq) {
+        u32 cross = A.suffix + B.prefix;
+        u32 cstart = A.first_x + A.len - A.suffix;
+        take(cross, cstart, C.best, C.bstart);
+        C.prefix = (A.prefix == A.len) ? A.len + B.prefix : A.prefix;
+        C.suffix = (B.suffix == B.len) ? B.len + A.suffix : B.suffix;
+        C.a4 = A.a4 + B.a4 + f4(cross) - f4(A.suffix) - f4(B.prefix);
+    } else {
+        C.prefix = A.prefix;
+        C.suffix = B.suffix;
+        C.a4 = A.a4 + B.a4;
+    }
+    return C;
+}
+
+constexpr int kWavesPerBlock = 4;
+
+__global__ __launch_bounds__(kWavesPerBlock * 64) void scan_tiles(
+    const PrimeDesc *__restrict__ descs, u32 nprimes, u32 tile_first, u32 tile_end, u32 chunks,
+    const u32 *__restrict__ tab_val, const u16 *__restrict__ tab_code, TileHdr *__restrict__ hdr,
+    RunSum *__restrict__ runs, u32 *__restrict__ counter, u32 *__restrict__ err, int persistent) {
+    __shared__ u32 s_val[kMaxM];
+    __shared__ u16 s_code[kMaxM];
+    __shared__ RunSum s_run[kWavesPerBlock][8];
+    __shared__ u32 s_codes[kWavesPerBlock];
+    __shared__ u32 s_first[kWavesPerBlock];
+    __shared__ u32 s_len[kWavesPerBlock];
+    __shared__ u32 s_tile, s_pi;
+    const u32 wave = threadIdx.x >> 6;
+    const u32 seg_positions = chunks * 64u;
+    u32 cur_pi = 0xFFFFFFFFu;
+    for (u32 iter = 0;; ++iter) {
+        if (threadIdx.x == 0) {
+            u32 t;
+            if (persistent)
+                t = tile_first + atomicAdd(counter, 1u);
+            else
+                t = iter == 0 ? tile_first + blockIdx.x : tile_end;
+            u32 pi = cur_pi;
+            if (t < tile_end) {
+                if (pi == 0xFFFFFFFFu || descs[pi].tile_begin > t) {
+                    u32 lo = 0, hi = nprimes;  // last prime with tile_begin <= t
+                    while (hi - lo > 1) {
+                        u32 mid = (lo + hi) >> 1;
+                        if (descs[mid].tile_begin <= t) lo = mid; else hi = mid;
+                    }
+                    pi = lo;
+                }
+                while (t >= descs[pi].tile_begin + descs[pi].ntiles) ++pi;
+            }
+            s_tile = t;
+            s_pi = pi;
+        }
+        __syncthreads();
+        const u32 tile = __builtin_amdgcn_readfirstlane(s_tile);
+        if (tile >= tile_end) break;
+        const u32 pi = __builtin_amdgcn_readfirstlane(s_pi);
+        const PrimeDesc d = descs[pi];
+        if (pi != cur_pi) {
+            for (u32 j = threadIdx.x; j < d.m; j += blockDim.x) {
+                s_val[j] = tab_val[d.tab_off + j];
+                s_code[j] = tab_code[d.tab_off + j];
+            }
+            cur_pi = pi;
+            __syncthreads();
+        }
+        const u32 lt = tile - d.tile_begin;
+        const u64 seg = (u64)lt * kWavesPerBlock + wave;
+        const u64 seg_start = 1ull + seg * seg_positions;
+        const u32 seg_len =
+            seg_start > d.nh ? 0u : (u32)min((u64)seg_positions, (u64)d.nh - seg_start + 1ull);
+        u32 codes = 0;
+        wave_scan(d, (u32)seg_start, seg_len, s_val, s_code, s_run[wave], codes, err);
+        if ((threadIdx.x & 63u) == 0) {
+            s_codes[wave] = codes;
+            s_first[wave] = (u32)seg_start;
+            s_len[wave] = seg_len;
+        }
+        __syncthreads();
+        if (threadIdx.x < 8) {
+            const u32 k = threadIdx.x, r = k + 2;
+            if (d.mask & (1u << k)) {
+                const u32 planes = plane_bits(r);
+                Agg acc;
+                acc.len = 0;
+                for (int w = 0; w < kWavesPerBlock; ++w) {
+                    Agg b;
+                    b.first_x = s_first[w];
+                    b.len = s_len[w];
+                    b.fcode = s_codes[w] & 0xFFFFu;
+                    b.lcode = s_codes[w] >> 16;
+                    b.prefix = s_run[w][k].prefix;
+                    b.suffix = s_run[w][k].suffix;
+                    b.best = s_run[w][k].best;
+                    b.bstart = s_run[w][k].bstart + s_first[w];
+                    b.a4 = s_run[w][k].a4;
+                    acc = merge(acc, b, planes);
+                }
+                const u32 kk = __builtin_popcount(d.mask & ((1u << k) - 1u));
+                RunSum o;
+                o.prefix = acc.prefix;
+                o.suffix = acc.suffix;
+                o.best = acc.best;
+                o.bstart = acc.bstart;
+                o.a4 = acc.a4;
+                runs[(u64)d.sum_off + (u64)lt * d.nr + kk] = o;
+                if (k == 0) {
+                    TileHdr h;
+                    h.first_x = acc.first_x;
+                    h.len = acc.len;
+                    h.codes = acc.fcode | (acc.lcode << 16);
+                    h.pad = 0;
+                    hdr[tile] = h;
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+constexpr int kMergeThreads = 256;
+
+// One block per prime: ordered reduction of its tile summaries per order,
+// then the reflection junction and the output contract.
+__global__ __launch_bounds__(kMergeThreads) void merge_primes(const PrimeDesc *__restrict__ descs,
+                                                              const TileHdr *__restrict__ hdr,
+                                                              const RunSum *__restrict__ runs,
+                                                              rbg_record *__restrict__ out,
+                                                              u32 *__restrict__ err) {
+    __shared__ Agg s_agg[kMergeThreads];
+    const PrimeDesc d = descs[blockIdx.x];
+    const u32 t = threadIdx.x;
+    const u32 t0 = (u32)((u64)d.ntiles * t / kMergeThreads);
+    const u32 t1 = (u32)((u64)d.ntiles * (t + 1) / kMergeThreads);
+    rbg_record *rec = &out[d.index];
+    if (t == 0) {
+        rec->p = d.p;
+        rec->mask = d.mask;
+        for (int k = 0; k < 8; ++k) {
+            rec->M[k] = 0;
+            rec->L[k] = 0;
+            rec->W[k] = 0;
+            rec->K[k] = 0;
+        }
+        rec->A4 = 0;
+        rec->status = 0;
+    }
+    u32 kk = 0;
+    for (int k = 0; k < 8; ++k) {
+        if (!(d.mask & (1u << k))) continue;
+        const u32 r = (u32)k + 2u;
+        const u32 planes = plane_bits(r);
+        Agg acc;
+        acc.len = 0;
+        for (u32 i = t0; i < t1; ++i) {
+            const TileHdr h = hdr[d.tile_begin + i];
+            const RunSum s = runs[(u64)d.sum_off + (u64)i * d.nr + kk];
+            Agg b;
+            b.first_x = h.first_x;
+            b.len = h.len;
+            b.fcode = h.codes & 0xFFFFu;
+            b.lcode = h.codes >> 16;
+            b.prefix = s.prefix;
+            b.suffix = s.suffix;
+            b.best = s.best;
+            b.bstart = s.bstart;
+            b.a4 = s.a4;
+            acc = merge(acc, b, planes);
+        }
+        s_agg[t] = acc;
+        __syncthreads();
+        for (u32 stride = 1; stride < kMergeThreads; stride <<= 1) {
+            if ((t % (2 * stride)) == 0) s_agg[t] = merge(s_agg[t], s_agg[t + stride], planes);
+            __syncthreads();
+        }
+        if (t == 0) {
+            const Agg S = s_agg[0];
+            if (S.len != d.nh || S.first_x != 1u) atomicOr(err, 2u);
+            const bool sigma0 = (((d.p - 1u) / 2u) % r) == 0u;
+            u32 M = S.best, W = S.bstart;
+            if (sigma0) take(2u * S.suffix, d.nh - S.suffix + 1u, M, W);  // central junction
+            u32 L = S.prefix;
+            if (sigma0 && S.prefix == d.nh) L = d.p - 1u;
+            const u64 bound = sigma0 ? 2ull * L + 2ull : (u64)L + 2ull;
+            rec->M[k] = M;
+            rec->L[k] = L;
+            rec->W[k] = W;
+            rec->K[k] = max((u64)M + 1ull, bound);
+            if (r == 2)
+                rec->A4 = sigma0 ? 2u * (S.a4 - f4(S.suffix)) + f4(2u * S.suffix) : 2u * S.a4;
+        }
+        __syncthreads();
+        ++kk;
+    }
+}
+
+}  // namespace rbg
+
+// ---- launch wrappers used by the host orchestration -----------------------
+
+extern "C" hipError_t rbg_launch_scan(const rbg::PrimeDesc *descs, uint32_t nprimes,
+                                      uint32_t tile_first, uint32_t tile_end, uint32_t chunks,
+                                      const uint32_t *tab_val, const uint16_t *tab_code,
+                                      rbg::TileHdr *hdr, rbg::RunSum *runs, uint32_t *counter,
+                                      uint32_t *err, int persistent, uint32_t grid,
+                                      hipStream_t stream) {
+    hipLaunchKernelGGL(rbg::scan_tiles, dim3(grid), dim3(rbg::kWavesPerBlock * 64), 0, stream,
+                       descs, nprimes, tile_first, tile_end, chunks, tab_val, tab_code, hdr, runs,
+                       counter, err, persistent);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t rbg_launch_merge(const rbg::PrimeDesc *descs, uint32_t nprimes,
+                                       const rbg::TileHdr *hdr, const rbg::RunSum *runs,
+                                       rbg_record *out, uint32_t *err, hipStream_t stream) {
+    hipLaunchKernelGGL(rbg::merge_primes, dim3(nprimes), dim3(rbg::kMergeThreads), 0, stream,
+                       descs, hdr, runs, out, err);
+    return hipGetLastError();
+}
+
+extern "C" int rbg_waves_per_block(void) { return rbg::kWavesPerBlock; }

@@ -1,0 +1,195 @@
+"""ctypes shim over librabung.so, the MI355X census-worker library.
+
+The library is the product path; this module only marshals arguments.  It
+refuses to run without the compiled HIP library (no CPU fallback).
+
+Python-side mirror of the reference's callers:
+  * Engine.profile_block(primes) -> records with M, L, W, K, A4 per order
+    (the worker's per-batch loop, src/census/worker32.cu:20-41, and the moment
+    worker's A4 values, src/census/moment_worker.cpp:35-41);
+  * baseline_profile(p, mask, full) -> (M by r, prefix by r)
+    (src/census/engine32/src/baseline_bridge.cu:21-27);
+  * codec: pack frames (scripts/atlas_codec.py layout).
+"""
+import ctypes
+import os
+from pathlib import Path
+
+from . import codec  # noqa: F401
+
+PKG_DIR = Path(__file__).resolve().parent.parent
+LIB_PATH = Path(os.environ.get("RBG_LIBRARY", PKG_DIR / "librabung.so"))
+
+METHOD_BATCHED = 0x0
+METHOD_PLAIN = 0x1
+CERTIFY = 0x10
+MAX_BLOCK = 4096
+
+STATUS = {0: "ok", 1: "invalid argument", 2: "device error", 3: "verification failed",
+          4: "capacity"}
+
+
+class Record(ctypes.Structure):
+    _fields_ = [("p", ctypes.c_uint32), ("mask", ctypes.c_uint32),
+                ("M", ctypes.c_uint32 * 8), ("L", ctypes.c_uint32 * 8),
+                ("W", ctypes.c_uint32 * 8), ("K", ctypes.c_uint64 * 8),
+                ("A4", ctypes.c_uint32), ("status", ctypes.c_uint32)]
+
+
+class Config(ctypes.Structure):
+    _fields_ = [("device", ctypes.c_int32), ("tile_chunks", ctypes.c_uint32),
+                ("blocks_per_cu", ctypes.c_uint32), ("hbm_budget", ctypes.c_uint64)]
+
+
+EXPORTS = ["rbg_open", "rbg_profile_block", "rbg_profile_block_device", "rbg_last_error",
+           "rbg_last_timing", "rbg_device_info", "rbg_close", "rbg_mask", "rbg_is_prime", "rbg_version",
+           "baseline_init", "baseline_profile"]
+
+
+class RabungError(RuntimeError):
+    def __init__(self, status, message):
+        super().__init__("%s (status %d: %s)" % (message, status, STATUS.get(status, "?")))
+        self.status = status
+
+
+_lib = None
+
+
+def load():
+    """Load the HIP library; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise RuntimeError("librabung.so is not built (run rabung-atlas32_amd/build.py); "
+                           "there is no CPU fallback")
+    lib = ctypes.CDLL(str(LIB_PATH))
+    P = ctypes.POINTER
+    lib.rbg_open.argtypes = [P(Config)]
+    lib.rbg_open.restype = ctypes.c_void_p
+    lib.rbg_profile_block.argtypes = [ctypes.c_void_p, P(ctypes.c_uint32), ctypes.c_uint32,
+                                      ctypes.c_uint32, P(Record)]
+    lib.rbg_profile_block.restype = ctypes.c_int
+    lib.rbg_profile_block_device.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32,
+                                             ctypes.c_uint32, ctypes.c_void_p]
+    lib.rbg_profile_block_device.restype = ctypes.c_int
+    lib.rbg_last_error.argtypes = [ctypes.c_void_p]
+    lib.rbg_last_error.restype = ctypes.c_char_p
+    lib.rbg_last_timing.argtypes = [ctypes.c_void_p, P(ctypes.c_double), P(ctypes.c_double),
+                                    P(ctypes.c_uint64), P(ctypes.c_uint64), P(ctypes.c_uint64)]
+    lib.rbg_last_timing.restype = ctypes.c_int
+    lib.rbg_device_info.argtypes = [ctypes.c_void_p, P(ctypes.c_uint32), P(ctypes.c_uint32),
+                                    P(ctypes.c_uint64)]
+    lib.rbg_device_info.restype = ctypes.c_int
+    lib.rbg_close.argtypes = [ctypes.c_void_p]
+    lib.rbg_close.restype = None
+    lib.rbg_mask.argtypes = [ctypes.c_uint32]
+    lib.rbg_mask.restype = ctypes.c_uint32
+    lib.rbg_is_prime.argtypes = [ctypes.c_uint32]
+    lib.rbg_is_prime.restype = ctypes.c_int
+    lib.rbg_version.argtypes = []
+    lib.rbg_version.restype = ctypes.c_char_p
+    lib.baseline_init.argtypes = [ctypes.c_uint]
+    lib.baseline_init.restype = None
+    lib.baseline_profile.argtypes = [ctypes.c_uint, ctypes.c_uint, P(ctypes.c_uint),
+                                     P(ctypes.c_uint), ctypes.c_int]
+    lib.baseline_profile.restype = ctypes.c_int
+    _lib = lib
+    return lib
+
+
+def record_to_dict(rec):
+    return {"p": rec.p, "mask": rec.mask, "M": list(rec.M), "L": list(rec.L),
+            "W": list(rec.W), "K": list(rec.K), "A4": rec.A4}
+
+
+class Engine:
+    """One library context (device, stream, workspaces)."""
+
+    def __init__(self, device=0, tile_chunks=0, blocks_per_cu=0, hbm_budget=0):
+        lib = load()
+        cfg = Config(device, tile_chunks, blocks_per_cu, hbm_budget)
+        self._ctx = lib.rbg_open(ctypes.byref(cfg))
+        if not self._ctx:
+            raise RabungError(2, "rbg_open failed (no HIP device %d?)" % device)
+        self._lib = lib
+
+    def close(self):
+        if self._ctx:
+            self._lib.rbg_close(self._ctx)
+            self._ctx = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def profile_block(self, primes, method=METHOD_BATCHED, certify=False):
+        """Profiles of up to 4096 primes; returns a list of dicts."""
+        primes = [int(p) for p in primes]
+        n = len(primes)
+        arr = (ctypes.c_uint32 * max(1, n))(*primes)
+        out = (Record * max(1, n))()
+        flags = method | (CERTIFY if certify else 0)
+        rc = self._lib.rbg_profile_block(self._ctx, arr, n, flags, out)
+        if rc:
+            raise RabungError(rc, self._lib.rbg_last_error(self._ctx).decode())
+        return [record_to_dict(out[i]) for i in range(n)]
+
+    def profile_block_device(self, d_primes_ptr, n, d_out_ptr, method=METHOD_BATCHED):
+        """Device-resident variant: pointers to n uint32 primes and n records."""
+        rc = self._lib.rbg_profile_block_device(self._ctx, ctypes.c_void_p(d_primes_ptr), n,
+                                                method, ctypes.c_void_p(d_out_ptr))
+        if rc:
+            raise RabungError(rc, self._lib.rbg_last_error(self._ctx).decode())
+
+    def device_info(self):
+        cus = ctypes.c_uint32()
+        clk = ctypes.c_uint32()
+        hbm = ctypes.c_uint64()
+        self._lib.rbg_device_info(self._ctx, ctypes.byref(cus), ctypes.byref(clk), ctypes.byref(hbm))
+        return {"cus": cus.value, "clock_khz": clk.value, "hbm_bytes": hbm.value}
+
+    def last_timing(self):
+        scan = ctypes.c_double()
+        total = ctypes.c_double()
+        launches = ctypes.c_uint64()
+        positions = ctypes.c_uint64()
+        products = ctypes.c_uint64()
+        self._lib.rbg_last_timing(self._ctx, ctypes.byref(scan), ctypes.byref(total),
+                                  ctypes.byref(launches), ctypes.byref(positions),
+                                  ctypes.byref(products))
+        return {"scan_ms": scan.value, "total_ms": total.value, "launches": launches.value,
+                "positions": positions.value, "products": products.value}
+
+
+RECORD_SIZE = ctypes.sizeof(Record)
+
+
+def mask(p):
+    return load().rbg_mask(p)
+
+
+def is_prime(p):
+    return bool(load().rbg_is_prime(p))
+
+
+def baseline_profile(p, mask_bits=0, full=True):
+    """Reference bridge: returns (status, M by r[10], prefix by r[10])."""
+    lib = load()
+    out = (ctypes.c_uint * 10)()
+    pre = (ctypes.c_uint * 10)()
+    rc = lib.baseline_profile(p, mask_bits, out, pre, 1 if full else 0)
+    return rc, list(out), list(pre)
+
+
+def rows(records):
+    """Library records -> codec rows."""
+    return codec.records_to_rows(records)

@@ -1,0 +1,37 @@
+"""Loaders for the committed golden fixtures (tests/golden/)."""
+import gzip
+import json
+from pathlib import Path
+
+from rabung_amd import codec
+
+GOLDEN = Path(__file__).resolve().parent / "golden"
+
+
+def sample_block(block):
+    """(rows, a4_values, manifest entry) of a reference sample block (0 or 49900)."""
+    man = json.loads((GOLDEN / "sample_manifest.json").read_text())
+    entry = [b for b in man["blocks"] if b["block"] == block][0]
+    rows, meta = codec.decode_scan_frame((GOLDEN / entry["scan_file"]).read_bytes())
+    values, ameta = codec.decode_a4_frame((GOLDEN / entry["a4_file"]).read_bytes())
+    return rows, values, entry
+
+
+def _expand(compact):
+    out = []
+    for p, prof in compact:
+        out.append({"p": p, "profiles": [{"r": r, "max": m, "prefix": l, "K": k, "witness": w}
+                                         for r, m, l, k, w in prof]})
+    return out
+
+
+def holdout_windows():
+    with gzip.open(GOLDEN / "holdout32.json.gz", "rt") as f:
+        data = json.load(f)
+    return [{"lo": w["lo"], "hi": w["hi"], "rows": _expand(w["rows"])} for w in data["windows"]]
+
+
+def night_rows():
+    with gzip.open(GOLDEN / "night32.json.gz", "rt") as f:
+        data = json.load(f)
+    return _expand(data["rows"])

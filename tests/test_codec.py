@@ -1,0 +1,77 @@
+"""Pack-frame codec: commitments, byte-identical re-encoding, corruption.
+
+Mirrors the reference's tests/test_integrity.py:12-23 (sample pairs decode;
+truncated, extended or bit-flipped frames raise) and the sample checks of
+scripts/verify_atlas.py:18-45 (prime/science/values hashes, A4 range).
+"""
+import hashlib
+
+import pytest
+
+import golden_data
+from rabung_amd import codec
+
+
+@pytest.mark.parametrize("block", [0, 49900])
+def test_sample_commitments(block):
+    rows, values, entry = golden_data.sample_block(block)
+    assert len(rows) == entry["primes"] == len(values)
+    primes = [r["p"] for r in rows]
+    assert codec.prime_hash(primes).hex() == entry["prime_hash"]
+    assert codec.science_hash(rows).hex() == entry["science_hash"]
+    assert hashlib.sha256(codec.encode_a4_frame(
+        primes, values, entry["lo"], entry["hi"], codec.science_hash(rows),
+        entry["zlib_level_a4"])).hexdigest() == entry["a4_sha256"]
+    for row, a4 in zip(rows, values):
+        p = row["p"]
+        assert a4 == 0 if p == 2 else 0 <= a4 <= (p - 3) // 2
+        assert all(lo <= p < hi for lo, hi in [(entry["lo"], entry["hi"])])
+
+
+@pytest.mark.parametrize("block", [0, 49900])
+def test_reencode_byte_identical(block):
+    rows, values, entry = golden_data.sample_block(block)
+    frame = codec.encode_scan_frame(rows, entry["lo"], entry["hi"], True, entry["zlib_level_scan"])
+    assert hashlib.sha256(frame).hexdigest() == entry["scan_sha256"]
+    assert frame == (golden_data.GOLDEN / entry["scan_file"]).read_bytes()
+
+
+@pytest.mark.parametrize("name,decoder", [("scan-00000.vdw", codec.decode_scan_frame),
+                                          ("quadruplets-00000.a4", codec.decode_a4_frame)])
+def test_damaged_frames_rejected(name, decoder):
+    original = (golden_data.GOLDEN / name).read_bytes()
+    for damaged in [original[:-1], original + b"0", bytes([original[0] ^ 1]) + original[1:],
+                    original[:-10] + bytes([original[-10] ^ 0x40]) + original[-9:]]:
+        with pytest.raises(ValueError):
+            decoder(damaged)
+
+
+def test_varint_canonical():
+    for v in [0, 1, 127, 128, 300, 2 ** 32 - 1, 2 ** 63, 2 ** 64 - 1]:
+        out = bytearray()
+        codec.put_varint(out, v)
+        assert codec.get_varint(bytes(out), 0) == (v, len(out))
+    with pytest.raises(ValueError):
+        codec.get_varint(b"\x80\x00", 0)  # non-canonical trailing zero
+    with pytest.raises(ValueError):
+        codec.get_varint(b"\x80", 0)  # truncated
+
+
+def test_k_formula_worked_example():
+    # p = 101, r = 2: (M, L, K, w) = (7, 1, 8, 19) (GPU_METHOD_EN.tex:300-314)
+    assert codec.threshold_k(101, 2, 7, 1) == 8
+    assert codec.admissible_mask(101) == 0b1101  # r = 2, 4, 5 (100 = 2^2 * 5^2)
+    assert codec.admissible_mask(2) == 0
+
+
+def test_chains_are_order_sensitive():
+    rows0, v0, e0 = golden_data.sample_block(0)
+    rows1, v1, e1 = golden_data.sample_block(49900)
+    z = bytes(32)
+    ph0 = codec.prime_hash([r["p"] for r in rows0])
+    sh0 = codec.science_hash(rows0)
+    ph1 = codec.prime_hash([r["p"] for r in rows1])
+    sh1 = codec.science_hash(rows1)
+    a = codec.chain_scan(codec.chain_scan(z, ph0, sh0, e0["lo"], e0["hi"]), ph1, sh1, e1["lo"], e1["hi"])
+    b = codec.chain_scan(codec.chain_scan(z, ph1, sh1, e1["lo"], e1["hi"]), ph0, sh0, e0["lo"], e0["hi"])
+    assert a != b and len(a) == 32
