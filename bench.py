@@ -47,9 +47,9 @@ def block_from(start, count):
 
 WORKLOADS = {
     "top256": ("top-of-range: the 256 largest primes below 2^32, all admissible r, "
-               "full profile (M, L, witness, K) + A4, dense scan", lambda: top_primes(256)),
+               "full profile (M, L, witness, K) + A4", lambda: top_primes(256)),
     "block2_24": ("plain block: 4096 consecutive primes from the first prime >= 2^24, all "
-                  "admissible r, full profile + A4, dense scan", lambda: block_from(1 << 24, 4096)),
+                  "admissible r, full profile + A4", lambda: block_from(1 << 24, 4096)),
 }
 
 
@@ -158,16 +158,17 @@ def main():
         for _ in range(warmup):
             eng.profile_block_device(d_primes.data_ptr(), n, d_out.data_ptr(), method)
         torch.cuda.synchronize()
-        scan_ms = []
+        timings = []
         t0 = time.perf_counter()
         for _ in range(steps):
             eng.profile_block_device(d_primes.data_ptr(), n, d_out.data_ptr(), method)
-            scan_ms.append(eng.last_timing()["scan_ms"])
+            timings.append(eng.last_timing())
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
-        return dt, scan_ms, eng.last_timing()
+        return dt, timings
 
-    dt, scan_ms, timing = run(rabung_amd.METHOD_BATCHED, args.steps, args.warmup)
+    dt, timings = run(rabung_amd.METHOD_BATCHED, args.steps, args.warmup)
+    timing = timings[-1]
     raw = d_out.cpu().numpy().tobytes()
     records = [rabung_amd.record_to_dict(Record.from_buffer_copy(raw[i * RECORD_SIZE:(i + 1) * RECORD_SIZE]))
                for i in range(n)]
@@ -175,36 +176,72 @@ def main():
     ms_step = dt * 1000.0 / args.steps
     value = profiles / (dt / args.steps)
 
+    peak_tops = info["cus"] * 64 * info["clock_khz"] * 1e3 / 1e12
+    peak_basis = "%d CUs x 64 lanes x %.0f MHz, INT32 VALU ops" % (info["cus"],
+                                                                   info["clock_khz"] / 1e3)
     plain = None
     if not args.no_plain:
-        pdt, pscan, ptiming = run(rabung_amd.METHOD_PLAIN, max(1, args.steps // 2), 1)
-        psteps = max(1, args.steps // 2)
+        psteps = max(1, args.steps // 3)
+        pdt, ptimings = run(rabung_amd.METHOD_PLAIN, psteps, 0)
         raw2 = d_out.cpu().numpy().tobytes()
         if raw2 != raw:
             raise SystemExit("bench: plain and batched methods disagree")
-        plain = {"profiles_per_s": profiles / (pdt / psteps), "ms_per_step": pdt * 1000.0 / psteps,
-                 "scan_launches_per_step": ptiming["launches"],
-                 "scan_ms_per_step": sum(pscan) / len(pscan)}
+        ops = int32_ops(primes)
+        scan_s = sum(t["dense_scan_ms"] for t in ptimings) / len(ptimings) / 1000.0
+        plain = {"method": "dense Montgomery scan, one launch per prime",
+                 "profiles_per_s": round(profiles / (pdt / psteps), 3),
+                 "ms_per_step": round(pdt * 1000.0 / psteps, 3),
+                 "scan_launches_per_step": ptimings[-1]["launches"],
+                 "roofline": {"bound": "valu", "kernel": "scan_tiles",
+                              "achieved": round(ops / scan_s / 1e12, 3),
+                              "peak": round(peak_tops, 3), "unit": "TOP/s",
+                              "frac": round(ops / scan_s / 1e12 / peak_tops, 4),
+                              "algorithmic_ops_per_step": ops,
+                              "scan_ms_per_step": round(scan_s * 1000.0, 3),
+                              "peak_basis": peak_basis}}
 
-    ops = int32_ops(primes)
-    avg_scan_s = (sum(scan_ms) / len(scan_ms)) / 1000.0
-    peak_tops = info["cus"] * 64 * info["clock_khz"] * 1e3 / 1e12
-    achieved = ops / avg_scan_s / 1e12
-    roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": round(peak_tops, 3),
-                "unit": "TOP/s", "frac": round(achieved / peak_tops, 4), "traffic": None,
-                "kernel": "scan_tiles", "algorithmic_ops_per_launch": ops,
-                "avg_launch_ms": round(avg_scan_s * 1000.0, 3),
-                "peak_basis": "%d CUs x 64 lanes x %.0f MHz, INT32 ops" % (
-                    info["cus"], info["clock_khz"] / 1e3)}
+    # dominant kernel of the batched path: the sparse anchor filter (or the
+    # dense scan when the whole workload falls below the sparse domain)
+    if timing["sparse_primes"]:
+        kern_ms = sum(t["sparse_filter_ms"] for t in timings) / len(timings)
+        windows = timing["filter_windows"]
+        ops = windows * 8
+        roofline = {"bound": "valu", "kernel": "sp_filter",
+                    "achieved": round(ops / (kern_ms / 1000.0) / 1e12, 3),
+                    "peak": round(peak_tops, 3), "unit": "TOP/s",
+                    "frac": round(ops / (kern_ms / 1000.0) / 1e12 / peak_tops, 4),
+                    "traffic": None, "avg_launch_ms": round(kern_ms, 3),
+                    "algorithmic_ops_per_launch": ops,
+                    "work": {"window_compares": windows, "words": timing["filter_words"],
+                             "completions": timing["filter_completions"],
+                             "ops_per_window_compare": 8},
+                    "stage_ms": {"prep": round(timing["sparse_prep_ms"], 3),
+                                 "seed": round(timing["sparse_seed_ms"], 3),
+                                 "filter": round(timing["sparse_filter_ms"], 3),
+                                 "dense_fallback": round(timing["dense_scan_ms"], 3),
+                                 "call_total": round(timing["total_ms"], 3)},
+                    "peak_basis": peak_basis}
+    else:
+        ops = int32_ops(primes)
+        kern_ms = sum(t["dense_scan_ms"] for t in timings) / len(timings)
+        roofline = {"bound": "valu", "kernel": "scan_tiles",
+                    "achieved": round(ops / (kern_ms / 1000.0) / 1e12, 3),
+                    "peak": round(peak_tops, 3), "unit": "TOP/s",
+                    "frac": round(ops / (kern_ms / 1000.0) / 1e12 / peak_tops, 4),
+                    "traffic": None, "avg_launch_ms": round(kern_ms, 3),
+                    "algorithmic_ops_per_launch": ops, "peak_basis": peak_basis}
     cpu = None if args.no_cpu else cpu_baseline(primes)
     line = {
         "metric": METRIC, "value": round(value, 3), "unit": "profiles/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u32",
         "data": "synthetic: deterministic prime block (%d primes %d..%d)" % (n, primes[0], primes[-1]),
-        "config": {"workload": desc, "primes": n, "profiles": profiles, "method": "batched",
-                   "positions_per_step": timing["positions"], "golden_checked": checked},
+        "config": {"workload": desc, "primes": n, "profiles": profiles,
+                   "method": "batched: sparse anchor filter, one persistent launch, dense fallback",
+                   "sparse_primes": timing["sparse_primes"], "fallbacks": timing["fallbacks"],
+                   "golden_checked": checked},
         "plain": plain,
+        "batched_vs_plain": round(value / plain["profiles_per_s"], 2) if plain else None,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -32,9 +32,13 @@ extern "C" {
 #define RBG_MAX_BLOCK 4096u /* primes per call: the pack block limit */
 
 /* flags for rbg_profile_block */
-#define RBG_METHOD_BATCHED 0x0u  /* one persistent launch over all tiles */
-#define RBG_METHOD_PLAIN 0x1u    /* one grid launch per prime */
-#define RBG_CERTIFY 0x10u        /* direct powmod check of witness and prefix */
+#define RBG_METHOD_BATCHED 0x0u /* sparse anchor filter over the whole block in one
+                                   persistent launch; dense scan for primes below
+                                   2^16 and for any prime the sparse engine refuses */
+#define RBG_METHOD_PLAIN 0x1u   /* dense scan, one grid launch per prime */
+#define RBG_METHOD_DENSE 0x2u   /* dense scan, one persistent launch over all tiles */
+#define RBG_METHOD_MASK 0xFu
+#define RBG_CERTIFY 0x10u       /* direct powmod check of witness and prefix */
 
 /* status codes */
 #define RBG_OK 0
@@ -85,17 +89,37 @@ int rbg_profile_block_device(rbg_ctx *ctx, const uint32_t *d_primes, uint32_t n,
 /* Message for the last nonzero status of this context ("" if none). */
 const char *rbg_last_error(const rbg_ctx *ctx);
 
-/* Milliseconds of device time of the last call's dominant kernel (the scan)
- * and of the whole call, measured with HIP events on the context's stream;
- * also the number of scan launches and Montgomery products of the call. */
-int rbg_last_timing(const rbg_ctx *ctx, double *scan_ms, double *total_ms,
-                    uint64_t *scan_launches, uint64_t *positions,
-                    uint64_t *products);
+/* Device-time breakdown of the last call (HIP events on the context's
+ * stream) and its work counts. */
+typedef struct rbg_timing {
+    double total_ms;         /* whole call, first copy to last copy */
+    double dense_scan_ms;    /* dense scan launch(es) */
+    double sparse_prep_ms;   /* labels, inverses, colour planes */
+    double sparse_seed_ms;   /* seed runs of 1..B */
+    double sparse_filter_ms; /* anchor filter + completion */
+    uint64_t dense_primes, sparse_primes, sparse_profiles, fallbacks;
+    uint64_t positions;      /* half-interval positions of the dense scan */
+    uint64_t products;       /* Montgomery products of the dense scan */
+    uint64_t launches;       /* kernel launches of the dominant stages */
+    uint64_t filter_windows; /* 64-bit plane windows compared by the filter */
+    uint64_t filter_words;   /* 64-candidate words the filter examined */
+    uint64_t filter_completions; /* surviving anchors extended to their runs */
+} rbg_timing;
+
+int rbg_last_timing(const rbg_ctx *ctx, rbg_timing *out);
 
 /* Compute units, peak engine clock (kHz) and HBM size of the context's device. */
 int rbg_device_info(const rbg_ctx *ctx, uint32_t *cus, uint32_t *clock_khz, uint64_t *hbm_bytes);
 
 void rbg_close(rbg_ctx *ctx);
+
+/* A4 values of n primes on the host through an elliptic point count
+ * (p >= 1000; dense count below; p = 2 gives 0).  Replaces the moment
+ * worker's per-prime loop, src/census/moment_worker.cpp:35-41 (cardinality +
+ * quadruples, src/census/theorie/elliptic4_mestre.cpp:42-67).  threads = 0
+ * picks the host's core count (at most 16).  Returns RBG_EINVAL for a
+ * non-prime and RBG_ECERTIFY if a count fails its congruence checks. */
+int rbg_a4_host(const uint32_t *primes, uint32_t n, uint32_t *out, uint32_t threads);
 
 /* Admissibility mask of p (bit r-2 iff r | p-1); 0 for p < 3. */
 uint32_t rbg_mask(uint32_t p);

@@ -2,14 +2,19 @@
 //
 // Replaces the reference's per-prime host chain (src/census/worker32.cu:20-41;
 // src/census/engine32/src/frozen_entry.cu:149-242 prepare_gpu_table /
-// gpu_prepared_group) for the dense path: per-prime constants and the sorted
-// root table are prepared on the host (character_root, frozen_entry.cu:18-24,
-// 158-167; original_scan_gpu.cu:369-376), uploaded in one copy, and the whole
-// block is scanned by one persistent launch (or one launch per prime for the
-// plain method), followed by one ordered merge launch and one copy back.
+// gpu_prepared_group).  Per-prime constants, certified covers and the sorted
+// root tables are prepared on the host (character_root, frozen_entry.cu:18-24,
+// 158-167; original_scan_gpu.cu:369-376) and uploaded in one copy each.  The
+// batched method sends every eligible prime of the block through the sparse
+// engine in one chain of launches (labels, inverses, planes, seeds, one
+// persistent filter, finalize) while the host counts A4 by elliptic curves;
+// primes below the sparse domain, or refused by it, go through the dense
+// scan (one persistent launch plus one ordered merge).  The plain method is
+// the dense scan with one launch per prime.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <mutex>
@@ -19,10 +24,13 @@
 #include <vector>
 
 #include "rabung.h"
+#include "rbg_sparse.hpp"
 #include "rbg_types.hpp"
 
 using rbg::PrimeDesc;
 using rbg::RunSum;
+using rbg::SparsePrime;
+using rbg::SparseProf;
 using rbg::TileHdr;
 using rbg::u16;
 using rbg::u32;
@@ -37,6 +45,24 @@ extern "C" hipError_t rbg_launch_merge(const PrimeDesc *descs, uint32_t nprimes,
                                        const TileHdr *hdr, const RunSum *runs, rbg_record *out,
                                        uint32_t *err, hipStream_t stream);
 extern "C" int rbg_waves_per_block(void);
+extern "C" hipError_t rbg_sparse_prepare(const SparsePrime *P, uint32_t nprimes,
+                                         const SparseProf *prof, uint32_t nprof,
+                                         uint32_t label_blocks, uint32_t inverse_blocks,
+                                         uint32_t plane_blocks, const uint32_t *tab_val,
+                                         const uint16_t *tab_lab, uint16_t *labels, uint32_t *inv,
+                                         uint32_t *planes, uint32_t *err, hipStream_t stream);
+extern "C" hipError_t rbg_sparse_seed(const SparsePrime *P, SparseProf *prof, uint32_t nprof,
+                                      const uint16_t *labels, unsigned long long *best,
+                                      uint32_t *failed, hipStream_t stream);
+extern "C" hipError_t rbg_sparse_filter(const SparsePrime *P, const SparseProf *prof,
+                                        uint32_t nprof, uint32_t dmax, const uint32_t *planes,
+                                        const uint32_t *inv, const uint32_t *tab_val,
+                                        const uint16_t *tab_lab, unsigned long long *best,
+                                        uint32_t *failed, unsigned long long *stats, uint32_t grid,
+                                        hipStream_t stream);
+extern "C" hipError_t rbg_sparse_finalize(const SparsePrime *P, uint32_t nprimes,
+                                          const SparseProf *prof, const unsigned long long *best,
+                                          const uint32_t *a4, rbg_record *out, hipStream_t stream);
 
 namespace {
 
@@ -100,6 +126,50 @@ u32 root_of_order(u32 p, u32 m) {
     return 0;
 }
 
+// Sorted Montgomery images z^j R mod p with their label j (and colour code).
+bool root_table(u32 p, u32 m, std::vector<u32> &vals, std::vector<u16> &codes,
+                std::vector<u16> *labs) {
+    const u32 z = root_of_order(p, m);
+    if (!z) return false;
+    std::vector<std::pair<u32, u32>> tab(m);
+    u32 cur = 1;
+    for (u32 j = 0; j < m; ++j) {
+        tab[j] = {(u32)(((u64)cur << 32) % p), j};
+        cur = rbg::mulmod64(cur, z, p);
+    }
+    std::sort(tab.begin(), tab.end());
+    for (auto &t : tab) {
+        vals.push_back(t.first);
+        codes.push_back((u16)rbg::colour_code(t.second, m));
+        if (labs) labs->push_back((u16)t.second);
+    }
+    return true;
+}
+
+u64 isqrt64(u64 n) {
+    u64 x = (u64)std::sqrt((double)n);
+    while (x * x > n) --x;
+    while ((x + 1) * (x + 1) <= n) ++x;
+    return x;
+}
+
+// Certified clipped cover for threshold cap Q (GPU_METHOD_EN.tex:254-326;
+// reference vdw_clipped::from_denominator): D ~ sqrt(p/H), least B with
+// 4BD - HD^2 > 4p, all checked in integers.
+bool choose_cover(u32 p, u32 Q, u32 &B, u32 &D) {
+    const u64 H = Q - 1;
+    u64 d = isqrt64(p / H);
+    while (d > 1 && H * d * d > 4ull * p) --d;
+    if (d < 1) return false;
+    const u64 b = (4ull * p + H * d * d) / (4ull * d) + 1;
+    if (b >= p || d >= p || b > (1u << 20) || H * d > 2 * b) return false;
+    if (4ull * b * d - H * d * d <= 4ull * p) return false;
+    if (2 * b + 2 >= (u64)p - 1) return false;  // table stays inside the first half
+    B = (u32)b;
+    D = (u32)d;
+    return true;
+}
+
 struct DevBuf {
     void *ptr = nullptr;
     size_t bytes = 0;
@@ -120,6 +190,11 @@ struct DevBuf {
     }
 };
 
+constexpr u32 kSparseMin = 1u << 16;  // smallest prime sent to the sparse engine
+constexpr u32 kCoverQ = 48;           // threshold cap of the cover
+
+enum Ev { kAll0, kAll1, kDense0, kDense1, kPrep0, kPrep1, kSeed0, kSeed1, kFilt0, kFilt1, kNumEv };
+
 }  // namespace
 
 struct rbg_ctx {
@@ -128,15 +203,23 @@ struct rbg_ctx {
     int clock_khz = 0;
     size_t hbm_bytes = 0;
     hipStream_t stream = nullptr;
-    hipEvent_t ev_scan0 = nullptr, ev_scan1 = nullptr, ev_all0 = nullptr, ev_all1 = nullptr;
+    hipEvent_t ev[kNumEv] = {};
     rbg_config cfg{};
     std::string error;
-    DevBuf descs, tab_val, tab_code, hdr, runs, recs, flags, primes;
+    // dense path
+    DevBuf descs, tab_val, tab_code, hdr, runs, recs, flags;
     std::vector<PrimeDesc> h_descs;
     std::vector<u32> h_val;
     std::vector<u16> h_code;
-    double scan_ms = 0, total_ms = 0;
-    u64 launches = 0, positions = 0, products = 0;
+    // sparse path
+    DevBuf sp_primes, sp_profs, sp_val, sp_lab, sp_labels, sp_inv, sp_planes, sp_best, sp_failed,
+        sp_a4, sp_stats;
+    std::vector<SparsePrime> h_sp;
+    std::vector<SparseProf> h_prof;
+    std::vector<u32> h_sval;
+    std::vector<u16> h_scode, h_slab;
+    // timing of the last call
+    rbg_timing timing{};
 
     int fail(int code, const std::string &msg) {
         error = msg;
@@ -159,6 +242,14 @@ int hip_fail(rbg_ctx *c, hipError_t e, const char *what) {
 u32 tile_chunks(const rbg_ctx *c) { return c->cfg.tile_chunks ? c->cfg.tile_chunks : 64u; }
 
 u32 blocks_per_cu(const rbg_ctx *c) { return c->cfg.blocks_per_cu ? c->cfg.blocks_per_cu : 8u; }
+
+u64 budget(const rbg_ctx *c) { return c->cfg.hbm_budget ? c->cfg.hbm_budget : (16ull << 30); }
+
+double elapsed(rbg_ctx *c, int a, int b) {
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, c->ev[a], c->ev[b]) != hipSuccess) return 0;
+    return ms;
+}
 
 // Host verification of one profile by direct powers (reference
 // certify_witness, VBB/src/advance.cpp:7-13, and the prefix loop of
@@ -203,22 +294,17 @@ int certify_all(rbg_ctx *c, const rbg_record *out, u32 n) {
     return RBG_OK;
 }
 
-// Prepare the device descriptors for the odd primes of the block; p = 2 and
-// the layout of records are handled by the caller.
-int prepare(rbg_ctx *c, const u32 *primes, u32 n, u64 *total_tiles, u64 *total_runs) {
+// ---- dense scan ----------------------------------------------------------
+
+int dense_run(rbg_ctx *c, const u32 *primes, const std::vector<u32> &idx, bool plain,
+              rbg_record *recs, u32 *err) {
     const u32 tile_pos = (u32)rbg_waves_per_block() * tile_chunks(c) * 64u;
     c->h_descs.clear();
     c->h_val.clear();
     c->h_code.clear();
-    u64 tiles = 0, runs = 0, products = 0, positions = 0;
-    for (u32 i = 0; i < n; ++i) {
+    u64 tiles = 0, runs = 0;
+    for (u32 i : idx) {
         const u32 p = primes[i];
-        if (!prime_u32(p)) {
-            char msg[96];
-            snprintf(msg, sizeof msg, "not a prime below 2^32: %u (index %u)", p, i);
-            return c->fail(RBG_EINVAL, msg);
-        }
-        if (p == 2) continue;
         PrimeDesc d{};
         d.p = p;
         d.nh = (p - 1) / 2;
@@ -242,27 +328,216 @@ int prepare(rbg_ctx *c, const u32 *primes, u32 n, u64 *total_tiles, u64 *total_r
             return c->fail(RBG_ECAPACITY, "block too large for one call");
         tiles += d.ntiles;
         runs += (u64)d.ntiles * d.nr;
-        positions += d.nh;
-        products += (u64)d.nh * (rbg::montpow_products(d.e) + 1);
-        const u32 z = root_of_order(p, d.m);
-        if (!z) return c->fail(RBG_EINVAL, "no root of the required order");
-        std::vector<std::pair<u32, u32>> tab(d.m);
-        u32 cur = 1;
-        for (u32 j = 0; j < d.m; ++j) {
-            tab[j] = {(u32)(((u64)cur << 32) % p), rbg::colour_code(j, d.m)};
-            cur = rbg::mulmod64(cur, z, p);
-        }
-        std::sort(tab.begin(), tab.end());
-        for (auto &t : tab) {
-            c->h_val.push_back(t.first);
-            c->h_code.push_back((u16)t.second);
-        }
+        c->timing.positions += d.nh;
+        c->timing.products += (u64)d.nh * rbg::montpow_products(d.e);
+        if (!root_table(p, d.m, c->h_val, c->h_code, nullptr))
+            return c->fail(RBG_EINVAL, "no root of the required order");
         c->h_descs.push_back(d);
     }
-    c->positions = positions;
-    c->products = products;
-    *total_tiles = tiles;
-    *total_runs = runs;
+    const u32 np = (u32)c->h_descs.size();
+    c->timing.dense_primes = np;
+    if (!np) return RBG_OK;
+    const u64 bytes = np * sizeof(PrimeDesc) + c->h_val.size() * 6 + tiles * sizeof(TileHdr) +
+                      runs * sizeof(RunSum);
+    if (bytes > budget(c)) return c->fail(RBG_ECAPACITY, "dense workspace exceeds the HBM budget");
+    HIPCK(c, c->descs.ensure(np * sizeof(PrimeDesc)));
+    HIPCK(c, c->tab_val.ensure(c->h_val.size() * 4));
+    HIPCK(c, c->tab_code.ensure(c->h_code.size() * 2));
+    HIPCK(c, c->hdr.ensure(tiles * sizeof(TileHdr)));
+    HIPCK(c, c->runs.ensure(runs * sizeof(RunSum)));
+    hipStream_t s = c->stream;
+    HIPCK(c, hipMemcpyAsync(c->descs.ptr, c->h_descs.data(), np * sizeof(PrimeDesc),
+                            hipMemcpyHostToDevice, s));
+    HIPCK(c, hipMemcpyAsync(c->tab_val.ptr, c->h_val.data(), c->h_val.size() * 4,
+                            hipMemcpyHostToDevice, s));
+    HIPCK(c, hipMemcpyAsync(c->tab_code.ptr, c->h_code.data(), c->h_code.size() * 2,
+                            hipMemcpyHostToDevice, s));
+    u32 *counter = (u32 *)c->flags.ptr;
+    HIPCK(c, hipMemsetAsync(counter, 0, sizeof(u32), s));
+    const PrimeDesc *dd = (const PrimeDesc *)c->descs.ptr;
+    HIPCK(c, hipEventRecord(c->ev[kDense0], s));
+    if (plain) {
+        for (u32 i = 0; i < np; ++i) {
+            const PrimeDesc &d = c->h_descs[i];
+            HIPCK(c, rbg_launch_scan(dd, np, d.tile_begin, d.tile_begin + d.ntiles, tile_chunks(c),
+                                     (const u32 *)c->tab_val.ptr, (const u16 *)c->tab_code.ptr,
+                                     (TileHdr *)c->hdr.ptr, (RunSum *)c->runs.ptr, counter, err, 0,
+                                     d.ntiles, s));
+            ++c->timing.launches;
+        }
+    } else {
+        const u32 grid = (u32)std::min<u64>((u64)c->cus * blocks_per_cu(c), tiles);
+        HIPCK(c, rbg_launch_scan(dd, np, 0, (u32)tiles, tile_chunks(c), (const u32 *)c->tab_val.ptr,
+                                 (const u16 *)c->tab_code.ptr, (TileHdr *)c->hdr.ptr,
+                                 (RunSum *)c->runs.ptr, counter, err, 1, std::max(1u, grid), s));
+        ++c->timing.launches;
+    }
+    HIPCK(c, hipEventRecord(c->ev[kDense1], s));
+    HIPCK(c, rbg_launch_merge(dd, np, (const TileHdr *)c->hdr.ptr, (const RunSum *)c->runs.ptr, recs,
+                              err, s));
+    return RBG_OK;
+}
+
+// ---- sparse engine -------------------------------------------------------
+
+// Plans the sparse primes; indices that the sparse engine cannot take are
+// appended to `rest`.
+int sparse_plan(rbg_ctx *c, const u32 *primes, const std::vector<u32> &idx,
+                std::vector<u32> &rest) {
+    c->h_sp.clear();
+    c->h_prof.clear();
+    c->h_sval.clear();
+    c->h_scode.clear();
+    c->h_slab.clear();
+    u64 lab = 0, inv = 0, words = 0, lblk = 0, iblk = 0, pblk = 0;
+    for (u32 i : idx) {
+        const u32 p = primes[i];
+        u32 B = 0, D = 0;
+        if (p < kSparseMin || !choose_cover(p, kCoverQ, B, D)) {
+            rest.push_back(i);
+            continue;
+        }
+        SparsePrime s{};
+        s.p = p;
+        s.mask = mask_of(p);
+        s.m = lcm_of(s.mask);
+        s.e = (p - 1) / s.m;
+        s.ninv = rbg::neg_inverse32(p);
+        s.one = (u32)((1ull << 32) % p);
+        s.r2 = (u32)((u64)s.one * s.one % p);
+        s.index = i;
+        s.B = B;
+        s.D = D;
+        s.Q = kCoverQ;
+        s.jneg = (s.e & 1u) ? s.m / 2 : 0;  // (-1)^e = z^(m/2) when e is odd
+        s.tab_off = (u32)c->h_sval.size();
+        s.lab_off = (u32)lab;
+        s.inv_off = (u32)inv;
+        s.prof_begin = (u32)c->h_prof.size();
+        s.lblk = (u32)lblk;
+        s.iblk = (u32)iblk;
+        s.pblk = (u32)pblk;
+        lab += B + 1ull;
+        inv += D + 1ull;
+        lblk += (B + 2047ull) / 2048ull;
+        iblk += (D + 1ull + 255ull) / 256ull;
+        if (!root_table(p, s.m, c->h_sval, c->h_scode, &c->h_slab))
+            return c->fail(RBG_EINVAL, "no root of the required order");
+        for (u32 r = 2; r <= 9; ++r)
+            if (s.mask & (1u << (r - 2))) {
+                SparseProf f{};
+                f.prime = (u32)c->h_sp.size();
+                f.r = r;
+                f.np = 32u - (u32)__builtin_clz(r - 1);
+                f.wpp = (2u * B + 1u + 160u + 31u) / 32u + 2u;
+                f.plane_off = (u32)words;
+                f.sigma0 = (((p - 1) / 2) % r) == 0;
+                f.pblk = (u32)pblk;
+                words += (u64)f.np * f.wpp;
+                pblk += ((f.wpp + 1ull) / 2ull + 63ull) / 64ull;
+                c->h_prof.push_back(f);
+                ++s.nprof;
+            }
+        c->h_sp.push_back(s);
+        if (lab > 0xFFFFFFFFull || words > 0xFFFFFFFFull || pblk > 0x7FFFFFFFull)
+            return c->fail(RBG_ECAPACITY, "sparse block too large for one call");
+    }
+    c->timing.sparse_primes = c->h_sp.size();
+    c->timing.sparse_profiles = c->h_prof.size();
+    return RBG_OK;
+}
+
+int sparse_run(rbg_ctx *c, rbg_record *recs, u32 *err, std::vector<u32> &failed_idx) {
+    const u32 np = (u32)c->h_sp.size(), nf = (u32)c->h_prof.size();
+    if (!np) return RBG_OK;
+    u64 lab = 0, inv = 0, words = 0, lblk = 0, iblk = 0, pblk = 0, dmax = 0;
+    for (const SparsePrime &s : c->h_sp) {
+        lab += s.B + 1ull;
+        inv += s.D + 1ull;
+        lblk += (s.B + 2047ull) / 2048ull;
+        iblk += (s.D + 1ull + 255ull) / 256ull;
+        dmax = std::max<u64>(dmax, s.D);
+    }
+    for (const SparseProf &f : c->h_prof) {
+        words += (u64)f.np * f.wpp;
+        pblk += ((f.wpp + 1ull) / 2ull + 63ull) / 64ull;
+    }
+    const u64 bytes = np * sizeof(SparsePrime) + nf * sizeof(SparseProf) + c->h_sval.size() * 4 +
+                      c->h_slab.size() * 2 + lab * 2 + inv * 4 + words * 4 + nf * 8 + np * 8;
+    if (bytes > budget(c)) return c->fail(RBG_ECAPACITY, "sparse workspace exceeds the HBM budget");
+    HIPCK(c, c->sp_primes.ensure(np * sizeof(SparsePrime)));
+    HIPCK(c, c->sp_profs.ensure(nf * sizeof(SparseProf)));
+    HIPCK(c, c->sp_val.ensure(c->h_sval.size() * 4));
+    HIPCK(c, c->sp_lab.ensure(c->h_slab.size() * 2));
+    HIPCK(c, c->sp_labels.ensure(lab * 2));
+    HIPCK(c, c->sp_inv.ensure(inv * 4));
+    HIPCK(c, c->sp_planes.ensure(words * 4));
+    HIPCK(c, c->sp_best.ensure(nf * 8));
+    HIPCK(c, c->sp_failed.ensure(np * 4));
+    HIPCK(c, c->sp_a4.ensure(np * 4));
+    HIPCK(c, c->sp_stats.ensure(3 * 8));
+    hipStream_t s = c->stream;
+    // A4 by elliptic point counts on the host while the device works
+    std::vector<u32> sp_p(np), a4(np, 0);
+    for (u32 i = 0; i < np; ++i) sp_p[i] = c->h_sp[i].p;
+    int a4_rc = RBG_OK;
+    std::thread a4_thread([&] { a4_rc = rbg_a4_host(sp_p.data(), np, a4.data(), 0); });
+    struct Joiner {
+        std::thread &t;
+        ~Joiner() {
+            if (t.joinable()) t.join();
+        }
+    } joiner{a4_thread};
+    HIPCK(c, hipMemcpyAsync(c->sp_primes.ptr, c->h_sp.data(), np * sizeof(SparsePrime),
+                            hipMemcpyHostToDevice, s));
+    HIPCK(c, hipMemcpyAsync(c->sp_profs.ptr, c->h_prof.data(), nf * sizeof(SparseProf),
+                            hipMemcpyHostToDevice, s));
+    HIPCK(c, hipMemcpyAsync(c->sp_val.ptr, c->h_sval.data(), c->h_sval.size() * 4,
+                            hipMemcpyHostToDevice, s));
+    HIPCK(c, hipMemcpyAsync(c->sp_lab.ptr, c->h_slab.data(), c->h_slab.size() * 2,
+                            hipMemcpyHostToDevice, s));
+    HIPCK(c, hipMemsetAsync(c->sp_failed.ptr, 0, np * 4, s));
+    HIPCK(c, hipMemsetAsync(c->sp_stats.ptr, 0, 3 * 8, s));
+    const SparsePrime *dP = (const SparsePrime *)c->sp_primes.ptr;
+    SparseProf *dF = (SparseProf *)c->sp_profs.ptr;
+    unsigned long long *best = (unsigned long long *)c->sp_best.ptr;
+    u32 *failed = (u32 *)c->sp_failed.ptr;
+    HIPCK(c, hipEventRecord(c->ev[kPrep0], s));
+    HIPCK(c, rbg_sparse_prepare(dP, np, dF, nf, (u32)lblk, (u32)iblk, (u32)pblk,
+                                (const u32 *)c->sp_val.ptr, (const u16 *)c->sp_lab.ptr,
+                                (u16 *)c->sp_labels.ptr, (u32 *)c->sp_inv.ptr,
+                                (u32 *)c->sp_planes.ptr, err, s));
+    HIPCK(c, hipEventRecord(c->ev[kPrep1], s));
+    HIPCK(c, hipEventRecord(c->ev[kSeed0], s));
+    HIPCK(c, rbg_sparse_seed(dP, dF, nf, (const u16 *)c->sp_labels.ptr, best, failed, s));
+    HIPCK(c, hipEventRecord(c->ev[kSeed1], s));
+    const u32 grid = std::max(1u, (u32)c->cus * blocks_per_cu(c));
+    HIPCK(c, hipEventRecord(c->ev[kFilt0], s));
+    HIPCK(c, rbg_sparse_filter(dP, dF, nf, (u32)dmax, (const u32 *)c->sp_planes.ptr,
+                               (const u32 *)c->sp_inv.ptr, (const u32 *)c->sp_val.ptr,
+                               (const u16 *)c->sp_lab.ptr, best, failed,
+                               (unsigned long long *)c->sp_stats.ptr, grid, s));
+    HIPCK(c, hipEventRecord(c->ev[kFilt1], s));
+    c->timing.launches += 6;
+    std::vector<u32> h_failed(np, 0);
+    u64 h_stats[3] = {0, 0, 0};
+    HIPCK(c, hipMemcpyAsync(h_failed.data(), failed, np * 4, hipMemcpyDeviceToHost, s));
+    HIPCK(c, hipMemcpyAsync(h_stats, c->sp_stats.ptr, sizeof h_stats, hipMemcpyDeviceToHost, s));
+    HIPCK(c, hipStreamSynchronize(s));
+    a4_thread.join();
+    if (a4_rc) return c->fail(a4_rc, "elliptic A4 count failed");
+    HIPCK(c, hipMemcpyAsync(c->sp_a4.ptr, a4.data(), np * 4, hipMemcpyHostToDevice, s));
+    HIPCK(c, rbg_sparse_finalize(dP, np, dF, best, (const u32 *)c->sp_a4.ptr, recs, s));
+    HIPCK(c, hipStreamSynchronize(s));  // a4 lives on this stack frame
+    for (u32 i = 0; i < np; ++i)
+        if (h_failed[i]) failed_idx.push_back(c->h_sp[i].index);
+    c->timing.fallbacks += failed_idx.size();
+    c->timing.sparse_prep_ms = elapsed(c, kPrep0, kPrep1);
+    c->timing.sparse_seed_ms = elapsed(c, kSeed0, kSeed1);
+    c->timing.sparse_filter_ms = elapsed(c, kFilt0, kFilt1);
+    c->timing.filter_windows = h_stats[0];
+    c->timing.filter_words = h_stats[1];
+    c->timing.filter_completions = h_stats[2];
     return RBG_OK;
 }
 
@@ -270,22 +545,20 @@ int run_block(rbg_ctx *c, const u32 *h_primes, u32 n, u32 flags, rbg_record *d_o
               rbg_record *h_out) {
     if (n > RBG_MAX_BLOCK) return c->fail(RBG_EINVAL, "more than 4096 primes in one block");
     c->error.clear();
-    c->scan_ms = c->total_ms = 0;
-    c->launches = 0;
-    u64 tiles = 0, runs = 0;
-    int rc = prepare(c, h_primes, n, &tiles, &runs);
-    if (rc) return rc;
-    const u32 np = (u32)c->h_descs.size();
-    const size_t bytes = np * sizeof(PrimeDesc) + c->h_val.size() * 6 + tiles * sizeof(TileHdr) +
-                         runs * sizeof(RunSum) + (size_t)n * sizeof(rbg_record);
-    const u64 budget = c->cfg.hbm_budget ? c->cfg.hbm_budget : (16ull << 30);
-    if (bytes > budget) return c->fail(RBG_ECAPACITY, "workspace exceeds the HBM budget");
+    c->timing = rbg_timing{};
+    const u32 method = flags & RBG_METHOD_MASK;
+    if (method > RBG_METHOD_DENSE) return c->fail(RBG_EINVAL, "unknown method");
+    std::vector<u32> odd, dense, sparse, failed;
+    for (u32 i = 0; i < n; ++i) {
+        const u32 p = h_primes[i];
+        if (!prime_u32(p)) {
+            char msg[96];
+            snprintf(msg, sizeof msg, "not a prime below 2^32: %u (index %u)", p, i);
+            return c->fail(RBG_EINVAL, msg);
+        }
+        if (p != 2) odd.push_back(i);
+    }
     HIPCK(c, hipSetDevice(c->device));
-    HIPCK(c, c->descs.ensure(std::max<size_t>(1, np) * sizeof(PrimeDesc)));
-    HIPCK(c, c->tab_val.ensure(std::max<size_t>(1, c->h_val.size()) * 4));
-    HIPCK(c, c->tab_code.ensure(std::max<size_t>(1, c->h_code.size()) * 2));
-    HIPCK(c, c->hdr.ensure(std::max<u64>(1, tiles) * sizeof(TileHdr)));
-    HIPCK(c, c->runs.ensure(std::max<u64>(1, runs) * sizeof(RunSum)));
     HIPCK(c, c->flags.ensure(2 * sizeof(u32)));
     rbg_record *recs = d_out;
     if (!recs) {
@@ -293,66 +566,37 @@ int run_block(rbg_ctx *c, const u32 *h_primes, u32 n, u32 flags, rbg_record *d_o
         recs = (rbg_record *)c->recs.ptr;
     }
     hipStream_t s = c->stream;
-    HIPCK(c, hipEventRecord(c->ev_all0, s));
-    if (np) {
-        HIPCK(c, hipMemcpyAsync(c->descs.ptr, c->h_descs.data(), np * sizeof(PrimeDesc),
-                                hipMemcpyHostToDevice, s));
-        HIPCK(c, hipMemcpyAsync(c->tab_val.ptr, c->h_val.data(), c->h_val.size() * 4,
-                                hipMemcpyHostToDevice, s));
-        HIPCK(c, hipMemcpyAsync(c->tab_code.ptr, c->h_code.data(), c->h_code.size() * 2,
-                                hipMemcpyHostToDevice, s));
-    }
-    HIPCK(c, hipMemsetAsync(c->flags.ptr, 0, 2 * sizeof(u32), s));
-    // p = 2 records (mask 0, A4 placeholder 0) and the rest come from the merge
-    {
-        std::vector<rbg_record> twos;
+    u32 *err = (u32 *)c->flags.ptr + 1;
+    HIPCK(c, hipEventRecord(c->ev[kAll0], s));
+    HIPCK(c, hipMemsetAsync(err, 0, sizeof(u32), s));
+    if (n) {  // p = 2 (mask 0, A4 placeholder 0) and zeroed slots for the rest
+        HIPCK(c, hipMemsetAsync(recs, 0, (size_t)n * sizeof(rbg_record), s));
         for (u32 i = 0; i < n; ++i)
             if (h_primes[i] == 2) {
-                rbg_record r{};
-                r.p = 2;
-                HIPCK(c, hipMemcpyAsync(recs + i, &r, sizeof r, hipMemcpyHostToDevice, s));
-                HIPCK(c, hipStreamSynchronize(s));
+                static const u32 two = 2;
+                HIPCK(c, hipMemcpyAsync(&recs[i].p, &two, sizeof two, hipMemcpyHostToDevice, s));
             }
     }
-    u32 *counter = (u32 *)c->flags.ptr;
-    u32 *err = counter + 1;
-    const PrimeDesc *dd = (const PrimeDesc *)c->descs.ptr;
-    HIPCK(c, hipEventRecord(c->ev_scan0, s));
-    if (np) {
-        if (flags & RBG_METHOD_PLAIN) {
-            for (u32 i = 0; i < np; ++i) {
-                const PrimeDesc &d = c->h_descs[i];
-                HIPCK(c, rbg_launch_scan(dd, np, d.tile_begin, d.tile_begin + d.ntiles,
-                                         tile_chunks(c), (const u32 *)c->tab_val.ptr,
-                                         (const u16 *)c->tab_code.ptr, (TileHdr *)c->hdr.ptr,
-                                         (RunSum *)c->runs.ptr, counter, err, 0, d.ntiles, s));
-                ++c->launches;
-            }
-        } else {
-            const u32 grid = (u32)std::min<u64>((u64)c->cus * blocks_per_cu(c), tiles);
-            HIPCK(c, rbg_launch_scan(dd, np, 0, (u32)tiles, tile_chunks(c),
-                                     (const u32 *)c->tab_val.ptr, (const u16 *)c->tab_code.ptr,
-                                     (TileHdr *)c->hdr.ptr, (RunSum *)c->runs.ptr, counter, err, 1,
-                                     std::max(1u, grid), s));
-            ++c->launches;
-        }
+    if (method == RBG_METHOD_BATCHED) {
+        int rc = sparse_plan(c, h_primes, odd, dense);
+        if (rc) return rc;
+        rc = sparse_run(c, recs, err, failed);
+        if (rc) return rc;
+        dense.insert(dense.end(), failed.begin(), failed.end());
+    } else {
+        dense = odd;
     }
-    HIPCK(c, hipEventRecord(c->ev_scan1, s));
-    if (np)
-        HIPCK(c, rbg_launch_merge(dd, np, (const TileHdr *)c->hdr.ptr, (const RunSum *)c->runs.ptr,
-                                  recs, err, s));
+    int rc = dense_run(c, h_primes, dense, method == RBG_METHOD_PLAIN, recs, err);
+    if (rc) return rc;
     u32 h_err = 0;
     HIPCK(c, hipMemcpyAsync(&h_err, err, sizeof(u32), hipMemcpyDeviceToHost, s));
     if (h_out && n)
         HIPCK(c, hipMemcpyAsync(h_out, recs, (size_t)n * sizeof(rbg_record), hipMemcpyDeviceToHost,
                                 s));
-    HIPCK(c, hipEventRecord(c->ev_all1, s));
+    HIPCK(c, hipEventRecord(c->ev[kAll1], s));
     HIPCK(c, hipStreamSynchronize(s));
-    float ms = 0;
-    HIPCK(c, hipEventElapsedTime(&ms, c->ev_scan0, c->ev_scan1));
-    c->scan_ms = ms;
-    HIPCK(c, hipEventElapsedTime(&ms, c->ev_all0, c->ev_all1));
-    c->total_ms = ms;
+    c->timing.total_ms = elapsed(c, kAll0, kAll1);
+    if (c->timing.dense_primes) c->timing.dense_scan_ms = elapsed(c, kDense0, kDense1);
     if (h_err & 1u) return c->fail(RBG_EDEVICE, "character value missing from the root table");
     if (h_err & 2u) return c->fail(RBG_EDEVICE, "incomplete coverage of the half interval");
     if ((flags & RBG_CERTIFY) && h_out) return certify_all(c, h_out, n);
@@ -372,22 +616,19 @@ rbg_ctx *rbg_open(const rbg_config *cfg) {
         delete c;
         return nullptr;
     }
-    if (hipSetDevice(c->device) != hipSuccess) {
-        delete c;
-        return nullptr;
-    }
     hipDeviceProp_t prop;
-    if (hipGetDeviceProperties(&prop, c->device) != hipSuccess) {
+    if (hipSetDevice(c->device) != hipSuccess ||
+        hipGetDeviceProperties(&prop, c->device) != hipSuccess) {
         delete c;
         return nullptr;
     }
     c->cus = prop.multiProcessorCount;
     c->clock_khz = prop.clockRate;
     c->hbm_bytes = prop.totalGlobalMem;
-    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreate(&c->ev_scan0) != hipSuccess || hipEventCreate(&c->ev_scan1) != hipSuccess ||
-        hipEventCreate(&c->ev_all0) != hipSuccess || hipEventCreate(&c->ev_all1) != hipSuccess) {
-        delete c;
+    bool ok = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) == hipSuccess;
+    for (int i = 0; i < kNumEv && ok; ++i) ok = hipEventCreate(&c->ev[i]) == hipSuccess;
+    if (!ok) {
+        rbg_close(c);
         return nullptr;
     }
     return c;
@@ -412,14 +653,9 @@ int rbg_profile_block_device(rbg_ctx *c, const uint32_t *d_primes, uint32_t n, u
 
 const char *rbg_last_error(const rbg_ctx *c) { return c ? c->error.c_str() : "no context"; }
 
-int rbg_last_timing(const rbg_ctx *c, double *scan_ms, double *total_ms, uint64_t *launches,
-                    uint64_t *positions, uint64_t *products) {
-    if (!c) return RBG_EINVAL;
-    if (scan_ms) *scan_ms = c->scan_ms;
-    if (total_ms) *total_ms = c->total_ms;
-    if (launches) *launches = c->launches;
-    if (positions) *positions = c->positions;
-    if (products) *products = c->products;
+int rbg_last_timing(const rbg_ctx *c, rbg_timing *out) {
+    if (!c || !out) return RBG_EINVAL;
+    *out = c->timing;
     return RBG_OK;
 }
 
@@ -435,12 +671,12 @@ void rbg_close(rbg_ctx *c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
     for (DevBuf *b : {&c->descs, &c->tab_val, &c->tab_code, &c->hdr, &c->runs, &c->recs, &c->flags,
-                      &c->primes})
+                      &c->sp_primes, &c->sp_profs, &c->sp_val, &c->sp_lab, &c->sp_labels,
+                      &c->sp_inv, &c->sp_planes, &c->sp_best, &c->sp_failed, &c->sp_a4,
+                      &c->sp_stats})
         b->release();
-    if (c->ev_scan0) (void)hipEventDestroy(c->ev_scan0);
-    if (c->ev_scan1) (void)hipEventDestroy(c->ev_scan1);
-    if (c->ev_all0) (void)hipEventDestroy(c->ev_all0);
-    if (c->ev_all1) (void)hipEventDestroy(c->ev_all1);
+    for (int i = 0; i < kNumEv; ++i)
+        if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -449,7 +685,7 @@ uint32_t rbg_mask(uint32_t p) { return mask_of(p); }
 
 int rbg_is_prime(uint32_t p) { return prime_u32(p) ? 1 : 0; }
 
-const char *rbg_version(void) { return "rabung-mi355x 0.1 (gfx950 dense scan)"; }
+const char *rbg_version(void) { return "rabung-mi355x 0.2 (gfx950 dense scan + sparse anchor filter)"; }
 
 // ---- reference bridge --------------------------------------------------
 

@@ -1,0 +1,120 @@
+// Wave-level run bookkeeping on 64-position change masks (device code).
+//
+// A wave holds 64 consecutive positions; one ballot gives the change mask C
+// (bit i set when position i starts a new run; bit 0 of the first chunk of a
+// stretch is forced).  The scalar state below follows the open run across
+// chunks and records the prefix run, the longest closed run with its least
+// start, and (at the end) the suffix run.  Every value is wave-uniform, so the
+// compiler keeps it in SGPRs and the work runs on the scalar unit.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "rbg_types.hpp"
+
+namespace rbg {
+
+__device__ __forceinline__ void take(u32 len, u32 start, u32 &best, u32 &bstart) {
+    if (len > best || (len == best && start < bstart)) {
+        best = len;
+        bstart = start;
+    }
+}
+
+__device__ __forceinline__ u64 ballot64(bool v) { return __builtin_amdgcn_ballot_w64(v); }
+
+// Lane i receives v of lane i-1; lane 0 receives `fill` (DPP wave_shr:1).
+__device__ __forceinline__ u32 shift_up_one(u32 v, u32 fill) {
+    return (u32)__builtin_amdgcn_update_dpp((int)fill, (int)v, 0x138, 0xF, 0xF, false);
+}
+
+// Longest interior run of one 64-position chunk.  N: same-as-left mask of the
+// valid positions; f < h are the first and last change bits.  Interior runs
+// start at the change bits in [f, h) and close before h; they start after
+// every run already seen, so only a run strictly longer than `best` matters.
+__device__ __forceinline__ void interior_best(u64 N, u32 f, u32 h, u32 base, u32 &best,
+                                              u32 &bstart) {
+    if (best > 62) return;
+    const u64 between = ((1ull << h) - 1ull) & ~((2ull << f) - 1ull);
+    const u64 T = N & between;
+    u64 X;
+    u32 l;
+    if (best == 0) {
+        if (T == 0) {  // every interior run has length 1; the earliest starts at f
+            take(1, base + f, best, bstart);
+            return;
+        }
+        X = T;
+        l = 1;
+    } else {
+        // X: positions i with ones at i..i+best-1 (a run of length >= best+1)
+        const u32 k = best;
+        X = T;
+        u32 s = 1;
+        while (2 * s <= k) {
+            X &= X >> s;
+            s *= 2;
+        }
+        X &= X >> (k - s);
+        if (X == 0) return;
+        l = k;
+    }
+    for (;;) {
+        const u64 Y = X & (T >> l);
+        if (Y == 0) break;
+        X = Y;
+        ++l;
+    }
+    // ones at c..c+l-1 with c = ctz(X): the colour run starts at c-1, length l+1
+    const u32 c = (u32)__builtin_ctzll(X);
+    take(l + 1, base + c - 1, best, bstart);
+}
+
+struct OrderState {
+    u32 prefix, cur, best, bstart;
+    u32 popen;  // the run that began at relative position 0 is still open
+};
+
+__device__ __forceinline__ void order_init(OrderState &s) {
+    s.prefix = 0;
+    s.cur = 0;
+    s.best = 0;
+    s.bstart = 0;
+    s.popen = 1;
+}
+
+// Advance the state over one chunk at relative offset `base` with change mask
+// C and same mask N (both restricted to the valid positions).
+__device__ __forceinline__ void order_update(OrderState &s, u64 C, u64 N, u32 base) {
+    if (C == 0) return;
+    const u32 f = (u32)__builtin_ctzll(C);
+    const u32 closed = base + f - s.cur;
+    if (closed) {
+        if (s.popen) {
+            s.prefix = closed;
+            s.popen = 0;
+        }
+        take(closed, s.cur, s.best, s.bstart);
+    }
+    s.cur = base + f;
+    const u64 rest = C & (C - 1ull);
+    if (rest) {
+        const u32 h = 63u - (u32)__builtin_clzll(rest);
+        if (s.popen) {  // the head run closes at the second change (first chunk)
+            s.prefix = base + (u32)__builtin_ctzll(rest) - s.cur;
+            s.popen = 0;
+        }
+        interior_best(N, f, h, base, s.best, s.bstart);
+        s.cur = base + h;
+    }
+}
+
+// Close the stretch of `len` positions: the suffix run (partial) counts for
+// `best`; a head run never closed makes the whole stretch the prefix.
+__device__ __forceinline__ u32 order_finish(OrderState &s, u32 len) {
+    const u32 suffix = len - s.cur;
+    if (s.popen) s.prefix = len;
+    if (len) take(suffix, s.cur, s.best, s.bstart);
+    return len ? suffix : 0u;
+}
+
+}  // namespace rbg

@@ -28,102 +28,13 @@
 #include <hip/hip_runtime.h>
 
 #include "rabung.h"
+#include "rbg_device.hpp"
+#include "rbg_runs.hpp"
 #include "rbg_types.hpp"
 
 namespace rbg {
 
 __device__ __forceinline__ u32 f4(u32 len) { return len > 3 ? len - 3 : 0; }
-
-__device__ __forceinline__ void take(u32 len, u32 start, u32 &best, u32 &bstart) {
-    if (len > best || (len == best && start < bstart)) {
-        best = len;
-        bstart = start;
-    }
-}
-
-// Montgomery product with 32-bit halves (v_mul_lo_u32 / v_mul_hi_u32); the
-// carry of hi + mh + (lo != 0) is kept because the sum reaches 2p > 2^32 for
-// p > 2^31.
-__device__ __forceinline__ u32 mont_mul(u32 a, u32 b, u32 p, u32 ninv) {
-    u32 lo = a * b;
-    u32 hi = __umulhi(a, b);
-    u32 m = lo * ninv;
-    u32 mh = __umulhi(m, p);
-    u64 s = (u64)hi + mh + (lo != 0u);
-    return (u32)(s >= p ? s - p : s);
-}
-
-__device__ __forceinline__ u32 mont_pow(u32 base, u32 e, u32 p, u32 ninv, u32 one) {
-    if (e == 0) return one;
-    u32 y = base;
-    int top = 31 - __builtin_clz(e);
-    for (int bit = top - 1; bit >= 0; --bit) {
-        y = mont_mul(y, y, p, ninv);
-        if ((e >> bit) & 1u) y = mont_mul(y, base, p, ninv);
-    }
-    return y;
-}
-
-__device__ __forceinline__ u32 add_mod(u32 a, u32 b, u32 p) {
-    u64 s = (u64)a + b;
-    return (u32)(s >= p ? s - p : s);
-}
-
-// Lane i receives v of lane i-1; lane 0 receives `fill`.
-__device__ __forceinline__ u32 shift_up_one(u32 v, u32 fill) {
-    // DPP wave_shr:1 (row/bank masks all enabled, lane 0 keeps `fill`)
-    return (u32)__builtin_amdgcn_update_dpp((int)fill, (int)v, 0x138, 0xF, 0xF, false);
-}
-
-__device__ __forceinline__ u64 ballot64(bool v) { return __builtin_amdgcn_ballot_w64(v); }
-
-// Longest interior run of one 64-position chunk.  C: change mask, N: same
-// mask (valid positions only); f < h are the first and last change bits.
-// Interior runs start at the change bits in [f, h) and close before h.  Only
-// a run strictly longer than `best` can matter (they start after every run
-// already seen).
-__device__ __forceinline__ void interior_best(u64 N, u32 f, u32 h, u32 base, u32 &best,
-                                              u32 &bstart) {
-    if (best > 62) return;
-    u64 between = ((1ull << h) - 1ull) & ~((2ull << f) - 1ull);
-    u64 T = N & between;  // same-as-left bits of interior runs
-    u64 X;
-    u32 l;
-    if (best == 0) {
-        if (T == 0) {  // every interior run has length 1; the earliest starts at f
-            take(1, base + f, best, bstart);
-            return;
-        }
-        X = T;
-        l = 1;
-    } else {
-        // X = positions i with ones at i..i+best-1 (a run of length >= best+1)
-        u32 k = best;
-        X = T;
-        u32 s = 1;
-        while (2 * s <= k) {
-            X &= X >> s;
-            s *= 2;
-        }
-        X &= X >> (k - s);
-        if (X == 0) return;
-        l = k;
-    }
-    for (;;) {
-        u64 Y = X & (T >> l);
-        if (Y == 0) break;
-        X = Y;
-        ++l;
-    }
-    // ones at c..c+l-1, c = ctz(X): the colour run starts at c-1 with length l+1
-    u32 c = (u32)__builtin_ctzll(X);
-    take(l + 1, base + c - 1, best, bstart);
-}
-
-struct OrderState {
-    u32 prefix, cur, best, bstart;
-    u32 popen;
-};
 
 // Scan one segment [seg_start, seg_start+seg_len) with the calling wave.
 // Writes per-order summaries (positions relative to seg_start) into out[r-2].
@@ -133,13 +44,7 @@ __device__ void wave_scan(const PrimeDesc &d, u32 seg_start, u32 seg_len, const 
     const u32 p = d.p, ninv = d.ninv, e = d.e, one = d.one, m = d.m, mask = d.mask;
     OrderState st[8];
 #pragma unroll
-    for (int k = 0; k < 8; ++k) {
-        st[k].prefix = 0;
-        st[k].cur = 0;
-        st[k].best = 0;
-        st[k].bstart = 0;
-        st[k].popen = 1;
-    }
+    for (int k = 0; k < 8; ++k) order_init(st[k]);
     u32 a4 = 0;
     u64 prevN = 0;
     u32 carry_code = 0, first_code = 0, last_code = 0;
@@ -152,13 +57,8 @@ __device__ void wave_scan(const PrimeDesc &d, u32 seg_start, u32 seg_len, const 
         const u32 valid = min(64u, seg_len - base);
         const u64 vmask = valid == 64u ? ~0ull : ((1ull << valid) - 1ull);
         // colour code of x: y = x^e, then its index in the sorted root table
-        u32 y = mont_pow(xm, e, p, ninv, one);
-        u32 lo = 0, n = m;
-        while (n > 1) {
-            u32 half = n >> 1;
-            lo = (s_val[lo + half] <= y) ? lo + half : lo;
-            n -= half;
-        }
+        const u32 y = mont_pow(xm, e, p, ninv, one);
+        const u32 lo = table_index(s_val, m, y);
         bad |= (s_val[lo] != y) & (lane < valid);
         u32 code = s_code[lo];
         u32 left = shift_up_one(code, carry_code);
@@ -179,28 +79,7 @@ __device__ void wave_scan(const PrimeDesc &d, u32 seg_start, u32 seg_len, const 
                 a4 += (u32)__builtin_popcountll(W);
                 prevN = N;
             }
-            if (C == 0) continue;
-            OrderState &s = st[k];
-            u32 f = (u32)__builtin_ctzll(C);
-            u32 closed = base + f - s.cur;
-            if (closed) {
-                if (s.popen) {
-                    s.prefix = closed;
-                    s.popen = 0;
-                }
-                take(closed, s.cur, s.best, s.bstart);
-            }
-            s.cur = base + f;
-            u64 rest = C & (C - 1ull);
-            if (rest) {
-                u32 h = 63u - (u32)__builtin_clzll(rest);
-                if (s.popen) {  // head run closes at the second change (chunk 0)
-                    s.prefix = base + (u32)__builtin_ctzll(rest) - s.cur;
-                    s.popen = 0;
-                }
-                interior_best(N, f, h, base, s.best, s.bstart);
-                s.cur = base + h;
-            }
+            order_update(st[k], C, N, base);
         }
         xm = add_mod(xm, d.step, p);
     }
@@ -209,13 +88,11 @@ __device__ void wave_scan(const PrimeDesc &d, u32 seg_start, u32 seg_len, const 
     for (int k = 0; k < 8; ++k) {
         if (!(mask & (1u << k))) continue;
         OrderState &s = st[k];
-        u32 suffix = seg_len - s.cur;
-        if (s.popen) s.prefix = seg_len;
-        if (seg_len) take(suffix, s.cur, s.best, s.bstart);
+        const u32 suffix = order_finish(s, seg_len);
         if (lane == 0) {
             RunSum o;
             o.prefix = s.prefix;
-            o.suffix = seg_len ? suffix : 0;
+            o.suffix = suffix;
             o.best = s.best;
             o.bstart = s.bstart;
             o.a4 = (k == 0) ? a4 : 0;

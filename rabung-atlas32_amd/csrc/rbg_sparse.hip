@@ -1,0 +1,449 @@
+// Certified rational-anchor sparse search for gfx950.
+//
+// Reference: paper/GPU_METHOD_EN.tex:246-568 (integer cover certificate,
+// reflection-compatible anchor schedule, conservative rejection, exact
+// completion) and its CUDA realisation src/census/engine32/src/*.  This is an
+// independent design for wave64:
+//
+//   * cover: B, D certified with exact integers (4BD - HD^2 > 4p, HD <= 2B);
+//     every window of Q positions has numerators a + jd, |a + jd| <= B, d <= D
+//     (reference clipped_cover.hpp:7-21, GPU_METHOD_EN.tex:254-300);
+//   * table: labels j(y) of y = 1..B (one Montgomery power each) and, per
+//     order r, colour bit planes over the signed range [-B, B] with
+//     cl(-y) = cl(y) + cl(-1);
+//   * seed: one wave scans 1..B in order with the ballot run machinery; runs
+//     there are real runs and give the first threshold;
+//   * filter: one wave per (order, denominator d) task.  The scheduled anchors
+//     b (b = n d + c, n = phi or t-2-phi mod s by the side of c, folded to
+//     b >= (t-1)d/2) form bands of consecutive b; each lane tests 64
+//     consecutive b of a band at once: for every plane, the 64-bit windows at
+//     numerators b - j d (j = 1..t-1) are compared with the window at b, and a
+//     lane stops as soon as no candidate is left;
+//   * completion: a surviving anchor is mapped to x = -b / d mod p and
+//     extended left and right through the table (numerators stay in [-B, B]
+//     for the short runs that occur) and, past the table, by direct powers;
+//     the run (length, least reflected start) is folded into a packed 64-bit
+//     key with one atomicMax;
+//   * thresholds are read per task from the running best: any q with
+//     3 <= q <= min(best, Q) preserves every run of maximal length, so no
+//     epoch barrier is needed.
+#include <hip/hip_runtime.h>
+
+#include "rabung.h"
+#include "rbg_device.hpp"
+#include "rbg_runs.hpp"
+#include "rbg_sparse.hpp"
+#include "rbg_types.hpp"
+
+namespace rbg {
+
+__device__ __forceinline__ u32 find_prime(const SparsePrime *P, u32 n, u32 blk, int which) {
+    u32 lo = 0, hi = n;
+    while (hi - lo > 1) {
+        const u32 mid = (lo + hi) >> 1;
+        const u32 b = which == 0 ? P[mid].lblk : which == 1 ? P[mid].iblk : P[mid].pblk;
+        if (b <= blk) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+constexpr u32 kLabelsPerThread = 8;
+
+// labels[lab_off + y] = j(y) for y in [1, B]; slot 0 holds 0.
+__global__ __launch_bounds__(256) void sp_labels(const SparsePrime *__restrict__ P, u32 nprimes,
+                                                 const u32 *__restrict__ tab_val,
+                                                 const u16 *__restrict__ tab_lab,
+                                                 u16 *__restrict__ labels, u32 *__restrict__ err) {
+    __shared__ u32 s_val[kMaxM];
+    __shared__ u16 s_lab[kMaxM];
+    const u32 pi = __builtin_amdgcn_readfirstlane(find_prime(P, nprimes, blockIdx.x, 0));
+    const SparsePrime d = P[pi];
+    for (u32 j = threadIdx.x; j < d.m; j += blockDim.x) {
+        s_val[j] = tab_val[d.tab_off + j];
+        s_lab[j] = tab_lab[d.tab_off + j];
+    }
+    __syncthreads();
+    const u32 y0 = 1u + (blockIdx.x - d.lblk) * (256u * kLabelsPerThread);
+    if (blockIdx.x == d.lblk && threadIdx.x == 0) labels[d.lab_off] = 0;
+    u32 bad = 0;
+    for (u32 k = 0; k < kLabelsPerThread; ++k) {
+        const u32 y = y0 + k * 256u + threadIdx.x;
+        if (y > d.B) break;
+        const u32 v = mont_pow(mont_mul(y, d.r2, d.p, d.ninv), d.e, d.p, d.ninv, d.one);
+        const u32 idx = table_index(s_val, d.m, v);
+        bad |= s_val[idx] != v;
+        labels[(u64)d.lab_off + y] = s_lab[idx];
+    }
+    if (bad) atomicOr(err, 1u);
+}
+
+// inv[inv_off + d] = d^{-1} mod p for d in [1, D] (d^(p-2) in Montgomery form).
+__global__ __launch_bounds__(256) void sp_inverses(const SparsePrime *__restrict__ P, u32 nprimes,
+                                                   u32 *__restrict__ inv) {
+    const u32 pi = __builtin_amdgcn_readfirstlane(find_prime(P, nprimes, blockIdx.x, 1));
+    const SparsePrime s = P[pi];
+    const u32 d = (blockIdx.x - s.iblk) * 256u + threadIdx.x;
+    if (d == 0 || d > s.D) return;
+    const u32 dm = mont_mul(d, s.r2, s.p, s.ninv);
+    const u32 im = mont_pow(dm, s.p - 2, s.p, s.ninv, s.one);
+    inv[(u64)s.inv_off + d] = mont_mul(im, 1u, s.p, s.ninv);
+}
+
+constexpr u32 kPlanePairsPerWave = 16;
+
+// Colour bit planes of one order over the signed table: bit i of plane k is
+// bit k of cl(i - B).  One wave builds 64 bits (two words) per step.
+__global__ __launch_bounds__(256) void sp_planes(const SparsePrime *__restrict__ P, u32 nprimes,
+                                                 const SparseProf *__restrict__ prof, u32 nprof,
+                                                 const u16 *__restrict__ labels,
+                                                 u32 *__restrict__ planes) {
+    // block -> profile through the per-profile block offsets kept in pblk of
+    // the profile records (searched linearly from the prime's first profile)
+    const u32 pi = __builtin_amdgcn_readfirstlane(find_prime(P, nprimes, blockIdx.x, 2));
+    const SparsePrime s = P[pi];
+    u32 k = s.prof_begin;
+    while (k + 1 < s.prof_begin + s.nprof && prof[k + 1].pblk <= blockIdx.x) ++k;
+    const SparseProf f = prof[k];
+    const u32 lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    const u32 pairs = (f.wpp + 1) / 2;
+    const u32 first = ((blockIdx.x - f.pblk) * 4u + wave) * kPlanePairsPerWave;
+    for (u32 q = 0; q < kPlanePairsPerWave; ++q) {
+        const u32 pair = first + q;
+        if (pair >= pairs) break;
+        const int64_t y = (int64_t)pair * 64 + lane - (int64_t)s.B;
+        u32 c = 0;
+        if (y > 0 && y <= (int64_t)s.B) {
+            c = labels[(u64)s.lab_off + (u64)y] % f.r;
+        } else if (y < 0 && -y <= (int64_t)s.B) {
+            c = ((u32)labels[(u64)s.lab_off + (u64)(-y)] + s.jneg) % s.m % f.r;
+        }
+        for (u32 b = 0; b < f.np; ++b) {
+            const u64 bits = ballot64((c >> b) & 1u);
+            if (lane == 0) {
+                u32 *pl = planes + f.plane_off + (u64)b * f.wpp;
+                pl[2 * pair] = (u32)bits;
+                if (2 * pair + 1 < f.wpp) pl[2 * pair + 1] = (u32)(bits >> 32);
+            }
+        }
+    }
+}
+
+// One wave per profile: runs of 1..B in order (real runs of the sequence).
+__global__ __launch_bounds__(64) void sp_seed(const SparsePrime *__restrict__ P,
+                                              SparseProf *__restrict__ prof, u32 nprof,
+                                              const u16 *__restrict__ labels,
+                                              unsigned long long *__restrict__ best,
+                                              u32 *__restrict__ failed) {
+    const u32 k = blockIdx.x;
+    if (k >= nprof) return;
+    const SparseProf f = prof[k];
+    const SparsePrime s = P[f.prime];
+    const u32 lane = threadIdx.x;
+    OrderState st;
+    order_init(st);
+    u32 carry = 0;
+    const u32 nchunks = (s.B + 63u) >> 6;
+    for (u32 c = 0; c < nchunks; ++c) {
+        const u32 base = c << 6;
+        const u32 valid = min(64u, s.B - base);
+        const u64 vmask = valid == 64u ? ~0ull : ((1ull << valid) - 1ull);
+        const u32 y = 1u + base + lane;
+        const u32 col = y <= s.B ? labels[(u64)s.lab_off + y] % f.r : 0u;
+        const u32 left = shift_up_one(col, carry);
+        carry = __builtin_amdgcn_readlane(col, 63);
+        u64 C = ballot64(col != left);
+        if (c == 0) C |= 1ull;
+        C &= vmask;
+        order_update(st, C, ~C & vmask, base);
+    }
+    order_finish(st, s.B);
+    if (lane == 0) {
+        // positions are relative to y = 1; runs inside 1..B < p/2 have
+        // reflected start = start
+        const u32 w = st.bstart + 1u;
+        best[k] = ((unsigned long long)st.best << 32) | (0xFFFFFFFFu - w);
+        prof[k].prefix = st.popen ? s.B : st.prefix;
+        if (st.popen || st.best < 3u) atomicOr(&failed[f.prime], 1u);
+    }
+}
+
+__device__ __forceinline__ u32 colour_at(const u32 *pl, u32 wpp, u32 np, int64_t a, u32 B) {
+    const u32 i = (u32)(a + (int64_t)B);
+    u32 c = 0;
+    for (u32 b = 0; b < np; ++b) c |= ((pl[(u64)b * wpp + (i >> 5)] >> (i & 31u)) & 1u) << b;
+    return c;
+}
+
+// Absolute colour of position z by a direct power (past the table).
+__device__ u32 colour_direct(const SparsePrime &s, u32 r, const u32 *tab_val,
+                             const u16 *tab_lab, u32 z) {
+    const u32 v = mont_pow(mont_mul(z, s.r2, s.p, s.ninv), s.e, s.p, s.ninv, s.one);
+    u32 lo = 0, n = s.m;
+    const u32 *vals = tab_val + s.tab_off;
+    while (n > 1) {
+        const u32 half = n >> 1;
+        lo = (vals[lo + half] <= v) ? lo + half : lo;
+        n -= half;
+    }
+    return tab_lab[s.tab_off + lo] % r;
+}
+
+constexpr u32 kExtendLimit = 1u << 22;
+
+// Extend the monochromatic window [x, x+t-1] (numerators -b + i d) to its run.
+__device__ void complete_anchor(const SparsePrime &s, const SparseProf &f, const u32 *pl, u32 b,
+                                u32 d, u32 dinv, u32 t, const u32 *tab_val, const u16 *tab_lab,
+                                unsigned long long *best_slot, u32 *failed) {
+    const u32 p = s.p;
+    const u32 bd = (u32)((u64)b * dinv % p);
+    const u32 x = p - bd;  // bd != 0 because 0 < b <= B < p
+    const int64_t B = s.B;
+    const u32 c0 = colour_at(pl, f.wpp, f.np, -(int64_t)b, s.B);
+    u32 abs0 = 0xFFFFFFFFu;
+    u32 steps = 0;
+    // left
+    u32 lo = x;
+    int64_t a = -(int64_t)b - d;
+    bool table = true;
+    while (lo > 1) {
+        bool same;
+        if (table && a >= -B) {
+            same = colour_at(pl, f.wpp, f.np, a, s.B) == c0;
+        } else {
+            table = false;
+            if (abs0 == 0xFFFFFFFFu) abs0 = colour_direct(s, f.r, tab_val, tab_lab, x);
+            same = colour_direct(s, f.r, tab_val, tab_lab, lo - 1) == abs0;
+        }
+        if (!same || ++steps > kExtendLimit) break;
+        --lo;
+        a -= d;
+    }
+    // right
+    u32 hi = x + t - 1u;
+    a = -(int64_t)b + (int64_t)t * d;
+    table = true;
+    while (hi < p - 1u) {
+        bool same;
+        if (table && a <= B) {
+            same = colour_at(pl, f.wpp, f.np, a, s.B) == c0;
+        } else {
+            table = false;
+            if (abs0 == 0xFFFFFFFFu) abs0 = colour_direct(s, f.r, tab_val, tab_lab, x);
+            same = colour_direct(s, f.r, tab_val, tab_lab, hi + 1u) == abs0;
+        }
+        if (!same || ++steps > kExtendLimit) break;
+        ++hi;
+        a += d;
+    }
+    if (steps > kExtendLimit) {
+        atomicOr(&failed[f.prime], 2u);
+        return;
+    }
+    const u32 len = hi - lo + 1u;
+    const u32 w = lo < p - hi ? lo : p - hi;
+    atomicMax(best_slot, ((unsigned long long)len << 32) | (0xFFFFFFFFu - w));
+}
+
+// 64-bit window of plane `pl` starting at signed table index i (i >= 0).
+__device__ __forceinline__ u64 window64(const u32 *pl, u32 i) {
+    const u32 w = i >> 5, sh = i & 31u;
+    const u32 a = pl[w], b = pl[w + 1], c = pl[w + 2];
+    const u32 lo = __builtin_amdgcn_alignbit(b, a, sh);
+    const u32 hi = __builtin_amdgcn_alignbit(c, b, sh);
+    return ((u64)hi << 32) | lo;
+}
+
+constexpr u64 kThirds = 0x9249249249249249ull;  // bits i = 0 mod 3
+
+__global__ __launch_bounds__(256) void sp_filter(const SparsePrime *__restrict__ P,
+                                                 const SparseProf *__restrict__ prof, u32 nprof,
+                                                 u32 dmax, const u32 *__restrict__ planes,
+                                                 const u32 *__restrict__ inv,
+                                                 const u32 *__restrict__ tab_val,
+                                                 const u16 *__restrict__ tab_lab,
+                                                 unsigned long long *__restrict__ best,
+                                                 u32 *__restrict__ failed,
+                                                 unsigned long long *__restrict__ stats) {
+    // per-lane work counts: 64-bit window comparisons, candidate words,
+    // completed anchors (summed per block, three atomics per block)
+    __shared__ unsigned long long s_stats[3];
+    if (threadIdx.x < 3) s_stats[threadIdx.x] = 0;
+    __syncthreads();
+    u64 n_windows = 0, n_words = 0, n_done = 0;
+    const u32 lane = threadIdx.x & 63u;
+    const u32 waves = gridDim.x * (blockDim.x >> 6);
+    const u32 wid = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const u64 items = (u64)dmax * nprof;
+    for (u64 item = wid; item < items; item += waves) {
+        const u32 d = 1u + (u32)(item / nprof);
+        const u32 k = __builtin_amdgcn_readfirstlane((u32)(item % nprof));
+        const SparseProf f = prof[k];
+        const SparsePrime s = P[f.prime];
+        if (d > s.D) continue;
+        const u64 key = __hip_atomic_load(&best[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const u32 q = min((u32)(key >> 32), s.Q);
+        if (q < 3u) continue;
+        u32 t = max(3u, (q + 1u) / 2u);
+        if (t > q) t = q;
+        const u32 sp = q - t + 1u;  // schedule period
+        const u32 phi = ((t - 1u) / 2u) % sp;
+        const u32 B = s.B;
+        const u32 bmin = ((t - 1u) * d + 1u) / 2u;
+        const u32 *pl = planes + f.plane_off;
+        const u32 dinv = inv[(u64)s.inv_off + d];
+        // band layout: kind 0 = c in [1, ca] (2c < d), kind 1 = c in [cb, d-1]
+        // (2c > d); for d <= 2 one unscheduled band covers [bmin', B].
+        for (u32 kind = 0; kind < 2; ++kind) {
+            u32 c_lo, c_hi, n0, nb, step;
+            if (d <= 2) {
+                if (kind) break;
+                const u32 lo_b = max(bmin, d == 1 ? t : t - 1u);
+                if (lo_b > B) break;
+                // a single band [lo_b, B]: treat as n = 0, c from lo_b
+                c_lo = lo_b;
+                c_hi = B;
+                n0 = 0;
+                nb = 1;
+                step = 0;
+            } else {
+                const u32 ca = (d - 1u) / 2u, cb = d / 2u + 1u;
+                c_lo = kind ? cb : 1u;
+                c_hi = kind ? d - 1u : ca;
+                if (c_lo > c_hi) continue;
+                const u32 res = kind ? (t - 2u - phi) % sp : phi;
+                const u32 n_lo = bmin / d, n_hi = B / d;
+                n0 = n_lo + (res + sp - n_lo % sp) % sp;
+                nb = n0 > n_hi ? 0u : (n_hi - n0) / sp + 1u;
+                step = sp;
+            }
+            const u32 width = c_hi - c_lo + 1u;
+            const u32 W = (width + 63u) >> 6;
+            const u64 total = (u64)nb * W;
+            for (u64 idx = lane; idx < ((total + 63u) & ~63ull); idx += 64u) {
+                if (idx >= total) continue;
+                const u32 band = (u32)(idx / W), w = (u32)(idx % W);
+                const u64 nd = (u64)(n0 + band * step) * d;
+                const u64 b0 = nd + c_lo + 64ull * w;
+                const u64 bend = min(nd + c_hi, (u64)B);
+                if (b0 > bend) continue;
+                const u32 nbits = (u32)min<u64>(64, bend - b0 + 1);
+                u64 valid = nbits == 64 ? ~0ull : ((1ull << nbits) - 1ull);
+                if (b0 < bmin) {
+                    const u64 below = bmin - b0;
+                    if (below >= nbits) continue;
+                    valid &= ~0ull << below;
+                }
+                if ((d & 1u) == 0) valid &= (b0 & 1u) ? 0x5555555555555555ull : 0xAAAAAAAAAAAAAAAAull;
+                if (d % 3u == 0) valid &= ~(kThirds << ((3u - (u32)(b0 % 3u)) % 3u));
+                if (!valid) continue;
+                const u32 ib = (u32)(b0 + B);
+                ++n_words;
+                u64 base[4];
+                for (u32 pb = 0; pb < f.np; ++pb) base[pb] = window64(pl + (u64)pb * f.wpp, ib);
+                u64 mism = 0;
+                for (u32 j = 1; j < t; ++j) {
+                    const u32 io = ib - j * d;
+                    for (u32 pb = 0; pb < f.np; ++pb)
+                        mism |= base[pb] ^ window64(pl + (u64)pb * f.wpp, io);
+                    n_windows += f.np;
+                    if ((valid & ~mism) == 0) break;
+                }
+                u64 surv = valid & ~mism;
+                while (surv) {
+                    const u32 bit = (u32)__builtin_ctzll(surv);
+                    surv &= surv - 1ull;
+                    complete_anchor(s, f, pl, (u32)b0 + bit, d, dinv, t, tab_val, tab_lab,
+                                    &best[k], failed);
+                    ++n_done;
+                }
+            }
+        }
+    }
+    atomicAdd(&s_stats[0], (unsigned long long)n_windows);
+    atomicAdd(&s_stats[1], (unsigned long long)n_words);
+    atomicAdd(&s_stats[2], (unsigned long long)n_done);
+    __syncthreads();
+    if (threadIdx.x < 3) atomicAdd(&stats[threadIdx.x], s_stats[threadIdx.x]);
+}
+
+// One thread per prime: the output contract from the reduced keys.
+__global__ void sp_finalize(const SparsePrime *__restrict__ P, u32 nprimes,
+                            const SparseProf *__restrict__ prof,
+                            const unsigned long long *__restrict__ best,
+                            const u32 *__restrict__ a4, rbg_record *__restrict__ out) {
+    const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nprimes) return;
+    const SparsePrime s = P[i];
+    rbg_record *rec = &out[s.index];
+    rec->p = s.p;
+    rec->mask = s.mask;
+    for (int k = 0; k < 8; ++k) {
+        rec->M[k] = 0;
+        rec->L[k] = 0;
+        rec->W[k] = 0;
+        rec->K[k] = 0;
+    }
+    for (u32 k = s.prof_begin; k < s.prof_begin + s.nprof; ++k) {
+        const SparseProf f = prof[k];
+        const u64 key = best[k];
+        const u32 M = (u32)(key >> 32);
+        const u32 W = 0xFFFFFFFFu - (u32)key;
+        const u32 L = f.prefix;
+        const u64 bound = f.sigma0 ? 2ull * L + 2ull : (u64)L + 2ull;
+        rec->M[f.r - 2] = M;
+        rec->W[f.r - 2] = W;
+        rec->L[f.r - 2] = L;
+        rec->K[f.r - 2] = max((u64)M + 1ull, bound);
+    }
+    rec->A4 = a4[i];
+    rec->status = 0;
+}
+
+}  // namespace rbg
+
+using rbg::SparsePrime;
+using rbg::SparseProf;
+
+extern "C" hipError_t rbg_sparse_prepare(const SparsePrime *P, uint32_t nprimes,
+                                         const SparseProf *prof, uint32_t nprof,
+                                         uint32_t label_blocks, uint32_t inverse_blocks,
+                                         uint32_t plane_blocks, const uint32_t *tab_val,
+                                         const uint16_t *tab_lab, uint16_t *labels,
+                                         uint32_t *inv, uint32_t *planes, uint32_t *err,
+                                         hipStream_t stream) {
+    hipLaunchKernelGGL(rbg::sp_labels, dim3(label_blocks), dim3(256), 0, stream, P, nprimes,
+                       tab_val, tab_lab, labels, err);
+    hipLaunchKernelGGL(rbg::sp_inverses, dim3(inverse_blocks), dim3(256), 0, stream, P, nprimes,
+                       inv);
+    hipLaunchKernelGGL(rbg::sp_planes, dim3(plane_blocks), dim3(256), 0, stream, P, nprimes, prof,
+                       nprof, labels, planes);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t rbg_sparse_seed(const SparsePrime *P, SparseProf *prof, uint32_t nprof,
+                                      const uint16_t *labels, unsigned long long *best,
+                                      uint32_t *failed, hipStream_t stream) {
+    hipLaunchKernelGGL(rbg::sp_seed, dim3(nprof), dim3(64), 0, stream, P, prof, nprof, labels,
+                       best, failed);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t rbg_sparse_filter(const SparsePrime *P, const SparseProf *prof,
+                                        uint32_t nprof, uint32_t dmax, const uint32_t *planes,
+                                        const uint32_t *inv, const uint32_t *tab_val,
+                                        const uint16_t *tab_lab, unsigned long long *best,
+                                        uint32_t *failed, unsigned long long *stats,
+                                        uint32_t grid, hipStream_t stream) {
+    hipLaunchKernelGGL(rbg::sp_filter, dim3(grid), dim3(256), 0, stream, P, prof, nprof, dmax,
+                       planes, inv, tab_val, tab_lab, best, failed, stats);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t rbg_sparse_finalize(const SparsePrime *P, uint32_t nprimes,
+                                          const SparseProf *prof, const unsigned long long *best,
+                                          const uint32_t *a4, rbg_record *out,
+                                          hipStream_t stream) {
+    hipLaunchKernelGGL(rbg::sp_finalize, dim3((nprimes + 255) / 256), dim3(256), 0, stream, P,
+                       nprimes, prof, best, a4, out);
+    return hipGetLastError();
+}

@@ -20,8 +20,9 @@ from . import codec  # noqa: F401
 PKG_DIR = Path(__file__).resolve().parent.parent
 LIB_PATH = Path(os.environ.get("RBG_LIBRARY", PKG_DIR / "librabung.so"))
 
-METHOD_BATCHED = 0x0
-METHOD_PLAIN = 0x1
+METHOD_BATCHED = 0x0  # sparse anchor filter, one persistent launch, dense fallback
+METHOD_PLAIN = 0x1    # dense scan, one launch per prime
+METHOD_DENSE = 0x2    # dense scan, one persistent launch
 CERTIFY = 0x10
 MAX_BLOCK = 4096
 
@@ -36,14 +37,25 @@ class Record(ctypes.Structure):
                 ("A4", ctypes.c_uint32), ("status", ctypes.c_uint32)]
 
 
+class Timing(ctypes.Structure):
+    _fields_ = [("total_ms", ctypes.c_double), ("dense_scan_ms", ctypes.c_double),
+                ("sparse_prep_ms", ctypes.c_double), ("sparse_seed_ms", ctypes.c_double),
+                ("sparse_filter_ms", ctypes.c_double), ("dense_primes", ctypes.c_uint64),
+                ("sparse_primes", ctypes.c_uint64), ("sparse_profiles", ctypes.c_uint64),
+                ("fallbacks", ctypes.c_uint64), ("positions", ctypes.c_uint64),
+                ("products", ctypes.c_uint64), ("launches", ctypes.c_uint64),
+                ("filter_windows", ctypes.c_uint64), ("filter_words", ctypes.c_uint64),
+                ("filter_completions", ctypes.c_uint64)]
+
+
 class Config(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int32), ("tile_chunks", ctypes.c_uint32),
                 ("blocks_per_cu", ctypes.c_uint32), ("hbm_budget", ctypes.c_uint64)]
 
 
 EXPORTS = ["rbg_open", "rbg_profile_block", "rbg_profile_block_device", "rbg_last_error",
-           "rbg_last_timing", "rbg_device_info", "rbg_close", "rbg_mask", "rbg_is_prime", "rbg_version",
-           "baseline_init", "baseline_profile"]
+           "rbg_last_timing", "rbg_device_info", "rbg_close", "rbg_mask", "rbg_is_prime",
+           "rbg_version", "rbg_a4_host", "baseline_init", "baseline_profile"]
 
 
 class RabungError(RuntimeError):
@@ -75,8 +87,7 @@ def load():
     lib.rbg_profile_block_device.restype = ctypes.c_int
     lib.rbg_last_error.argtypes = [ctypes.c_void_p]
     lib.rbg_last_error.restype = ctypes.c_char_p
-    lib.rbg_last_timing.argtypes = [ctypes.c_void_p, P(ctypes.c_double), P(ctypes.c_double),
-                                    P(ctypes.c_uint64), P(ctypes.c_uint64), P(ctypes.c_uint64)]
+    lib.rbg_last_timing.argtypes = [ctypes.c_void_p, P(Timing)]
     lib.rbg_last_timing.restype = ctypes.c_int
     lib.rbg_device_info.argtypes = [ctypes.c_void_p, P(ctypes.c_uint32), P(ctypes.c_uint32),
                                     P(ctypes.c_uint64)]
@@ -89,6 +100,9 @@ def load():
     lib.rbg_is_prime.restype = ctypes.c_int
     lib.rbg_version.argtypes = []
     lib.rbg_version.restype = ctypes.c_char_p
+    lib.rbg_a4_host.argtypes = [P(ctypes.c_uint32), ctypes.c_uint32, P(ctypes.c_uint32),
+                                ctypes.c_uint32]
+    lib.rbg_a4_host.restype = ctypes.c_int
     lib.baseline_init.argtypes = [ctypes.c_uint]
     lib.baseline_init.restype = None
     lib.baseline_profile.argtypes = [ctypes.c_uint, ctypes.c_uint, P(ctypes.c_uint),
@@ -158,16 +172,9 @@ class Engine:
         return {"cus": cus.value, "clock_khz": clk.value, "hbm_bytes": hbm.value}
 
     def last_timing(self):
-        scan = ctypes.c_double()
-        total = ctypes.c_double()
-        launches = ctypes.c_uint64()
-        positions = ctypes.c_uint64()
-        products = ctypes.c_uint64()
-        self._lib.rbg_last_timing(self._ctx, ctypes.byref(scan), ctypes.byref(total),
-                                  ctypes.byref(launches), ctypes.byref(positions),
-                                  ctypes.byref(products))
-        return {"scan_ms": scan.value, "total_ms": total.value, "launches": launches.value,
-                "positions": positions.value, "products": products.value}
+        t = Timing()
+        self._lib.rbg_last_timing(self._ctx, ctypes.byref(t))
+        return {name: getattr(t, name) for name, _ in Timing._fields_}
 
 
 RECORD_SIZE = ctypes.sizeof(Record)
@@ -179,6 +186,19 @@ def mask(p):
 
 def is_prime(p):
     return bool(load().rbg_is_prime(p))
+
+
+def a4_host(primes, threads=0):
+    """A4 values through the host elliptic point count (no GPU needed)."""
+    lib = load()
+    primes = [int(p) for p in primes]
+    n = len(primes)
+    arr = (ctypes.c_uint32 * max(1, n))(*primes)
+    out = (ctypes.c_uint32 * max(1, n))()
+    rc = lib.rbg_a4_host(arr, n, out, threads)
+    if rc:
+        raise RabungError(rc, "A4 point count failed")
+    return list(out[:n])
 
 
 def baseline_profile(p, mask_bits=0, full=True):

@@ -1,0 +1,152 @@
+"""Plain-integer model of the sparse anchor search (test helper, small p).
+
+Follows the same cover, fold, band schedule, wheel masks, thresholds and
+completion as the HIP kernels in rabung-atlas32_amd/csrc/rbg_sparse.hip, with
+colours read one by one instead of through 64-bit planes.  Used to check the
+algorithmic design against the oracle on the CPU.
+"""
+from math import gcd, isqrt
+
+
+def cover(p, Q=48):
+    H = Q - 1
+    d = isqrt(p // H)
+    while d > 1 and H * d * d > 4 * p:
+        d -= 1
+    if d < 1:
+        return None
+    b = (4 * p + H * d * d) // (4 * d) + 1
+    if b >= p or d >= p or b > (1 << 20) or H * d > 2 * b:
+        return None
+    if 4 * b * d - H * d * d <= 4 * p:
+        return None
+    if 2 * b + 2 >= p - 1:
+        return None
+    return b, d
+
+
+def labels_of(p, m):
+    """label j(y) = ind(y) mod m relative to some element of order m."""
+    e = (p - 1) // m
+    # element of exact order m
+    for a in range(2, p):
+        z = pow(a, e, p)
+        if all(pow(z, m // q, p) != 1 for q in (2, 3, 5, 7) if m % q == 0):
+            break
+    root = {}
+    cur = 1
+    for j in range(m):
+        root[cur] = j
+        cur = cur * z % p
+    return lambda y: root[pow(y, e, p)]
+
+
+def sparse_profile(p, r, Q=48, order="ascending"):
+    """(M, witness, L) of order r by the anchor search; None if refused."""
+    c = cover(p, Q)
+    if c is None:
+        return None
+    B, D = c
+    mask = sum(1 << (k - 2) for k in range(2, 10) if (p - 1) % k == 0)
+    m = 1
+    for k in range(2, 10):
+        if mask & (1 << (k - 2)):
+            m = m * k // gcd(m, k)
+    lab = labels_of(p, m)
+    e = (p - 1) // m
+    jneg = m // 2 if e & 1 else 0
+    tab = [0] * (B + 1)
+    for y in range(1, B + 1):
+        tab[y] = lab(y)
+
+    def col_num(a):  # colour of a signed numerator, |a| <= B, a != 0
+        return tab[a] % r if a > 0 else (tab[-a] + jneg) % m % r
+
+    def col_abs(z):
+        return lab(z) % r
+
+    # seed: runs of 1..B in order
+    best, bw, start, prefix = 0, 0, 1, None
+    for y in range(2, B + 2):
+        if y == B + 1 or tab[y] % r != tab[y - 1] % r:
+            length = y - start
+            if prefix is None:
+                prefix = length
+            if length > best:
+                best, bw = length, start
+            start = y
+    if prefix == B or best < 3:
+        return None
+    key = (best, -bw)
+
+    def complete(b, d, t):
+        dinv = pow(d, p - 2, p)
+        x = p - b * dinv % p
+        c0 = col_num(-b)
+        lo = x
+        a = -b - d
+        table = True
+        abs0 = None
+        while lo > 1:
+            if table and a >= -B:
+                same = col_num(a) == c0
+            else:
+                table = False
+                if abs0 is None:
+                    abs0 = col_abs(x)
+                same = col_abs(lo - 1) == abs0
+            if not same:
+                break
+            lo -= 1
+            a -= d
+        hi = x + t - 1
+        a = -b + t * d
+        table = True
+        while hi < p - 1:
+            if table and a <= B:
+                same = col_num(a) == c0
+            else:
+                table = False
+                if abs0 is None:
+                    abs0 = col_abs(x)
+                same = col_abs(hi + 1) == abs0
+            if not same:
+                break
+            hi += 1
+            a += d
+        return hi - lo + 1, min(lo, p - hi)
+
+    for d in range(1, D + 1):
+        q = min(key[0], Q)
+        t = min(q, max(3, (q + 1) // 2))
+        sp = q - t + 1
+        phi = ((t - 1) // 2) % sp
+        bmin = ((t - 1) * d + 1) // 2
+        ranges = []
+        if d <= 2:
+            lo_b = max(bmin, t if d == 1 else t - 1)
+            ranges.append((lo_b, B))
+        else:
+            ca, cb = (d - 1) // 2, d // 2 + 1
+            for kind in (0, 1):
+                c_lo, c_hi = (cb, d - 1) if kind else (1, ca)
+                if c_lo > c_hi:
+                    continue
+                res = (t - 2 - phi) % sp if kind else phi
+                n_lo, n_hi = bmin // d, B // d
+                n0 = n_lo + (res + sp - n_lo % sp) % sp
+                n = n0
+                while n <= n_hi:
+                    ranges.append((max(n * d + c_lo, bmin), min(n * d + c_hi, B)))
+                    n += sp
+        for lo_b, hi_b in ranges:
+            for b in range(lo_b, hi_b + 1):
+                if d % 2 == 0 and b % 2 == 0:
+                    continue
+                if d % 3 == 0 and b % 3 == 0:
+                    continue
+                c0 = col_num(b)
+                if all(col_num(b - j * d) == c0 for j in range(1, t)):
+                    length, w = complete(b, d, t)
+                    key = max(key, (length, -w))
+    return key[0], -key[1], prefix

@@ -30,7 +30,8 @@ def as_rows(records):
     return codec.records_to_rows(records)
 
 
-@pytest.mark.parametrize("method", [rabung_amd.METHOD_BATCHED, rabung_amd.METHOD_PLAIN])
+@pytest.mark.parametrize("method", [rabung_amd.METHOD_BATCHED, rabung_amd.METHOD_PLAIN,
+                                    rabung_amd.METHOD_DENSE])
 def test_sample_block0_all(engine, method):
     rows, a4, _ = golden_data.sample_block(0)
     primes = [r["p"] for r in rows]
@@ -97,9 +98,12 @@ def test_oracle_near_2_24(engine):
 def test_methods_agree_and_are_deterministic(engine):
     primes = oracle.next_primes(3_000_000_000, 16) + [4294967291, 4273768801]
     a = engine.profile_block(primes, method=rabung_amd.METHOD_BATCHED)
+    t = engine.last_timing()
+    assert t["sparse_primes"] == len(primes) and t["fallbacks"] == 0
     b = engine.profile_block(primes, method=rabung_amd.METHOD_PLAIN)
-    c = engine.profile_block(primes, method=rabung_amd.METHOD_BATCHED)
-    assert a == b == c
+    c = engine.profile_block(primes, method=rabung_amd.METHOD_DENSE)
+    d = engine.profile_block(primes, method=rabung_amd.METHOD_BATCHED)
+    assert a == b == c == d
 
 
 def test_tile_geometry_invariance():
@@ -108,7 +112,8 @@ def test_tile_geometry_invariance():
     ref = None
     for chunks, bpc in [(1, 1), (3, 2), (64, 8), (200, 4)]:
         with rabung_amd.Engine(tile_chunks=chunks, blocks_per_cu=bpc) as eng:
-            got = eng.profile_block(primes)
+            got = eng.profile_block(primes, method=rabung_amd.METHOD_DENSE)
+            assert eng.profile_block(primes) == got  # sparse grid size too
         if ref is None:
             ref = got
         assert got == ref, (chunks, bpc)
@@ -152,3 +157,29 @@ def test_baseline_bridge():
             assert out[r] == exp["M"][r - 2] and pre[r] == exp["L"][r - 2]
     rc, out, pre = rabung_amd.baseline_profile(1000003, 1 << (7 - 2), False)
     assert rc == 2  # 7 does not divide 1000002
+
+
+def test_sparse_block49900_large_subset(engine):
+    """Sparse engine on 400 seeded primes just below 2^32 (profiles and A4)."""
+    rows, a4, _ = golden_data.sample_block(49900)
+    rng = random.Random(400)
+    idx = sorted(rng.sample(range(len(rows)), 400))
+    recs = engine.profile_block([rows[i]["p"] for i in idx])
+    t = engine.last_timing()
+    assert t["sparse_primes"] == 400
+    got = as_rows(recs)
+    for j, i in enumerate(idx):
+        assert got[j] == rows[i], (rows[i]["p"], got[j], rows[i])
+        assert recs[j]["A4"] == a4[i]
+
+
+def test_sparse_all_orders_and_small_domain(engine):
+    """Primes = 1 mod 2520 (every order admissible) across the sparse domain."""
+    primes = [p for p in range(2520 * 27 + 1, 2520 * 3000, 2520 * 7) if oracle.is_prime(p)][:24]
+    top = (4294967295 // 2520) * 2520 + 1 - 2520 * 4000  # = 1 mod 2520, near 2^32
+    primes += [p for p in range(top, 4294967295, 2520) if oracle.is_prime(p)][:4]
+    sparse = engine.profile_block(primes)
+    dense = engine.profile_block(primes, method=rabung_amd.METHOD_DENSE)
+    assert sparse == dense
+    small = [p for p in primes if p < 10_000_000]
+    assert engine.profile_block(small) == oracle.profile_many(small)

@@ -1,0 +1,38 @@
+"""CPU check of the sparse anchor search design (tests/sparse_model.py).
+
+The model uses the kernels' cover, fold, band schedule, wheel masks,
+thresholds and completion with plain integers; it must reproduce the oracle's
+(M, witness, L) for every admissible order.
+"""
+import random
+
+import oracle
+import sparse_model
+
+
+def _check(primes):
+    for p in primes:
+        rec = oracle.profile_many([p])[0]
+        for r in range(2, 10):
+            if rec["mask"] & (1 << (r - 2)):
+                got = sparse_model.sparse_profile(p, r)
+                assert got == (rec["M"][r - 2], rec["W"][r - 2], rec["L"][r - 2]), (p, r, got)
+
+
+def test_model_random_primes():
+    rng = random.Random(1)
+    _check(oracle.next_primes(65537, 4) +
+           [oracle.next_primes(rng.randrange(70000, 400000), 1)[0] for _ in range(8)])
+
+
+def test_model_all_orders():
+    _check([p for p in range(2520 * 30 + 1, 2520 * 400, 2520) if oracle.is_prime(p)][:6])
+
+
+def test_cover_certificate():
+    for p in oracle.next_primes(65537, 3) + [16777259, 4294967291]:
+        B, D = sparse_model.cover(p)
+        H = 47
+        assert 4 * B * D - H * D * D > 4 * p and H * D <= 2 * B and B < p
+        # the least B: one less fails the area inequality
+        assert 4 * (B - 1) * D - H * D * D <= 4 * p
