@@ -511,7 +511,9 @@ int sparse_run(rbg_ctx *c, rbg_record *recs, u32 *err, std::vector<u32> &failed_
     HIPCK(c, hipEventRecord(c->ev[kSeed0], s));
     HIPCK(c, rbg_sparse_seed(dP, dF, nf, (const u16 *)c->sp_labels.ptr, best, failed, s));
     HIPCK(c, hipEventRecord(c->ev[kSeed1], s));
-    const u32 grid = std::max(1u, (u32)c->cus * blocks_per_cu(c));
+    // the filter's XCD-aware schedule needs a grid that is a multiple of 8
+    const u32 grid = std::max(8u, ((u32)c->cus * blocks_per_cu(c) + 7u) / 8u * 8u);
+    if (dmax * (nf + 64ull) > 0xFFFFFFFFull) return c->fail(RBG_ECAPACITY, "too many filter tasks");
     HIPCK(c, hipEventRecord(c->ev[kFilt0], s));
     HIPCK(c, rbg_sparse_filter(dP, dF, nf, (u32)dmax, (const u32 *)c->sp_planes.ptr,
                                (const u32 *)c->sp_inv.ptr, (const u32 *)c->sp_val.ptr,

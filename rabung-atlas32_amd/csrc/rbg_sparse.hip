@@ -77,7 +77,7 @@ __global__ __launch_bounds__(256) void sp_labels(const SparsePrime *__restrict__
     if (bad) atomicOr(err, 1u);
 }
 
-// inv[inv_off + d] = d^{-1} mod p for d in [1, D] (d^(p-2) in Montgomery form).
+// inv[inv_off + d] = Montgomery form d^{-1} R mod p for d in [1, D] (d^(p-2)).
 __global__ __launch_bounds__(256) void sp_inverses(const SparsePrime *__restrict__ P, u32 nprimes,
                                                    u32 *__restrict__ inv) {
     const u32 pi = __builtin_amdgcn_readfirstlane(find_prime(P, nprimes, blockIdx.x, 1));
@@ -86,7 +86,7 @@ __global__ __launch_bounds__(256) void sp_inverses(const SparsePrime *__restrict
     if (d == 0 || d > s.D) return;
     const u32 dm = mont_mul(d, s.r2, s.p, s.ninv);
     const u32 im = mont_pow(dm, s.p - 2, s.p, s.ninv, s.one);
-    inv[(u64)s.inv_off + d] = mont_mul(im, 1u, s.p, s.ninv);
+    inv[(u64)s.inv_off + d] = im;
 }
 
 constexpr u32 kPlanePairsPerWave = 16;
@@ -195,7 +195,8 @@ __device__ void complete_anchor(const SparsePrime &s, const SparseProf &f, const
                                 u32 d, u32 dinv, u32 t, const u32 *tab_val, const u16 *tab_lab,
                                 unsigned long long *best_slot, u32 *failed) {
     const u32 p = s.p;
-    const u32 bd = (u32)((u64)b * dinv % p);
+    // dinv is the Montgomery form d^{-1} R, so REDC(b * dinv) = b / d mod p
+    const u32 bd = mont_mul(b, dinv, p, s.ninv);
     const u32 x = p - bd;  // bd != 0 because 0 < b <= B < p
     const int64_t B = s.B;
     const u32 c0 = colour_at(pl, f.wpp, f.np, -(int64_t)b, s.B);
@@ -254,6 +255,199 @@ __device__ __forceinline__ u64 window64(const u32 *pl, u32 i) {
 }
 
 constexpr u64 kThirds = 0x9249249249249249ull;  // bits i = 0 mod 3
+constexpr u32 kXcds = 8;   // XCDs of an MI355X
+constexpr u32 kGroup = 8;  // profiles one XCD works on at a time
+
+// Bilateral screen (every placement of a q-window containing the anchor;
+// reference lemma GPU_METHOD_EN.tex:457-480, vdw_cut::screen): with
+// h = q - t extra positions needed, a candidate anchor is kept iff it has
+// L matching positions to its left and R to its right with L + R >= h.  In
+// numerator space (negated, so equality is unchanged) position x-k reads
+// b + k d and position x+t-1+k reads b - (t-1+k) d; numerators outside
+// [-B, B] are unknown and never reject.  R is kept as a bit-sliced count
+// (5 planes, h <= 23) so that every candidate of the word is screened at once.
+__device__ __forceinline__ u64 match_left(const u32 *pl, u32 wpp, u32 np, const u64 *base,
+                                          u32 b0, u32 B, u32 d, u32 k, u64 &n_windows) {
+    const int64_t lo = (int64_t)b0 + (int64_t)k * d, Bs = B;
+    if (lo > Bs) return ~0ull;
+    u64 mm = 0;
+    const u32 idx = (u32)(lo + Bs);
+    for (u32 pb = 0; pb < np; ++pb) mm |= base[pb] ^ window64(pl + (u64)pb * wpp, idx);
+    n_windows += np;
+    const int64_t known = Bs - lo + 1;  // bits 0..known-1 lie in the table
+    if (known < 64) mm &= (1ull << known) - 1ull;
+    return ~mm;
+}
+
+__device__ __forceinline__ u64 match_right(const u32 *pl, u32 wpp, u32 np, const u64 *base,
+                                           u32 b0, u32 B, u32 d, u32 t, u32 k, u64 &n_windows) {
+    const int64_t lo = (int64_t)b0 - (int64_t)(t - 1 + k) * d, Bs = B;
+    if (lo + 63 < -Bs) return ~0ull;
+    u64 mm = 0;
+    if (lo >= -Bs) {
+        const u32 idx = (u32)(lo + Bs);
+        for (u32 pb = 0; pb < np; ++pb) mm |= base[pb] ^ window64(pl + (u64)pb * wpp, idx);
+    } else {
+        const u32 shift = (u32)(-Bs - lo);  // 1..63 low bits are below -B: unknown
+        for (u32 pb = 0; pb < np; ++pb) mm |= base[pb] ^ (window64(pl + (u64)pb * wpp, 0) << shift);
+        mm &= ~((1ull << shift) - 1ull);
+    }
+    n_windows += np;
+    return ~mm;
+}
+
+__device__ __forceinline__ u64 screen(const u32 *pl, u32 wpp, u32 np, const u64 *base, u32 b0,
+                                      u32 B, u32 d, u32 t, u32 h, u64 &n_windows) {
+    // R: consecutive right matches per candidate, bit-sliced (r0 = LSB)
+    u64 r0 = 0, r1 = 0, r2 = 0, r3 = 0, r4 = 0, cr = ~0ull;
+    for (u32 k = 1; k <= h; ++k) {
+        cr &= match_right(pl, wpp, np, base, b0, B, d, t, k, n_windows);
+        if (!cr) break;
+        u64 c = cr, x;
+        x = r0 & c; r0 ^= c; c = x;
+        x = r1 & c; r1 ^= c; c = x;
+        x = r2 & c; r2 ^= c; c = x;
+        x = r3 & c; r3 ^= c; c = x;
+        r4 ^= c;
+    }
+    u64 pass = 0, cl = ~0ull;
+    for (u32 k = 0; k <= h; ++k) {
+        if (k) {
+            cl &= match_left(pl, wpp, np, base, b0, B, d, k, n_windows);
+            if (!cl) break;
+        }
+        // candidates with R >= h - k (compare the sliced count with a constant)
+        const u32 need = h - k;
+        const u64 rb[5] = {r0, r1, r2, r3, r4};
+        u64 gt = 0, eq = ~0ull;
+        for (int bit = 4; bit >= 0; --bit) {
+            if ((need >> bit) & 1u) {
+                eq &= rb[bit];
+            } else {
+                gt |= eq & rb[bit];
+                eq &= ~rb[bit];
+            }
+        }
+        pass |= cl & (gt | eq);
+    }
+    return pass;
+}
+
+// Anchor test of one 64-candidate word: for j = 1..t-1 the windows at
+// numerators b - j d must equal the base windows.  Stages are issued in
+// batches of four (two for three or four planes; all loads of a batch before any wait), so a word costs
+// one memory round trip per batch instead of one per stage; the batch
+// stops early once no candidate is left.  Stages past t-1 re-read stage t-1
+// (harmless, keeps every address inside the table).
+template <int NP>
+__device__ __forceinline__ u64 anchor_test(const u32 *pl, u32 wpp, const u64 *base, u32 ib, u32 d,
+                                           u32 t, u64 valid, u64 &n_windows) {
+    constexpr int G = NP <= 2 ? 4 : 2;  // stages per batch (register budget)
+    u64 mism = 0;
+    for (u32 j0 = 1; j0 < t; j0 += G) {
+        u64 w[G][NP];
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            const u32 j = min(j0 + (u32)g, t - 1u);
+            const u32 io = ib - j * d;
+#pragma unroll
+            for (int pb = 0; pb < NP; ++pb) w[g][pb] = window64(pl + (u64)pb * wpp, io);
+        }
+#pragma unroll
+        for (int g = 0; g < G; ++g)
+#pragma unroll
+            for (int pb = 0; pb < NP; ++pb) mism |= base[pb] ^ w[g][pb];
+        n_windows += (u64)NP * min((u32)G, t - j0);
+        if ((valid & ~mism) == 0) break;
+    }
+    return valid & ~mism;
+}
+
+struct FilterCtx {
+    const SparsePrime *s;
+    const SparseProf *f;
+    const u32 *pl;
+    const u32 *tab_val;
+    const u16 *tab_lab;
+    unsigned long long *best;
+    u32 *failed;
+    u32 d, dinv, t, q, B, bmin;
+};
+
+// All scheduled words of one (profile, denominator) task for NP planes.
+template <int NP>
+__device__ void filter_item(const FilterCtx &c, u32 sp, u32 phi, u64 &n_windows, u64 &n_words,
+                            u64 &n_done) {
+    const u32 lane = threadIdx.x & 63u;
+    const u32 d = c.d, t = c.t, B = c.B, bmin = c.bmin, wpp = c.f->wpp;
+    // band layout: kind 0 = c in [1, ca] (2c < d), kind 1 = c in [cb, d-1]
+    // (2c > d); for d <= 2 one unscheduled band covers [max(bmin, .), B].
+    for (u32 kind = 0; kind < 2; ++kind) {
+        u32 c_lo, c_hi, n0, nb, step;
+        if (d <= 2) {
+            if (kind) break;
+            const u32 lo_b = max(bmin, d == 1 ? t : t - 1u);
+            if (lo_b > B) break;
+            c_lo = lo_b;
+            c_hi = B;
+            n0 = 0;
+            nb = 1;
+            step = 0;
+        } else {
+            const u32 ca = (d - 1u) / 2u, cb = d / 2u + 1u;
+            c_lo = kind ? cb : 1u;
+            c_hi = kind ? d - 1u : ca;
+            if (c_lo > c_hi) continue;
+            const u32 res = kind ? (t - 2u - phi) % sp : phi;
+            const u32 n_lo = bmin / d, n_hi = B / d;
+            n0 = n_lo + (res + sp - n_lo % sp) % sp;
+            nb = n0 > n_hi ? 0u : (n_hi - n0) / sp + 1u;
+            step = sp;
+        }
+        const u32 width = c_hi - c_lo + 1u;
+        const u32 W = (width + 63u) >> 6;
+        const u32 total = nb * W;  // < 2^32: at most B/64 + nb words
+        // idx -> (band, word) with a float reciprocal and an exact fix-up
+        const float rW = 1.0f / (float)W;
+        for (u32 base_idx = 0; base_idx < total; base_idx += 64u) {
+            const u32 idx = base_idx + lane;
+            if (idx >= total) continue;
+            u32 band = (u32)((float)idx * rW);
+            while (band * W > idx) --band;
+            while ((band + 1u) * W <= idx) ++band;
+            const u32 w = idx - band * W;
+            const u32 nd = (n0 + band * step) * d;  // <= B + d < 2^32
+            const u32 b0 = nd + c_lo + 64u * w;
+            const u32 bend = min(nd + c_hi, B);
+            if (b0 > bend) continue;
+            const u32 nbits = min(64u, bend - b0 + 1u);
+            u64 valid = nbits == 64u ? ~0ull : ((1ull << nbits) - 1ull);
+            if (b0 < bmin) {
+                const u32 below = bmin - b0;
+                if (below >= nbits) continue;
+                valid &= ~0ull << below;
+            }
+            if ((d & 1u) == 0) valid &= (b0 & 1u) ? 0x5555555555555555ull : 0xAAAAAAAAAAAAAAAAull;
+            if (d % 3u == 0) valid &= ~(kThirds << ((3u - b0 % 3u) % 3u));
+            if (!valid) continue;
+            const u32 ib = b0 + B;
+            ++n_words;
+            u64 base[NP];
+#pragma unroll
+            for (int pb = 0; pb < NP; ++pb) base[pb] = window64(c.pl + (u64)pb * wpp, ib);
+            u64 surv = anchor_test<NP>(c.pl, wpp, base, ib, d, t, valid, n_windows);
+            if (surv && c.q > t)
+                surv &= screen(c.pl, wpp, NP, base, b0, B, d, t, c.q - t, n_windows);
+            while (surv) {
+                const u32 bit = (u32)__builtin_ctzll(surv);
+                surv &= surv - 1ull;
+                complete_anchor(*c.s, *c.f, c.pl, b0 + bit, d, c.dinv, t, c.tab_val, c.tab_lab,
+                                c.best, c.failed);
+                ++n_done;
+            }
+        }
+    }
+}
 
 __global__ __launch_bounds__(256) void sp_filter(const SparsePrime *__restrict__ P,
                                                  const SparseProf *__restrict__ prof, u32 nprof,
@@ -270,93 +464,55 @@ __global__ __launch_bounds__(256) void sp_filter(const SparsePrime *__restrict__
     if (threadIdx.x < 3) s_stats[threadIdx.x] = 0;
     __syncthreads();
     u64 n_windows = 0, n_words = 0, n_done = 0;
-    const u32 lane = threadIdx.x & 63u;
-    const u32 waves = gridDim.x * (blockDim.x >> 6);
-    const u32 wid = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-    const u64 items = (u64)dmax * nprof;
-    for (u64 item = wid; item < items; item += waves) {
-        const u32 d = 1u + (u32)(item / nprof);
-        const u32 k = __builtin_amdgcn_readfirstlane((u32)(item % nprof));
-        const SparseProf f = prof[k];
+    // XCD-aware schedule: workgroups are dispatched round-robin over the 8
+    // XCDs, so block b runs on XCD b % 8.  XCD x owns the profiles
+    // k = x, x+8, x+16, ... and walks them in groups of kGroup profiles,
+    // denominator-major inside a group: the waves of one XCD then touch the
+    // planes of at most kGroup profiles at a time (they stay in its L2),
+    // while each profile's denominators are still spread over the group.
+    const u32 xcd = blockIdx.x % kXcds;
+    const u32 wpb = blockDim.x >> 6;
+    const u32 wl = (blockIdx.x / kXcds) * wpb + (threadIdx.x >> 6);
+    const u32 waves_x = (gridDim.x / kXcds) * wpb;  // grid is a multiple of 8
+    const u32 nprof_x = nprof > xcd ? (nprof - xcd + kXcds - 1) / kXcds : 0u;
+    const u32 ngroups = (nprof_x + kGroup - 1) / kGroup;
+    const u32 items = ngroups * kGroup * dmax;  // checked below 2^32 by the host
+    for (u32 item = wl; item < items; item += waves_x) {
+        const u32 g = item / (kGroup * dmax);
+        const u32 rem = item - g * (kGroup * dmax);
+        const u32 dd = __builtin_amdgcn_readfirstlane(1u + rem / kGroup);
+        const u32 local = g * kGroup + rem % kGroup;
+        if (local >= nprof_x) continue;
+        const u32 kk = __builtin_amdgcn_readfirstlane(xcd + local * kXcds);
+        const SparseProf f = prof[kk];
         const SparsePrime s = P[f.prime];
-        if (d > s.D) continue;
-        const u64 key = __hip_atomic_load(&best[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (dd > s.D) continue;
+        const u64 key = __hip_atomic_load(&best[kk], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         const u32 q = min((u32)(key >> 32), s.Q);
         if (q < 3u) continue;
         u32 t = max(3u, (q + 1u) / 2u);
         if (t > q) t = q;
         const u32 sp = q - t + 1u;  // schedule period
         const u32 phi = ((t - 1u) / 2u) % sp;
-        const u32 B = s.B;
-        const u32 bmin = ((t - 1u) * d + 1u) / 2u;
-        const u32 *pl = planes + f.plane_off;
-        const u32 dinv = inv[(u64)s.inv_off + d];
-        // band layout: kind 0 = c in [1, ca] (2c < d), kind 1 = c in [cb, d-1]
-        // (2c > d); for d <= 2 one unscheduled band covers [bmin', B].
-        for (u32 kind = 0; kind < 2; ++kind) {
-            u32 c_lo, c_hi, n0, nb, step;
-            if (d <= 2) {
-                if (kind) break;
-                const u32 lo_b = max(bmin, d == 1 ? t : t - 1u);
-                if (lo_b > B) break;
-                // a single band [lo_b, B]: treat as n = 0, c from lo_b
-                c_lo = lo_b;
-                c_hi = B;
-                n0 = 0;
-                nb = 1;
-                step = 0;
-            } else {
-                const u32 ca = (d - 1u) / 2u, cb = d / 2u + 1u;
-                c_lo = kind ? cb : 1u;
-                c_hi = kind ? d - 1u : ca;
-                if (c_lo > c_hi) continue;
-                const u32 res = kind ? (t - 2u - phi) % sp : phi;
-                const u32 n_lo = bmin / d, n_hi = B / d;
-                n0 = n_lo + (res + sp - n_lo % sp) % sp;
-                nb = n0 > n_hi ? 0u : (n_hi - n0) / sp + 1u;
-                step = sp;
-            }
-            const u32 width = c_hi - c_lo + 1u;
-            const u32 W = (width + 63u) >> 6;
-            const u64 total = (u64)nb * W;
-            for (u64 idx = lane; idx < ((total + 63u) & ~63ull); idx += 64u) {
-                if (idx >= total) continue;
-                const u32 band = (u32)(idx / W), w = (u32)(idx % W);
-                const u64 nd = (u64)(n0 + band * step) * d;
-                const u64 b0 = nd + c_lo + 64ull * w;
-                const u64 bend = min(nd + c_hi, (u64)B);
-                if (b0 > bend) continue;
-                const u32 nbits = (u32)min<u64>(64, bend - b0 + 1);
-                u64 valid = nbits == 64 ? ~0ull : ((1ull << nbits) - 1ull);
-                if (b0 < bmin) {
-                    const u64 below = bmin - b0;
-                    if (below >= nbits) continue;
-                    valid &= ~0ull << below;
-                }
-                if ((d & 1u) == 0) valid &= (b0 & 1u) ? 0x5555555555555555ull : 0xAAAAAAAAAAAAAAAAull;
-                if (d % 3u == 0) valid &= ~(kThirds << ((3u - (u32)(b0 % 3u)) % 3u));
-                if (!valid) continue;
-                const u32 ib = (u32)(b0 + B);
-                ++n_words;
-                u64 base[4];
-                for (u32 pb = 0; pb < f.np; ++pb) base[pb] = window64(pl + (u64)pb * f.wpp, ib);
-                u64 mism = 0;
-                for (u32 j = 1; j < t; ++j) {
-                    const u32 io = ib - j * d;
-                    for (u32 pb = 0; pb < f.np; ++pb)
-                        mism |= base[pb] ^ window64(pl + (u64)pb * f.wpp, io);
-                    n_windows += f.np;
-                    if ((valid & ~mism) == 0) break;
-                }
-                u64 surv = valid & ~mism;
-                while (surv) {
-                    const u32 bit = (u32)__builtin_ctzll(surv);
-                    surv &= surv - 1ull;
-                    complete_anchor(s, f, pl, (u32)b0 + bit, d, dinv, t, tab_val, tab_lab,
-                                    &best[k], failed);
-                    ++n_done;
-                }
-            }
+        FilterCtx c;
+        c.s = &s;
+        c.f = &f;
+        c.pl = planes + f.plane_off;
+        c.tab_val = tab_val;
+        c.tab_lab = tab_lab;
+        c.best = &best[kk];
+        c.failed = failed;
+        c.d = dd;
+        c.dinv = inv[(u64)s.inv_off + dd];
+        c.t = t;
+        c.q = q;
+        c.B = s.B;
+        c.bmin = ((t - 1u) * dd + 1u) / 2u;
+        switch (f.np) {
+        case 1: filter_item<1>(c, sp, phi, n_windows, n_words, n_done); break;
+        case 2: filter_item<2>(c, sp, phi, n_windows, n_words, n_done); break;
+        case 3: filter_item<3>(c, sp, phi, n_windows, n_words, n_done); break;
+        default: filter_item<4>(c, sp, phi, n_windows, n_words, n_done); break;
         }
     }
     atomicAdd(&s_stats[0], (unsigned long long)n_windows);

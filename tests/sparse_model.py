@@ -41,8 +41,11 @@ def labels_of(p, m):
     return lambda y: root[pow(y, e, p)]
 
 
-def sparse_profile(p, r, Q=48, order="ascending"):
+def sparse_profile(p, r, Q=48, screen=True, stats=None):
     """(M, witness, L) of order r by the anchor search; None if refused."""
+    if stats is None:
+        stats = {}
+    stats["completions"] = 0
     c = cover(p, Q)
     if c is None:
         return None
@@ -116,6 +119,23 @@ def sparse_profile(p, r, Q=48, order="ascending"):
             a += d
         return hi - lo + 1, min(lo, p - hi)
 
+    def passes_screen(b, d, t, q, c0):
+        """L + R >= q - t, numerators outside [-B, B] count as matches."""
+        h = q - t
+        L = 0
+        while L < h:
+            a = b + (L + 1) * d
+            if a <= B and col_num(a) != c0:
+                break
+            L += 1
+        R = 0
+        while R < h:
+            a = b - (t - 1 + R + 1) * d
+            if a >= -B and col_num(a) != c0:
+                break
+            R += 1
+        return L + R >= h
+
     for d in range(1, D + 1):
         q = min(key[0], Q)
         t = min(q, max(3, (q + 1) // 2))
@@ -147,6 +167,9 @@ def sparse_profile(p, r, Q=48, order="ascending"):
                     continue
                 c0 = col_num(b)
                 if all(col_num(b - j * d) == c0 for j in range(1, t)):
+                    if screen and not passes_screen(b, d, t, q, c0):
+                        continue
+                    stats["completions"] += 1
                     length, w = complete(b, d, t)
                     key = max(key, (length, -w))
     return key[0], -key[1], prefix

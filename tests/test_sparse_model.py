@@ -36,3 +36,13 @@ def test_cover_certificate():
         assert 4 * B * D - H * D * D > 4 * p and H * D <= 2 * B and B < p
         # the least B: one less fails the area inequality
         assert 4 * (B - 1) * D - H * D * D <= 4 * p
+
+
+def test_screen_prunes_without_changing_results():
+    for p in oracle.next_primes(200003, 3):
+        rec = oracle.profile_many([p])[0]
+        a, b = {}, {}
+        assert sparse_model.sparse_profile(p, 2, screen=True, stats=a) == \
+            sparse_model.sparse_profile(p, 2, screen=False, stats=b) == \
+            (rec["M"][0], rec["W"][0], rec["L"][0])
+        assert a["completions"] < b["completions"]
