@@ -51,6 +51,42 @@ WORKLOADS = {
     "block2_24": ("plain block: 4096 consecutive primes from the first prime >= 2^24, all "
                   "admissible r, full profile + A4", lambda: block_from(1 << 24, 4096)),
 }
+BATCH_DESC = ("batching: 64 blocks of U{1..4096} consecutive primes from log-uniform starts in "
+              "[2^20, 2^32) (seed 2026), all admissible r, one profile_batch call")
+
+
+def bench_batch64(args):
+    """The cross-block batching case: one rbg_profile_batch call per step."""
+    import torch
+    import rabung_amd
+    from rabung_amd import workloads
+    blocks = workloads.batch_blocks(seed=2026, nblocks=64)
+    flat = [p for b in blocks for p in b]
+    profiles = sum(bin(rabung_amd.mask(p)).count("1") for p in flat)
+    with rabung_amd.Engine(0, hbm_budget=64 << 30) as eng:
+        for _ in range(args.warmup):
+            eng.profile_batch(flat)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            recs = eng.profile_batch(flat)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / args.steps
+        t = eng.last_timing()
+    checked = golden_check(flat, recs)
+    line = {"metric": METRIC, "value": round(profiles / dt, 3), "unit": "profiles/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(dt * 1000.0, 3),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u32",
+            "data": "synthetic: seeded prime blocks",
+            "config": {"workload": BATCH_DESC, "blocks": len(blocks), "primes": len(flat),
+                       "profiles": profiles, "sparse_primes": t["sparse_primes"],
+                       "dense_primes": t["dense_primes"], "fallbacks": t["fallbacks"],
+                       "golden_checked": checked},
+            "stage_ms": {"prep": round(t["sparse_prep_ms"], 3), "seed": round(t["sparse_seed_ms"], 3),
+                         "filter": round(t["sparse_filter_ms"], 3),
+                         "dense": round(t["dense_scan_ms"], 3), "device_total": round(t["total_ms"], 3)}}
+    print(json.dumps(line), flush=True)
+    return 0
 
 
 def mm_products(e):
@@ -131,14 +167,19 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="top256", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="top256", choices=sorted(WORKLOADS) + ["batch64"])
     ap.add_argument("--no-plain", action="store_true", help="skip timing the plain method")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--serve-seconds", type=float, default=10.0,
+                    help="length of the Poisson serving run (0 skips it)")
+    ap.add_argument("--qps", type=float, default=20.0, help="serving arrival rate")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (the path shards across primes with "
               "no exchange; see DESIGN.md); run with --gpus 1", file=sys.stderr)
         return 2
+    if args.workload == "batch64":
+        return bench_batch64(args)
     import torch
     import rabung_amd
     from rabung_amd import Record, RECORD_SIZE
@@ -230,6 +271,19 @@ def main():
                     "frac": round(ops / (kern_ms / 1000.0) / 1e12 / peak_tops, 4),
                     "traffic": None, "avg_launch_ms": round(kern_ms, 3),
                     "algorithmic_ops_per_launch": ops, "peak_basis": peak_basis}
+    serving = None
+    if args.serve_seconds > 0:
+        from rabung_amd import serve, workloads
+        stream = workloads.serving_stream(seed=4, qps=args.qps, seconds=args.serve_seconds)
+        with serve.Server(slots=2) as srv:
+            serve.run_stream(srv, workloads.serving_stream(seed=5, qps=args.qps, seconds=1.0))
+            srv.batches.clear()
+            _, summ = serve.run_stream(srv, stream)
+        serving = {"workload": "Poisson arrivals at %.1f QPS for %.0f s (seed 4): 80%% single "
+                               "primes (next prime >= uniform u32), 20%% intervals of U{16..4096} "
+                               "consecutive primes from log-uniform starts in [2^16, 2^32); "
+                               "2 slots, continuous batching" % (args.qps, args.serve_seconds),
+                   **{k: (round(v, 3) if isinstance(v, float) else v) for k, v in summ.items()}}
     cpu = None if args.no_cpu else cpu_baseline(primes)
     line = {
         "metric": METRIC, "value": round(value, 3), "unit": "profiles/s", "n_gpus": 1,
@@ -243,6 +297,7 @@ def main():
         "plain": plain,
         "batched_vs_plain": round(value / plain["profiles_per_s"], 2) if plain else None,
         "roofline": roofline,
+        "serving": serving,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -79,6 +79,21 @@ rbg_ctx *rbg_open(const rbg_config *cfg);
 int rbg_profile_block(rbg_ctx *ctx, const uint32_t *primes, uint32_t n,
                       uint32_t flags, rbg_record *out);
 
+/* Batching across blocks: n <= RBG_MAX_BATCH primes (e.g. many pack blocks
+ * concatenated) profiled in as few launch chains as the HBM budget allows
+ * (greedy sub-batches of at most 16384 primes, one persistent filter launch
+ * each).  No reference counterpart: the reference processed 128-prime
+ * batches per worker process (reproducibility/gpu_method/source/worker32.cu:
+ * 37-49).  All-or-nothing like rbg_profile_block. */
+#define RBG_MAX_BATCH (1u << 20)
+int rbg_profile_batch(rbg_ctx *ctx, const uint32_t *primes, uint32_t n,
+                      uint32_t flags, rbg_record *out);
+
+/* Device workspace (bytes) that profiling these primes with `flags` needs;
+ * serving loops use it for admission by HBM budget. */
+int rbg_estimate_bytes(const rbg_ctx *ctx, const uint32_t *primes, uint32_t n,
+                       uint32_t flags, uint64_t *bytes);
+
 /* Same with device-resident input and output (d_primes: n uint32 in device
  * memory, d_out: n records in device memory).  Enqueues on the context's
  * stream and waits for completion.  Host-side preparation reads the primes
@@ -123,6 +138,11 @@ int rbg_a4_host(const uint32_t *primes, uint32_t n, uint32_t *out, uint32_t thre
 
 /* Admissibility mask of p (bit r-2 iff r | p-1); 0 for p < 3. */
 uint32_t rbg_mask(uint32_t p);
+
+/* Up to `count` consecutive primes >= start (and < 2^32) into out; returns how
+ * many were written.  Builds prime blocks (the reference's segmented sieve,
+ * src/census/prime_sieve.cpp:8-22, serves the same purpose). */
+uint32_t rbg_primes_from(uint64_t start, uint32_t count, uint32_t *out);
 
 /* Deterministic primality for 32-bit inputs. */
 int rbg_is_prime(uint32_t p);

@@ -192,6 +192,7 @@ struct DevBuf {
 
 constexpr u32 kSparseMin = 1u << 16;  // smallest prime sent to the sparse engine
 constexpr u32 kCoverQ = 48;           // threshold cap of the cover
+constexpr u32 kBatchChunk = 16384;    // primes per launch chain of a batch
 
 enum Ev { kAll0, kAll1, kDense0, kDense1, kPrep0, kPrep1, kSeed0, kSeed1, kFilt0, kFilt1, kNumEv };
 
@@ -543,9 +544,28 @@ int sparse_run(rbg_ctx *c, rbg_record *recs, u32 *err, std::vector<u32> &failed_
     return RBG_OK;
 }
 
+// Device workspace one prime needs under `flags` (the planning rules of
+// sparse_plan and dense_run).
+u64 prime_bytes(u32 p, u32 flags, u32 tile_pos) {
+    if (p < 3) return sizeof(rbg_record);
+    const u32 mask = mask_of(p), m = lcm_of(mask), nr = (u32)__builtin_popcount(mask);
+    u64 bytes = sizeof(rbg_record) + (u64)m * 8 + 64;
+    u32 B = 0, D = 0;
+    if ((flags & RBG_METHOD_MASK) == RBG_METHOD_BATCHED && p >= kSparseMin &&
+        choose_cover(p, kCoverQ, B, D)) {
+        bytes += 2ull * (B + 1) + 4ull * (D + 1) + 80;
+        for (u32 r = 2; r <= 9; ++r)
+            if (mask & (1u << (r - 2)))
+                bytes += 4ull * (32u - (u32)__builtin_clz(r - 1)) * ((2u * B + 192u) / 32u + 2u) + 48;
+        return bytes;
+    }
+    const u64 tiles = ((p - 1) / 2 + tile_pos - 1) / tile_pos;
+    return bytes + tiles * (sizeof(TileHdr) + nr * sizeof(RunSum));
+}
+
 int run_block(rbg_ctx *c, const u32 *h_primes, u32 n, u32 flags, rbg_record *d_out,
-              rbg_record *h_out) {
-    if (n > RBG_MAX_BLOCK) return c->fail(RBG_EINVAL, "more than 4096 primes in one block");
+              rbg_record *h_out, u32 limit = RBG_MAX_BLOCK) {
+    if (n > limit) return c->fail(RBG_EINVAL, "more than 4096 primes in one block");
     c->error.clear();
     c->timing = rbg_timing{};
     const u32 method = flags & RBG_METHOD_MASK;
@@ -644,6 +664,63 @@ int rbg_profile_block(rbg_ctx *c, const uint32_t *primes, uint32_t n, uint32_t f
     return rc;
 }
 
+int rbg_estimate_bytes(const rbg_ctx *c, const uint32_t *primes, uint32_t n, uint32_t flags,
+                       uint64_t *bytes) {
+    if (!c || (!primes && n) || !bytes) return RBG_EINVAL;
+    const u32 tile_pos = (u32)rbg_waves_per_block() * tile_chunks(c) * 64u;
+    u64 total = 0;
+    for (u32 i = 0; i < n; ++i) total += prime_bytes(primes[i], flags, tile_pos);
+    *bytes = total;
+    return RBG_OK;
+}
+
+int rbg_profile_batch(rbg_ctx *c, const uint32_t *primes, uint32_t n, uint32_t flags,
+                      rbg_record *out) {
+    if (!c || (!primes && n) || (!out && n)) return RBG_EINVAL;
+    if (n > RBG_MAX_BATCH) return c->fail(RBG_EINVAL, "batch larger than RBG_MAX_BATCH");
+    const u32 tile_pos = (u32)rbg_waves_per_block() * tile_chunks(c) * 64u;
+    const u64 cap = budget(c) / 10 * 9;
+    rbg_timing sum{};
+    u32 first = 0;
+    while (first < n) {
+        // greedy sub-batch under the HBM budget and the task-count limit
+        u32 last = first;
+        u64 bytes = 0;
+        while (last < n && last - first < kBatchChunk) {
+            const u64 b = prime_bytes(primes[last], flags, tile_pos);
+            if (last > first && bytes + b > cap) break;
+            bytes += b;
+            ++last;
+        }
+        int rc = run_block(c, primes + first, last - first, flags, nullptr, out + first,
+                           kBatchChunk);
+        if (rc) {
+            memset(out, 0, sizeof(rbg_record) * n);
+            return rc;
+        }
+        for (u32 i = first; i < last; ++i) out[i].status = 0;
+        const rbg_timing &t = c->timing;
+        sum.total_ms += t.total_ms;
+        sum.dense_scan_ms += t.dense_scan_ms;
+        sum.sparse_prep_ms += t.sparse_prep_ms;
+        sum.sparse_seed_ms += t.sparse_seed_ms;
+        sum.sparse_filter_ms += t.sparse_filter_ms;
+        sum.dense_primes += t.dense_primes;
+        sum.sparse_primes += t.sparse_primes;
+        sum.sparse_profiles += t.sparse_profiles;
+        sum.fallbacks += t.fallbacks;
+        sum.positions += t.positions;
+        sum.products += t.products;
+        sum.launches += t.launches;
+        sum.filter_windows += t.filter_windows;
+        sum.filter_words += t.filter_words;
+        sum.filter_completions += t.filter_completions;
+        first = last;
+    }
+    c->timing = sum;
+    return RBG_OK;
+}
+
 int rbg_profile_block_device(rbg_ctx *c, const uint32_t *d_primes, uint32_t n, uint32_t flags,
                              rbg_record *d_out) {
     if (!c || (!d_primes && n) || (!d_out && n)) return RBG_EINVAL;
@@ -684,6 +761,14 @@ void rbg_close(rbg_ctx *c) {
 }
 
 uint32_t rbg_mask(uint32_t p) { return mask_of(p); }
+
+uint32_t rbg_primes_from(uint64_t start, uint32_t count, uint32_t *out) {
+    if (!out) return 0;
+    u32 k = 0;
+    for (u64 x = start < 2 ? 2 : start; k < count && x < (1ull << 32); ++x)
+        if (prime_u32((u32)x)) out[k++] = (u32)x;
+    return k;
+}
 
 int rbg_is_prime(uint32_t p) { return prime_u32(p) ? 1 : 0; }
 

@@ -53,9 +53,10 @@ class Config(ctypes.Structure):
                 ("blocks_per_cu", ctypes.c_uint32), ("hbm_budget", ctypes.c_uint64)]
 
 
-EXPORTS = ["rbg_open", "rbg_profile_block", "rbg_profile_block_device", "rbg_last_error",
+EXPORTS = ["rbg_open", "rbg_profile_block", "rbg_profile_block_device", "rbg_profile_batch",
+           "rbg_estimate_bytes", "rbg_last_error",
            "rbg_last_timing", "rbg_device_info", "rbg_close", "rbg_mask", "rbg_is_prime",
-           "rbg_version", "rbg_a4_host", "baseline_init", "baseline_profile"]
+           "rbg_version", "rbg_a4_host", "rbg_primes_from", "baseline_init", "baseline_profile"]
 
 
 class RabungError(RuntimeError):
@@ -85,6 +86,12 @@ def load():
     lib.rbg_profile_block_device.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32,
                                              ctypes.c_uint32, ctypes.c_void_p]
     lib.rbg_profile_block_device.restype = ctypes.c_int
+    lib.rbg_profile_batch.argtypes = [ctypes.c_void_p, P(ctypes.c_uint32), ctypes.c_uint32,
+                                      ctypes.c_uint32, P(Record)]
+    lib.rbg_profile_batch.restype = ctypes.c_int
+    lib.rbg_estimate_bytes.argtypes = [ctypes.c_void_p, P(ctypes.c_uint32), ctypes.c_uint32,
+                                       ctypes.c_uint32, P(ctypes.c_uint64)]
+    lib.rbg_estimate_bytes.restype = ctypes.c_int
     lib.rbg_last_error.argtypes = [ctypes.c_void_p]
     lib.rbg_last_error.restype = ctypes.c_char_p
     lib.rbg_last_timing.argtypes = [ctypes.c_void_p, P(Timing)]
@@ -100,6 +107,8 @@ def load():
     lib.rbg_is_prime.restype = ctypes.c_int
     lib.rbg_version.argtypes = []
     lib.rbg_version.restype = ctypes.c_char_p
+    lib.rbg_primes_from.argtypes = [ctypes.c_uint64, ctypes.c_uint32, P(ctypes.c_uint32)]
+    lib.rbg_primes_from.restype = ctypes.c_uint32
     lib.rbg_a4_host.argtypes = [P(ctypes.c_uint32), ctypes.c_uint32, P(ctypes.c_uint32),
                                 ctypes.c_uint32]
     lib.rbg_a4_host.restype = ctypes.c_int
@@ -157,6 +166,27 @@ class Engine:
             raise RabungError(rc, self._lib.rbg_last_error(self._ctx).decode())
         return [record_to_dict(out[i]) for i in range(n)]
 
+    def profile_batch(self, primes, method=METHOD_BATCHED, certify=False):
+        """Profiles of up to 2^20 primes (many blocks), split by HBM budget."""
+        primes = [int(p) for p in primes]
+        n = len(primes)
+        arr = (ctypes.c_uint32 * max(1, n))(*primes)
+        out = (Record * max(1, n))()
+        flags = method | (CERTIFY if certify else 0)
+        rc = self._lib.rbg_profile_batch(self._ctx, arr, n, flags, out)
+        if rc:
+            raise RabungError(rc, self._lib.rbg_last_error(self._ctx).decode())
+        return [record_to_dict(out[i]) for i in range(n)]
+
+    def estimate_bytes(self, primes, method=METHOD_BATCHED):
+        primes = [int(p) for p in primes]
+        arr = (ctypes.c_uint32 * max(1, len(primes)))(*primes)
+        out = ctypes.c_uint64()
+        rc = self._lib.rbg_estimate_bytes(self._ctx, arr, len(primes), method, ctypes.byref(out))
+        if rc:
+            raise RabungError(rc, "estimate failed")
+        return out.value
+
     def profile_block_device(self, d_primes_ptr, n, d_out_ptr, method=METHOD_BATCHED):
         """Device-resident variant: pointers to n uint32 primes and n records."""
         rc = self._lib.rbg_profile_block_device(self._ctx, ctypes.c_void_p(d_primes_ptr), n,
@@ -186,6 +216,13 @@ def mask(p):
 
 def is_prime(p):
     return bool(load().rbg_is_prime(p))
+
+
+def primes_from(start, count):
+    """Up to `count` consecutive primes >= start below 2^32."""
+    out = (ctypes.c_uint32 * max(1, count))()
+    k = load().rbg_primes_from(start, count, out)
+    return list(out[:k])
 
 
 def a4_host(primes, threads=0):

@@ -1,0 +1,182 @@
+"""Continuous-batching request loop over the census library.
+
+Requests (a single prime or an interval of consecutive primes) arrive at any
+time.  Each of `slots` worker threads owns one library context: a HIP stream
+and its resident device workspaces (root tables, label tables, colour planes,
+inverses and run keys stay allocated between batches).  Whenever a slot is
+free it takes every waiting request, in arrival order, while the batch stays
+within `max_primes` primes and within the slot's share of the HBM budget
+(estimated with the library's own workspace model, `rbg_estimate_bytes`);
+requests that arrive while batches run are admitted into the next free slot.
+Batches of different slots run concurrently on their own streams, and the
+host-side work of one (root tables, elliptic A4 counts) overlaps the device
+work of another.  Results are returned per request, identical to profiling
+the request alone (every record depends on its prime only).
+
+The reference has no serving path; its workers read 128-prime batches from
+stdin (src/census/worker32.cu:10-45).
+"""
+import collections
+import threading
+import time
+
+from . import METHOD_BATCHED, Engine
+
+
+class Request:
+    __slots__ = ("rid", "primes", "arrival", "admitted", "done", "records", "error", "bytes",
+                 "_event")
+
+    def __init__(self, rid, primes, arrival):
+        self.rid = rid
+        self.primes = list(primes)
+        self.arrival = arrival
+        self.admitted = None
+        self.done = None
+        self.records = None
+        self.error = None
+        self.bytes = 0
+        self._event = threading.Event()
+
+    def wait(self, timeout=None):
+        self._event.wait(timeout)
+        if self.error:
+            raise self.error
+        return self.records
+
+    @property
+    def latency(self):
+        return None if self.done is None else self.done - self.arrival
+
+
+class Server:
+    def __init__(self, slots=2, hbm_budget=64 << 30, max_primes=16384, method=METHOD_BATCHED,
+                 engine_factory=None, clock=time.perf_counter):
+        self.clock = clock
+        self.method = method
+        self.max_primes = max_primes
+        self.slot_budget = hbm_budget // max(1, slots)
+        factory = engine_factory or (lambda: Engine(0, hbm_budget=self.slot_budget))
+        self.engines = [factory() for _ in range(slots)]
+        self.queue = collections.deque()
+        self.cond = threading.Condition()
+        self.stopping = False
+        self.batches = []  # (slot, n_requests, n_primes, t_start, t_end)
+        self._next_id = 0
+        self.threads = [threading.Thread(target=self._worker, args=(i,), daemon=True)
+                        for i in range(slots)]
+        for t in self.threads:
+            t.start()
+
+    def submit(self, primes, arrival=None):
+        """Queue a request; returns a Request whose wait() gives its records."""
+        req = Request(self._next_id, primes, self.clock() if arrival is None else arrival)
+        self._next_id += 1
+        req.bytes = self.engines[0].estimate_bytes(req.primes, self.method)
+        with self.cond:
+            self.queue.append(req)
+            self.cond.notify()
+        return req
+
+    def _take(self):
+        """Admit waiting requests in order while the batch fits (caller holds cond)."""
+        batch, primes, nbytes = [], 0, 0
+        while self.queue:
+            req = self.queue[0]
+            if batch and (primes + len(req.primes) > self.max_primes
+                          or nbytes + req.bytes > self.slot_budget):
+                break
+            batch.append(self.queue.popleft())
+            primes += len(req.primes)
+            nbytes += req.bytes
+        return batch
+
+    def _worker(self, slot):
+        eng = self.engines[slot]
+        while True:
+            with self.cond:
+                while not self.queue and not self.stopping:
+                    self.cond.wait()
+                if self.stopping and not self.queue:
+                    return
+                batch = self._take()
+            t0 = self.clock()
+            for req in batch:
+                req.admitted = t0
+            flat = [p for req in batch for p in req.primes]
+            try:
+                records = eng.profile_batch(flat, method=self.method)
+                err = None
+            except Exception as exc:  # refuse the whole batch, visibly
+                records, err = None, exc
+            t1 = self.clock()
+            self.batches.append((slot, len(batch), len(flat), t0, t1))
+            pos = 0
+            for req in batch:
+                if err is None:
+                    req.records = records[pos:pos + len(req.primes)]
+                else:
+                    req.error = err
+                pos += len(req.primes)
+                req.done = t1
+                req._event.set()
+
+    def close(self):
+        with self.cond:
+            self.stopping = True
+            self.cond.notify_all()
+        for t in self.threads:
+            t.join()
+        for eng in self.engines:
+            if hasattr(eng, "close"):
+                eng.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+def percentile(values, q):
+    vals = sorted(values)
+    if not vals:
+        return None
+    k = (len(vals) - 1) * q
+    lo = int(k)
+    hi = min(lo + 1, len(vals) - 1)
+    return vals[lo] + (vals[hi] - vals[lo]) * (k - lo)
+
+
+def run_stream(server, stream, clock=time.perf_counter, sleep=time.sleep):
+    """Replay [(arrival_s, primes)] against the server in real time.
+
+    Returns (requests, summary) with p50/p99 latency in milliseconds and the
+    profile throughput over the whole run."""
+    t0 = clock()
+    reqs = []
+    for arrival, primes in stream:
+        now = clock() - t0
+        if arrival > now:
+            sleep(arrival - now)
+        reqs.append(server.submit(primes, arrival=t0 + arrival))
+    for r in reqs:
+        r.wait()
+    t_end = max(r.done for r in reqs) if reqs else t0
+    lat = [r.latency * 1000.0 for r in reqs]
+    single = [r.latency * 1000.0 for r in reqs if len(r.primes) == 1]
+    profiles = sum(bin(rec["mask"]).count("1") for r in reqs for rec in r.records)
+    summary = {
+        "requests": len(reqs),
+        "primes": sum(len(r.primes) for r in reqs),
+        "profiles": profiles,
+        "p50_ms": percentile(lat, 0.50),
+        "p99_ms": percentile(lat, 0.99),
+        "p50_single_ms": percentile(single, 0.50),
+        "p99_single_ms": percentile(single, 0.99),
+        "wall_s": t_end - t0,
+        "profiles_per_s": profiles / (t_end - t0) if t_end > t0 else None,
+        "batches": len(server.batches),
+        "max_batch_requests": max((b[1] for b in server.batches), default=0),
+    }
+    return reqs, summary
