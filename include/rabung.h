@@ -62,9 +62,11 @@ typedef struct rbg_record {
 
 typedef struct rbg_config {
     int32_t device;          /* HIP device ordinal */
-    uint32_t tile_chunks;    /* 64-position chunks per wave segment; 0 = tuned */
-    uint32_t blocks_per_cu;  /* persistent blocks per CU; 0 = tuned */
+    uint32_t tile_chunks;    /* dense scan: 64-position chunks per wave segment; 0 = 64 */
+    uint32_t blocks_per_cu;  /* persistent blocks (4 waves) per CU; 0 = 8 */
     uint64_t hbm_budget;     /* bytes of device workspace allowed; 0 = 16 GiB */
+    uint32_t xcd_group;      /* sparse filter: profiles one XCD works on at a time; 0 = 8 */
+    uint32_t reserved;
 } rbg_config;
 
 typedef struct rbg_ctx rbg_ctx;
@@ -88,6 +90,15 @@ int rbg_profile_block(rbg_ctx *ctx, const uint32_t *primes, uint32_t n,
 #define RBG_MAX_BATCH (1u << 20)
 int rbg_profile_batch(rbg_ctx *ctx, const uint32_t *primes, uint32_t n,
                       uint32_t flags, rbg_record *out);
+
+/* Autotuner: times the method of `flags` on the sample primes (n <= 4096)
+ * over launch shapes (batched: blocks per CU x XCD profile group; dense:
+ * segment length x blocks per CU), checks that every shape gives identical
+ * records, keeps the fastest in the context and copies it to *chosen.  No
+ * reference counterpart (the reference hand-picked its launch constants,
+ * src/census/GPU_CONFIG.json:38-59). */
+int rbg_autotune(rbg_ctx *ctx, const uint32_t *primes, uint32_t n, uint32_t flags,
+                 rbg_config *chosen);
 
 /* Device workspace (bytes) that profiling these primes with `flags` needs;
  * serving loops use it for admission by HBM budget. */

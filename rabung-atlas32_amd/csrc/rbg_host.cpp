@@ -51,15 +51,18 @@ extern "C" hipError_t rbg_sparse_prepare(const SparsePrime *P, uint32_t nprimes,
                                          uint32_t plane_blocks, const uint32_t *tab_val,
                                          const uint16_t *tab_lab, uint16_t *labels, uint32_t *inv,
                                          uint32_t *planes, uint32_t *err, hipStream_t stream);
+extern "C" uint32_t rbg_seed_segments(uint32_t bmax);
+extern "C" size_t rbg_seed_agg_bytes(void);
 extern "C" hipError_t rbg_sparse_seed(const SparsePrime *P, SparseProf *prof, uint32_t nprof,
-                                      const uint16_t *labels, unsigned long long *best,
-                                      uint32_t *failed, hipStream_t stream);
+                                      const uint16_t *labels, void *segs, uint32_t segs_per_prof,
+                                      unsigned long long *best, uint32_t *failed,
+                                      hipStream_t stream);
 extern "C" hipError_t rbg_sparse_filter(const SparsePrime *P, const SparseProf *prof,
                                         uint32_t nprof, uint32_t dmax, const uint32_t *planes,
                                         const uint32_t *inv, const uint32_t *tab_val,
                                         const uint16_t *tab_lab, unsigned long long *best,
                                         uint32_t *failed, unsigned long long *stats, uint32_t grid,
-                                        hipStream_t stream);
+                                        uint32_t group, hipStream_t stream);
 extern "C" hipError_t rbg_sparse_finalize(const SparsePrime *P, uint32_t nprimes,
                                           const SparseProf *prof, const unsigned long long *best,
                                           const uint32_t *a4, rbg_record *out, hipStream_t stream);
@@ -214,7 +217,7 @@ struct rbg_ctx {
     std::vector<u16> h_code;
     // sparse path
     DevBuf sp_primes, sp_profs, sp_val, sp_lab, sp_labels, sp_inv, sp_planes, sp_best, sp_failed,
-        sp_a4, sp_stats;
+        sp_a4, sp_stats, sp_segs;
     std::vector<SparsePrime> h_sp;
     std::vector<SparseProf> h_prof;
     std::vector<u32> h_sval;
@@ -243,6 +246,8 @@ int hip_fail(rbg_ctx *c, hipError_t e, const char *what) {
 u32 tile_chunks(const rbg_ctx *c) { return c->cfg.tile_chunks ? c->cfg.tile_chunks : 64u; }
 
 u32 blocks_per_cu(const rbg_ctx *c) { return c->cfg.blocks_per_cu ? c->cfg.blocks_per_cu : 8u; }
+
+u32 xcd_group(const rbg_ctx *c) { return c->cfg.xcd_group ? c->cfg.xcd_group : 8u; }
 
 u64 budget(const rbg_ctx *c) { return c->cfg.hbm_budget ? c->cfg.hbm_budget : (16ull << 30); }
 
@@ -451,14 +456,16 @@ int sparse_plan(rbg_ctx *c, const u32 *primes, const std::vector<u32> &idx,
 int sparse_run(rbg_ctx *c, rbg_record *recs, u32 *err, std::vector<u32> &failed_idx) {
     const u32 np = (u32)c->h_sp.size(), nf = (u32)c->h_prof.size();
     if (!np) return RBG_OK;
-    u64 lab = 0, inv = 0, words = 0, lblk = 0, iblk = 0, pblk = 0, dmax = 0;
+    u64 lab = 0, inv = 0, words = 0, lblk = 0, iblk = 0, pblk = 0, dmax = 0, bmax = 0;
     for (const SparsePrime &s : c->h_sp) {
         lab += s.B + 1ull;
         inv += s.D + 1ull;
         lblk += (s.B + 2047ull) / 2048ull;
         iblk += (s.D + 1ull + 255ull) / 256ull;
         dmax = std::max<u64>(dmax, s.D);
+        bmax = std::max<u64>(bmax, s.B);
     }
+    const u32 segs_per_prof = rbg_seed_segments((u32)bmax);
     for (const SparseProf &f : c->h_prof) {
         words += (u64)f.np * f.wpp;
         pblk += ((f.wpp + 1ull) / 2ull + 63ull) / 64ull;
@@ -477,6 +484,7 @@ int sparse_run(rbg_ctx *c, rbg_record *recs, u32 *err, std::vector<u32> &failed_
     HIPCK(c, c->sp_failed.ensure(np * 4));
     HIPCK(c, c->sp_a4.ensure(np * 4));
     HIPCK(c, c->sp_stats.ensure(3 * 8));
+    HIPCK(c, c->sp_segs.ensure((u64)nf * segs_per_prof * rbg_seed_agg_bytes()));
     hipStream_t s = c->stream;
     // A4 by elliptic point counts on the host while the device works
     std::vector<u32> sp_p(np), a4(np, 0);
@@ -510,16 +518,18 @@ int sparse_run(rbg_ctx *c, rbg_record *recs, u32 *err, std::vector<u32> &failed_
                                 (u32 *)c->sp_planes.ptr, err, s));
     HIPCK(c, hipEventRecord(c->ev[kPrep1], s));
     HIPCK(c, hipEventRecord(c->ev[kSeed0], s));
-    HIPCK(c, rbg_sparse_seed(dP, dF, nf, (const u16 *)c->sp_labels.ptr, best, failed, s));
+    HIPCK(c, rbg_sparse_seed(dP, dF, nf, (const u16 *)c->sp_labels.ptr, c->sp_segs.ptr,
+                             segs_per_prof, best, failed, s));
     HIPCK(c, hipEventRecord(c->ev[kSeed1], s));
     // the filter's XCD-aware schedule needs a grid that is a multiple of 8
     const u32 grid = std::max(8u, ((u32)c->cus * blocks_per_cu(c) + 7u) / 8u * 8u);
-    if (dmax * (nf + 64ull) > 0xFFFFFFFFull) return c->fail(RBG_ECAPACITY, "too many filter tasks");
+    if (dmax * (nf + 8ull * xcd_group(c)) > 0xFFFFFFFFull)
+        return c->fail(RBG_ECAPACITY, "too many filter tasks");
     HIPCK(c, hipEventRecord(c->ev[kFilt0], s));
     HIPCK(c, rbg_sparse_filter(dP, dF, nf, (u32)dmax, (const u32 *)c->sp_planes.ptr,
                                (const u32 *)c->sp_inv.ptr, (const u32 *)c->sp_val.ptr,
                                (const u16 *)c->sp_lab.ptr, best, failed,
-                               (unsigned long long *)c->sp_stats.ptr, grid, s));
+                               (unsigned long long *)c->sp_stats.ptr, grid, xcd_group(c), s));
     HIPCK(c, hipEventRecord(c->ev[kFilt1], s));
     c->timing.launches += 6;
     std::vector<u32> h_failed(np, 0);
@@ -721,6 +731,63 @@ int rbg_profile_batch(rbg_ctx *c, const uint32_t *primes, uint32_t n, uint32_t f
     return RBG_OK;
 }
 
+int rbg_autotune(rbg_ctx *c, const uint32_t *primes, uint32_t n, uint32_t flags,
+                 rbg_config *chosen) {
+    if (!c || !primes || !n || n > RBG_MAX_BLOCK) return RBG_EINVAL;
+    const u32 method = flags & RBG_METHOD_MASK;
+    struct Cand {
+        u32 chunks, bpc, group;
+    };
+    std::vector<Cand> cands;
+    if (method == RBG_METHOD_BATCHED) {
+        for (u32 bpc : {4u, 6u, 8u, 12u, 16u})
+            for (u32 g : {4u, 8u, 16u}) cands.push_back({c->cfg.tile_chunks, bpc, g});
+    } else {
+        for (u32 ch : {16u, 32u, 64u, 128u, 256u})
+            for (u32 bpc : {4u, 8u}) cands.push_back({ch, bpc, c->cfg.xcd_group});
+    }
+    const rbg_config saved = c->cfg;
+    std::vector<rbg_record> ref(n), got(n);
+    double best_ms = 1e300;
+    Cand best = cands[0];
+    bool have_ref = false;
+    for (const Cand &k : cands) {
+        c->cfg.tile_chunks = k.chunks;
+        c->cfg.blocks_per_cu = k.bpc;
+        c->cfg.xcd_group = k.group;
+        double ms = 1e300;
+        for (int rep = 0; rep < 2; ++rep) {
+            int rc = run_block(c, primes, n, method, nullptr, got.data());
+            if (rc) {
+                c->cfg = saved;
+                return rc;
+            }
+            const rbg_timing &t = c->timing;
+            const double stage = method == RBG_METHOD_BATCHED
+                                     ? t.sparse_filter_ms + t.sparse_seed_ms + t.dense_scan_ms
+                                     : t.dense_scan_ms;
+            ms = std::min(ms, stage);
+        }
+        // every configuration must give the same records
+        if (!have_ref) {
+            ref = got;
+            have_ref = true;
+        } else if (memcmp(ref.data(), got.data(), n * sizeof(rbg_record)) != 0) {
+            c->cfg = saved;
+            return c->fail(RBG_ECERTIFY, "autotune: configurations disagree");
+        }
+        if (ms < best_ms) {
+            best_ms = ms;
+            best = k;
+        }
+    }
+    c->cfg.tile_chunks = best.chunks;
+    c->cfg.blocks_per_cu = best.bpc;
+    c->cfg.xcd_group = best.group;
+    if (chosen) *chosen = c->cfg;
+    return RBG_OK;
+}
+
 int rbg_profile_block_device(rbg_ctx *c, const uint32_t *d_primes, uint32_t n, uint32_t flags,
                              rbg_record *d_out) {
     if (!c || (!d_primes && n) || (!d_out && n)) return RBG_EINVAL;
@@ -752,7 +819,7 @@ void rbg_close(rbg_ctx *c) {
     for (DevBuf *b : {&c->descs, &c->tab_val, &c->tab_code, &c->hdr, &c->runs, &c->recs, &c->flags,
                       &c->sp_primes, &c->sp_profs, &c->sp_val, &c->sp_lab, &c->sp_labels,
                       &c->sp_inv, &c->sp_planes, &c->sp_best, &c->sp_failed, &c->sp_a4,
-                      &c->sp_stats})
+                      &c->sp_stats, &c->sp_segs})
         b->release();
     for (int i = 0; i < kNumEv; ++i)
         if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);

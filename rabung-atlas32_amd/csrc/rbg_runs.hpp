@@ -117,4 +117,43 @@ __device__ __forceinline__ u32 order_finish(OrderState &s, u32 len) {
     return len ? suffix : 0u;
 }
 
+__device__ __forceinline__ u32 f4(u32 len) { return len > 3 ? len - 3 : 0; }
+
+// Associative run monoid of a contiguous stretch (the reference's SummOp,
+// original_scan_gpu.cu:116-128, with the least-start witness of
+// dense_oracle.cu:32-44 and the A4 window count): merge(A, B) for A
+// immediately left of B.  `planes` selects the colour bits of the order.
+// Positions are absolute.
+struct Agg {
+    u32 first_x, len, fcode, lcode;
+    u32 prefix, suffix, best, bstart, a4;
+};
+
+__device__ __forceinline__ Agg merge(const Agg &A, const Agg &B, u32 planes) {
+    if (A.len == 0) return B;
+    if (B.len == 0) return A;
+    Agg C;
+    C.first_x = A.first_x;
+    C.len = A.len + B.len;
+    C.fcode = A.fcode;
+    C.lcode = B.lcode;
+    C.best = A.best;
+    C.bstart = A.bstart;
+    take(B.best, B.bstart, C.best, C.bstart);
+    const bool eq = ((A.lcode ^ B.fcode) & planes) == 0u;
+    if (eq) {
+        const u32 cross = A.suffix + B.prefix;
+        const u32 cstart = A.first_x + A.len - A.suffix;
+        take(cross, cstart, C.best, C.bstart);
+        C.prefix = (A.prefix == A.len) ? A.len + B.prefix : A.prefix;
+        C.suffix = (B.suffix == B.len) ? B.len + A.suffix : B.suffix;
+        C.a4 = A.a4 + B.a4 + f4(cross) - f4(A.suffix) - f4(B.prefix);
+    } else {
+        C.prefix = A.prefix;
+        C.suffix = B.suffix;
+        C.a4 = A.a4 + B.a4;
+    }
+    return C;
+}
+
 }  // namespace rbg

@@ -34,8 +34,6 @@
 
 namespace rbg {
 
-__device__ __forceinline__ u32 f4(u32 len) { return len > 3 ? len - 3 : 0; }
-
 // Scan one segment [seg_start, seg_start+seg_len) with the calling wave.
 // Writes per-order summaries (positions relative to seg_start) into out[r-2].
 __device__ void wave_scan(const PrimeDesc &d, u32 seg_start, u32 seg_len, const u32 *s_val,
@@ -100,40 +98,6 @@ __device__ void wave_scan(const PrimeDesc &d, u32 seg_start, u32 seg_len, const 
         }
     }
     codes_out = first_code | (last_code << 16);
-}
-
-// Ordered merge of two adjacent summaries A (left) and B (right) of one
-// order with plane bits `planes`.  Positions in A/B are absolute.
-struct Agg {
-    u32 first_x, len, fcode, lcode;
-    u32 prefix, suffix, best, bstart, a4;
-};
-
-__device__ __forceinline__ Agg merge(const Agg &A, const Agg &B, u32 planes) {
-    if (A.len == 0) return B;
-    if (B.len == 0) return A;
-    Agg C;
-    C.first_x = A.first_x;
-    C.len = A.len + B.len;
-    C.fcode = A.fcode;
-    C.lcode = B.lcode;
-    C.best = A.best;
-    C.bstart = A.bstart;
-    take(B.best, B.bstart, C.best, C.bstart);
-    bool eq = ((A.lcode ^ B.fcode) & planes) == 0u;
-    if (eq) {
-        u32 cross = A.suffix + B.prefix;
-        u32 cstart = A.first_x + A.len - A.suffix;
-        take(cross, cstart, C.best, C.bstart);
-        C.prefix = (A.prefix == A.len) ? A.len + B.prefix : A.prefix;
-        C.suffix = (B.suffix == B.len) ? B.len + A.suffix : B.suffix;
-        C.a4 = A.a4 + B.a4 + f4(cross) - f4(A.suffix) - f4(B.prefix);
-    } else {
-        C.prefix = A.prefix;
-        C.suffix = B.suffix;
-        C.a4 = A.a4 + B.a4;
-    }
-    return C;
 }
 
 constexpr int kWavesPerBlock = 4;

@@ -128,43 +128,75 @@ __global__ __launch_bounds__(256) void sp_planes(const SparsePrime *__restrict__
     }
 }
 
-// One wave per profile: runs of 1..B in order (real runs of the sequence).
-__global__ __launch_bounds__(64) void sp_seed(const SparsePrime *__restrict__ P,
-                                              SparseProf *__restrict__ prof, u32 nprof,
-                                              const u16 *__restrict__ labels,
-                                              unsigned long long *__restrict__ best,
-                                              u32 *__restrict__ failed) {
-    const u32 k = blockIdx.x;
+constexpr u32 kSeedSeg = 4096;  // positions of 1..B per seed wave
+
+// Seed runs of 1..B (real runs of the sequence): one wave per segment of
+// kSeedSeg positions writes the segment's run summary (Agg over colour values).
+__global__ __launch_bounds__(256) void sp_seed_scan(const SparsePrime *__restrict__ P,
+                                                    const SparseProf *__restrict__ prof,
+                                                    u32 nprof, const u16 *__restrict__ labels,
+                                                    Agg *__restrict__ segs, u32 segs_per_prof) {
+    const u32 wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const u32 k = __builtin_amdgcn_readfirstlane(wave / segs_per_prof);
+    const u32 sg = __builtin_amdgcn_readfirstlane(wave % segs_per_prof);
     if (k >= nprof) return;
     const SparseProf f = prof[k];
     const SparsePrime s = P[f.prime];
-    const u32 lane = threadIdx.x;
+    const u32 lane = threadIdx.x & 63u;
+    const u32 first = 1u + sg * kSeedSeg;
+    const u32 len = first > s.B ? 0u : min(kSeedSeg, s.B - first + 1u);
     OrderState st;
     order_init(st);
-    u32 carry = 0;
-    const u32 nchunks = (s.B + 63u) >> 6;
+    u32 carry = 0, fcol = 0, lcol = 0;
+    const u32 nchunks = (len + 63u) >> 6;
     for (u32 c = 0; c < nchunks; ++c) {
         const u32 base = c << 6;
-        const u32 valid = min(64u, s.B - base);
+        const u32 valid = min(64u, len - base);
         const u64 vmask = valid == 64u ? ~0ull : ((1ull << valid) - 1ull);
-        const u32 y = 1u + base + lane;
+        const u32 y = first + base + lane;
         const u32 col = y <= s.B ? labels[(u64)s.lab_off + y] % f.r : 0u;
         const u32 left = shift_up_one(col, carry);
         carry = __builtin_amdgcn_readlane(col, 63);
+        if (c == 0) fcol = __builtin_amdgcn_readfirstlane(col);
+        lcol = __builtin_amdgcn_readlane(col, valid - 1);
         u64 C = ballot64(col != left);
         if (c == 0) C |= 1ull;
         C &= vmask;
         order_update(st, C, ~C & vmask, base);
     }
-    order_finish(st, s.B);
+    const u32 suffix = order_finish(st, len);
     if (lane == 0) {
-        // positions are relative to y = 1; runs inside 1..B < p/2 have
-        // reflected start = start
-        const u32 w = st.bstart + 1u;
-        best[k] = ((unsigned long long)st.best << 32) | (0xFFFFFFFFu - w);
-        prof[k].prefix = st.popen ? s.B : st.prefix;
-        if (st.popen || st.best < 3u) atomicOr(&failed[f.prime], 1u);
+        Agg a;
+        a.first_x = first;
+        a.len = len;
+        a.fcode = fcol;
+        a.lcode = lcol;
+        a.prefix = st.prefix;
+        a.suffix = suffix;
+        a.best = st.best;
+        a.bstart = st.bstart + first;
+        a.a4 = 0;
+        segs[(u64)k * segs_per_prof + sg] = a;
     }
+}
+
+// One thread per profile: ordered merge of its segment summaries; the key of
+// the longest seed run (partial end run included) and the prefix L.
+__global__ void sp_seed_merge(const SparsePrime *__restrict__ P, SparseProf *__restrict__ prof,
+                              u32 nprof, const Agg *__restrict__ segs, u32 segs_per_prof,
+                              unsigned long long *__restrict__ best, u32 *__restrict__ failed) {
+    const u32 k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nprof) return;
+    const SparseProf f = prof[k];
+    const SparsePrime s = P[f.prime];
+    Agg acc;
+    acc.len = 0;
+    for (u32 i = 0; i < segs_per_prof; ++i) acc = merge(acc, segs[(u64)k * segs_per_prof + i], ~0u);
+    // runs inside 1..B < p/2 have reflected start = start
+    best[k] = ((unsigned long long)acc.best << 32) | (0xFFFFFFFFu - acc.bstart);
+    const bool whole = acc.prefix == s.B;  // the identity run may continue past B
+    prof[k].prefix = acc.prefix;
+    if (whole || acc.best < 3u || acc.len != s.B || acc.first_x != 1u) atomicOr(&failed[f.prime], 1u);
 }
 
 __device__ __forceinline__ u32 colour_at(const u32 *pl, u32 wpp, u32 np, int64_t a, u32 B) {
@@ -255,8 +287,7 @@ __device__ __forceinline__ u64 window64(const u32 *pl, u32 i) {
 }
 
 constexpr u64 kThirds = 0x9249249249249249ull;  // bits i = 0 mod 3
-constexpr u32 kXcds = 8;   // XCDs of an MI355X
-constexpr u32 kGroup = 8;  // profiles one XCD works on at a time
+constexpr u32 kXcds = 8;  // XCDs of an MI355X
 
 // Bilateral screen (every placement of a q-window containing the anchor;
 // reference lemma GPU_METHOD_EN.tex:457-480, vdw_cut::screen): with
@@ -335,17 +366,21 @@ __device__ __forceinline__ u64 screen(const u32 *pl, u32 wpp, u32 np, const u64 
 
 // Anchor test of one 64-candidate word: for j = 1..t-1 the windows at
 // numerators b - j d must equal the base windows.  Stages are issued in
-// batches of four (two for three or four planes; all loads of a batch before any wait), so a word costs
+// batches (8, 4 or 2 stages for 1, 2 or more planes; all loads of a batch
+// before any wait, the base windows with the first batch), so a word costs
 // one memory round trip per batch instead of one per stage; the batch
 // stops early once no candidate is left.  Stages past t-1 re-read stage t-1
 // (harmless, keeps every address inside the table).
 template <int NP>
-__device__ __forceinline__ u64 anchor_test(const u32 *pl, u32 wpp, const u64 *base, u32 ib, u32 d,
-                                           u32 t, u64 valid, u64 &n_windows) {
-    constexpr int G = NP <= 2 ? 4 : 2;  // stages per batch (register budget)
+__device__ __forceinline__ u64 anchor_test(const u32 *pl, u32 wpp, u64 *base, u32 ib, u32 d, u32 t,
+                                           u64 valid, u64 &n_windows) {
+    constexpr int G = NP == 1 ? 8 : NP == 2 ? 4 : 2;  // stages per batch (register budget)
+    u64 w[G][NP];
+    // the base windows travel with the first batch: one round trip
+#pragma unroll
+    for (int pb = 0; pb < NP; ++pb) base[pb] = window64(pl + (u64)pb * wpp, ib);
     u64 mism = 0;
     for (u32 j0 = 1; j0 < t; j0 += G) {
-        u64 w[G][NP];
 #pragma unroll
         for (int g = 0; g < G; ++g) {
             const u32 j = min(j0 + (u32)g, t - 1u);
@@ -433,8 +468,6 @@ __device__ void filter_item(const FilterCtx &c, u32 sp, u32 phi, u64 &n_windows,
             const u32 ib = b0 + B;
             ++n_words;
             u64 base[NP];
-#pragma unroll
-            for (int pb = 0; pb < NP; ++pb) base[pb] = window64(c.pl + (u64)pb * wpp, ib);
             u64 surv = anchor_test<NP>(c.pl, wpp, base, ib, d, t, valid, n_windows);
             if (surv && c.q > t)
                 surv &= screen(c.pl, wpp, NP, base, b0, B, d, t, c.q - t, n_windows);
@@ -457,7 +490,8 @@ __global__ __launch_bounds__(256) void sp_filter(const SparsePrime *__restrict__
                                                  const u16 *__restrict__ tab_lab,
                                                  unsigned long long *__restrict__ best,
                                                  u32 *__restrict__ failed,
-                                                 unsigned long long *__restrict__ stats) {
+                                                 unsigned long long *__restrict__ stats,
+                                                 u32 kGroup) {
     // per-lane work counts: 64-bit window comparisons, candidate words,
     // completed anchors (summed per block, three atomics per block)
     __shared__ unsigned long long s_stats[3];
@@ -466,7 +500,8 @@ __global__ __launch_bounds__(256) void sp_filter(const SparsePrime *__restrict__
     u64 n_windows = 0, n_words = 0, n_done = 0;
     // XCD-aware schedule: workgroups are dispatched round-robin over the 8
     // XCDs, so block b runs on XCD b % 8.  XCD x owns the profiles
-    // k = x, x+8, x+16, ... and walks them in groups of kGroup profiles,
+    // k = x, x+8, x+16, ... and walks them in groups of kGroup profiles (a
+    // tuning knob, 8 by default),
     // denominator-major inside a group: the waves of one XCD then touch the
     // planes of at most kGroup profiles at a time (they stay in its L2),
     // while each profile's denominators are still spread over the group.
@@ -576,11 +611,21 @@ extern "C" hipError_t rbg_sparse_prepare(const SparsePrime *P, uint32_t nprimes,
     return hipGetLastError();
 }
 
+extern "C" uint32_t rbg_seed_segments(uint32_t bmax) {
+    return (bmax + rbg::kSeedSeg - 1) / rbg::kSeedSeg;
+}
+
+extern "C" size_t rbg_seed_agg_bytes(void) { return sizeof(rbg::Agg); }
+
 extern "C" hipError_t rbg_sparse_seed(const SparsePrime *P, SparseProf *prof, uint32_t nprof,
-                                      const uint16_t *labels, unsigned long long *best,
-                                      uint32_t *failed, hipStream_t stream) {
-    hipLaunchKernelGGL(rbg::sp_seed, dim3(nprof), dim3(64), 0, stream, P, prof, nprof, labels,
-                       best, failed);
+                                      const uint16_t *labels, void *segs, uint32_t segs_per_prof,
+                                      unsigned long long *best, uint32_t *failed,
+                                      hipStream_t stream) {
+    const uint32_t waves = nprof * segs_per_prof;
+    hipLaunchKernelGGL(rbg::sp_seed_scan, dim3((waves + 3) / 4), dim3(256), 0, stream, P, prof,
+                       nprof, labels, (rbg::Agg *)segs, segs_per_prof);
+    hipLaunchKernelGGL(rbg::sp_seed_merge, dim3((nprof + 63) / 64), dim3(64), 0, stream, P, prof,
+                       nprof, (const rbg::Agg *)segs, segs_per_prof, best, failed);
     return hipGetLastError();
 }
 
@@ -589,9 +634,9 @@ extern "C" hipError_t rbg_sparse_filter(const SparsePrime *P, const SparseProf *
                                         const uint32_t *inv, const uint32_t *tab_val,
                                         const uint16_t *tab_lab, unsigned long long *best,
                                         uint32_t *failed, unsigned long long *stats,
-                                        uint32_t grid, hipStream_t stream) {
+                                        uint32_t grid, uint32_t group, hipStream_t stream) {
     hipLaunchKernelGGL(rbg::sp_filter, dim3(grid), dim3(256), 0, stream, P, prof, nprof, dmax,
-                       planes, inv, tab_val, tab_lab, best, failed, stats);
+                       planes, inv, tab_val, tab_lab, best, failed, stats, group);
     return hipGetLastError();
 }
 

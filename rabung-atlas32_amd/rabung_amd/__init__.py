@@ -50,11 +50,12 @@ class Timing(ctypes.Structure):
 
 class Config(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int32), ("tile_chunks", ctypes.c_uint32),
-                ("blocks_per_cu", ctypes.c_uint32), ("hbm_budget", ctypes.c_uint64)]
+                ("blocks_per_cu", ctypes.c_uint32), ("hbm_budget", ctypes.c_uint64),
+                ("xcd_group", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
 
 
 EXPORTS = ["rbg_open", "rbg_profile_block", "rbg_profile_block_device", "rbg_profile_batch",
-           "rbg_estimate_bytes", "rbg_last_error",
+           "rbg_estimate_bytes", "rbg_autotune", "rbg_last_error",
            "rbg_last_timing", "rbg_device_info", "rbg_close", "rbg_mask", "rbg_is_prime",
            "rbg_version", "rbg_a4_host", "rbg_primes_from", "baseline_init", "baseline_profile"]
 
@@ -92,6 +93,9 @@ def load():
     lib.rbg_estimate_bytes.argtypes = [ctypes.c_void_p, P(ctypes.c_uint32), ctypes.c_uint32,
                                        ctypes.c_uint32, P(ctypes.c_uint64)]
     lib.rbg_estimate_bytes.restype = ctypes.c_int
+    lib.rbg_autotune.argtypes = [ctypes.c_void_p, P(ctypes.c_uint32), ctypes.c_uint32,
+                                 ctypes.c_uint32, P(Config)]
+    lib.rbg_autotune.restype = ctypes.c_int
     lib.rbg_last_error.argtypes = [ctypes.c_void_p]
     lib.rbg_last_error.restype = ctypes.c_char_p
     lib.rbg_last_timing.argtypes = [ctypes.c_void_p, P(Timing)]
@@ -129,9 +133,9 @@ def record_to_dict(rec):
 class Engine:
     """One library context (device, stream, workspaces)."""
 
-    def __init__(self, device=0, tile_chunks=0, blocks_per_cu=0, hbm_budget=0):
+    def __init__(self, device=0, tile_chunks=0, blocks_per_cu=0, hbm_budget=0, xcd_group=0):
         lib = load()
-        cfg = Config(device, tile_chunks, blocks_per_cu, hbm_budget)
+        cfg = Config(device, tile_chunks, blocks_per_cu, hbm_budget, xcd_group, 0)
         self._ctx = lib.rbg_open(ctypes.byref(cfg))
         if not self._ctx:
             raise RabungError(2, "rbg_open failed (no HIP device %d?)" % device)
@@ -177,6 +181,17 @@ class Engine:
         if rc:
             raise RabungError(rc, self._lib.rbg_last_error(self._ctx).decode())
         return [record_to_dict(out[i]) for i in range(n)]
+
+    def autotune(self, primes, method=METHOD_BATCHED):
+        """Pick the fastest launch shape on these sample primes; returns it."""
+        primes = [int(p) for p in primes]
+        arr = (ctypes.c_uint32 * max(1, len(primes)))(*primes)
+        cfg = Config()
+        rc = self._lib.rbg_autotune(self._ctx, arr, len(primes), method, ctypes.byref(cfg))
+        if rc:
+            raise RabungError(rc, self._lib.rbg_last_error(self._ctx).decode())
+        return {"tile_chunks": cfg.tile_chunks, "blocks_per_cu": cfg.blocks_per_cu,
+                "xcd_group": cfg.xcd_group}
 
     def estimate_bytes(self, primes, method=METHOD_BATCHED):
         primes = [int(p) for p in primes]

@@ -3,11 +3,15 @@
 Requests (a single prime or an interval of consecutive primes) arrive at any
 time.  Each of `slots` worker threads owns one library context: a HIP stream
 and its resident device workspaces (root tables, label tables, colour planes,
-inverses and run keys stay allocated between batches).  Whenever a slot is
-free it takes every waiting request, in arrival order, while the batch stays
-within `max_primes` primes and within the slot's share of the HBM budget
-(estimated with the library's own workspace model, `rbg_estimate_bytes`);
-requests that arrive while batches run are admitted into the next free slot.
+inverses and run keys stay allocated between batches).  A request is cut
+into pieces of at most `chunk` primes.  Whenever a slot is free it admits,
+first, every waiting small request (at most `small` primes, in arrival
+order), then bulk pieces in arrival order, while the batch stays within
+`max_primes` primes and within the slot's share of the HBM budget (estimated
+with the library's own workspace model, `rbg_estimate_bytes`); pieces that
+arrive while batches run join the next free slot.  Short batches and the
+small-first rule bound how long a single-prime query waits behind an
+interval query.
 Batches of different slots run concurrently on their own streams, and the
 host-side work of one (root tables, elliptic A4 counts) overlaps the device
 work of another.  Results are returned per request, identical to profiling
@@ -24,7 +28,7 @@ from . import METHOD_BATCHED, Engine
 
 
 class Request:
-    __slots__ = ("rid", "primes", "arrival", "admitted", "done", "records", "error", "bytes",
+    __slots__ = ("rid", "primes", "arrival", "admitted", "done", "records", "error", "pending",
                  "_event")
 
     def __init__(self, rid, primes, arrival):
@@ -33,9 +37,9 @@ class Request:
         self.arrival = arrival
         self.admitted = None
         self.done = None
-        self.records = None
+        self.records = [None] * len(self.primes)
         self.error = None
-        self.bytes = 0
+        self.pending = 0
         self._event = threading.Event()
 
     def wait(self, timeout=None):
@@ -49,15 +53,28 @@ class Request:
         return None if self.done is None else self.done - self.arrival
 
 
+class _Piece:
+    __slots__ = ("req", "offset", "primes", "bytes")
+
+    def __init__(self, req, offset, primes, nbytes):
+        self.req = req
+        self.offset = offset
+        self.primes = primes
+        self.bytes = nbytes
+
+
 class Server:
-    def __init__(self, slots=2, hbm_budget=64 << 30, max_primes=16384, method=METHOD_BATCHED,
-                 engine_factory=None, clock=time.perf_counter):
+    def __init__(self, slots=2, hbm_budget=64 << 30, max_primes=512, chunk=64, small=16,
+                 method=METHOD_BATCHED, engine_factory=None, clock=time.perf_counter):
         self.clock = clock
         self.method = method
         self.max_primes = max_primes
+        self.chunk = max(1, chunk)
+        self.small = small
         self.slot_budget = hbm_budget // max(1, slots)
         factory = engine_factory or (lambda: Engine(0, hbm_budget=self.slot_budget))
         self.engines = [factory() for _ in range(slots)]
+        self.small_queue = collections.deque()
         self.queue = collections.deque()
         self.cond = threading.Condition()
         self.stopping = False
@@ -72,53 +89,72 @@ class Server:
         """Queue a request; returns a Request whose wait() gives its records."""
         req = Request(self._next_id, primes, self.clock() if arrival is None else arrival)
         self._next_id += 1
-        req.bytes = self.engines[0].estimate_bytes(req.primes, self.method)
+        pieces = []
+        for off in range(0, len(req.primes), self.chunk):
+            part = req.primes[off:off + self.chunk]
+            pieces.append(_Piece(req, off, part, self.engines[0].estimate_bytes(part, self.method)))
+        req.pending = len(pieces)
+        if not pieces:  # nothing to compute
+            req.done = req.arrival
+            req._event.set()
+            return req
         with self.cond:
-            self.queue.append(req)
+            (self.small_queue if len(req.primes) <= self.small else self.queue).extend(pieces)
             self.cond.notify()
         return req
 
     def _take(self):
-        """Admit waiting requests in order while the batch fits (caller holds cond)."""
+        """Admit small requests first, then bulk pieces, in arrival order, while
+        the batch fits (caller holds cond)."""
         batch, primes, nbytes = [], 0, 0
-        while self.queue:
-            req = self.queue[0]
-            if batch and (primes + len(req.primes) > self.max_primes
-                          or nbytes + req.bytes > self.slot_budget):
-                break
-            batch.append(self.queue.popleft())
-            primes += len(req.primes)
-            nbytes += req.bytes
+        for q in (self.small_queue, self.queue):
+            while q:
+                piece = q[0]
+                if batch and (primes + len(piece.primes) > self.max_primes
+                              or nbytes + piece.bytes > self.slot_budget):
+                    return batch
+                batch.append(q.popleft())
+                primes += len(piece.primes)
+                nbytes += piece.bytes
         return batch
 
     def _worker(self, slot):
         eng = self.engines[slot]
         while True:
             with self.cond:
-                while not self.queue and not self.stopping:
+                while not self.queue and not self.small_queue and not self.stopping:
                     self.cond.wait()
-                if self.stopping and not self.queue:
+                if self.stopping and not self.queue and not self.small_queue:
                     return
                 batch = self._take()
             t0 = self.clock()
-            for req in batch:
-                req.admitted = t0
-            flat = [p for req in batch for p in req.primes]
+            for piece in batch:
+                if piece.req.admitted is None:
+                    piece.req.admitted = t0
+            flat = [p for piece in batch for p in piece.primes]
             try:
                 records = eng.profile_batch(flat, method=self.method)
                 err = None
             except Exception as exc:  # refuse the whole batch, visibly
                 records, err = None, exc
             t1 = self.clock()
-            self.batches.append((slot, len(batch), len(flat), t0, t1))
+            self.batches.append((slot, len({id(pc.req) for pc in batch}), len(flat), t0, t1))
             pos = 0
-            for req in batch:
-                if err is None:
-                    req.records = records[pos:pos + len(req.primes)]
-                else:
-                    req.error = err
-                pos += len(req.primes)
-                req.done = t1
+            finished = []
+            with self.cond:
+                for piece in batch:
+                    req = piece.req
+                    if err is None:
+                        req.records[piece.offset:piece.offset + len(piece.primes)] = \
+                            records[pos:pos + len(piece.primes)]
+                    else:
+                        req.error = err
+                    pos += len(piece.primes)
+                    req.pending -= 1
+                    if req.pending == 0:
+                        req.done = t1
+                        finished.append(req)
+            for req in finished:
                 req._event.set()
 
     def close(self):

@@ -183,3 +183,24 @@ def test_sparse_all_orders_and_small_domain(engine):
     assert sparse == dense
     small = [p for p in primes if p < 10_000_000]
     assert engine.profile_block(small) == oracle.profile_many(small)
+
+
+def test_autotune_keeps_results():
+    primes = oracle.next_primes(4_200_000_000, 12) + oracle.next_primes(1 << 24, 12)
+    with rabung_amd.Engine() as eng:
+        before = eng.profile_block(primes)
+        cfg = eng.autotune(primes[:16])
+        assert cfg["blocks_per_cu"] in (4, 6, 8, 12, 16) and cfg["xcd_group"] in (4, 8, 16)
+        assert eng.profile_block(primes) == before
+        dcfg = eng.autotune(primes[-8:], rabung_amd.METHOD_DENSE)
+        assert dcfg["tile_chunks"] in (16, 32, 64, 128, 256)
+        assert eng.profile_block(primes, method=rabung_amd.METHOD_DENSE) == before
+
+
+def test_batch_across_blocks(engine):
+    blocks = [oracle.next_primes(s, n) for s, n in [(1 << 20, 300), (3 << 30, 40), (70001, 900)]]
+    flat = [p for b in blocks for p in b]
+    got = engine.profile_batch(flat)
+    t = engine.last_timing()
+    assert t["sparse_primes"] == len(flat)
+    assert got == [r for b in blocks for r in engine.profile_block(b)]

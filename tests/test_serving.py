@@ -43,7 +43,8 @@ def test_requests_batched_in_order_and_routed():
         engs.append(FakeEngine(delay=0.05))
         return engs[-1]
 
-    with serve.Server(slots=1, hbm_budget=10**9, max_primes=50, engine_factory=factory) as srv:
+    with serve.Server(slots=1, hbm_budget=10**9, max_primes=50, small=0,
+                      engine_factory=factory) as srv:
         first = srv.submit([101])
         time.sleep(0.01)  # the slot is now busy with `first`
         reqs = [srv.submit([103 + 2 * i, 107 + 2 * i]) for i in range(40)]
@@ -61,12 +62,35 @@ def test_requests_batched_in_order_and_routed():
 
 
 def test_hbm_budget_limits_batches():
-    with serve.Server(slots=2, hbm_budget=2 * 5000, max_primes=10**6,
+    with serve.Server(slots=2, hbm_budget=2 * 5000, max_primes=10**6, small=0,
                       engine_factory=lambda: FakeEngine(delay=0.02)) as srv:
         reqs = [srv.submit([3, 5, 7]) for _ in range(10)]  # 3000 bytes each
         for r in reqs:
             r.wait(5)
         assert all(b[2] <= 3 for b in srv.batches)  # one request (3 primes) per batch
+
+
+def test_small_requests_first_and_chunked_intervals():
+    engs = []
+
+    def factory():
+        engs.append(FakeEngine(delay=0.05))
+        return engs[-1]
+
+    with serve.Server(slots=1, hbm_budget=10**9, max_primes=8, chunk=4, small=2,
+                      engine_factory=factory) as srv:
+        first = srv.submit([3])
+        time.sleep(0.01)
+        big = srv.submit(list(range(1000, 1020)))  # 20 primes -> 5 pieces of 4
+        single = srv.submit([7])
+        assert [r["p"] for r in big.wait(5)] == list(range(1000, 1020))
+        single.wait(5)
+        first.wait(5)
+    calls = engs[0].calls
+    assert calls[0] == [3]
+    assert calls[1][0] == 7  # the later single jumps ahead of the interval pieces
+    assert all(len(c) <= 8 for c in calls)
+    assert single.done < big.done
 
 
 def test_refusal_is_per_batch_and_visible():
