@@ -4,9 +4,9 @@ Requests (a single prime or an interval of consecutive primes) arrive at any
 time.  Each of `slots` worker threads owns one library context: a HIP stream
 and its resident device workspaces (root tables, label tables, colour planes,
 inverses and run keys stay allocated between batches).  A request is cut
-into pieces of at most `chunk` primes.  Whenever a slot is free it admits,
-first, every waiting small request (at most `small` primes, in arrival
-order), then bulk pieces in arrival order, while the batch stays within
+into pieces of at most `chunk` primes.  Whenever a slot is free it admits
+either every waiting small request (at most `small` primes each: an express
+batch) or, when none waits, bulk pieces in arrival order, while the batch stays within
 `max_primes` primes and within the slot's share of the HBM budget (estimated
 with the library's own workspace model, `rbg_estimate_bytes`); pieces that
 arrive while batches run join the next free slot.  Short batches and the
@@ -64,7 +64,7 @@ class _Piece:
 
 
 class Server:
-    def __init__(self, slots=2, hbm_budget=64 << 30, max_primes=512, chunk=64, small=16,
+    def __init__(self, slots=2, hbm_budget=64 << 30, max_primes=256, chunk=64, small=16,
                  method=METHOD_BATCHED, engine_factory=None, clock=time.perf_counter):
         self.clock = clock
         self.method = method
@@ -104,18 +104,19 @@ class Server:
         return req
 
     def _take(self):
-        """Admit small requests first, then bulk pieces, in arrival order, while
-        the batch fits (caller holds cond)."""
+        """Admit, in arrival order, either the waiting small requests (an
+        express batch) or bulk pieces, while the batch fits (caller holds
+        cond)."""
         batch, primes, nbytes = [], 0, 0
-        for q in (self.small_queue, self.queue):
-            while q:
-                piece = q[0]
-                if batch and (primes + len(piece.primes) > self.max_primes
-                              or nbytes + piece.bytes > self.slot_budget):
-                    return batch
-                batch.append(q.popleft())
-                primes += len(piece.primes)
-                nbytes += piece.bytes
+        q = self.small_queue if self.small_queue else self.queue
+        while q:
+            piece = q[0]
+            if batch and (primes + len(piece.primes) > self.max_primes
+                          or nbytes + piece.bytes > self.slot_budget):
+                break
+            batch.append(q.popleft())
+            primes += len(piece.primes)
+            nbytes += piece.bytes
         return batch
 
     def _worker(self, slot):

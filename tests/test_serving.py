@@ -88,7 +88,7 @@ def test_small_requests_first_and_chunked_intervals():
         first.wait(5)
     calls = engs[0].calls
     assert calls[0] == [3]
-    assert calls[1][0] == 7  # the later single jumps ahead of the interval pieces
+    assert calls[1] == [7]  # the later single gets an express batch ahead of the pieces
     assert all(len(c) <= 8 for c in calls)
     assert single.done < big.done
 
