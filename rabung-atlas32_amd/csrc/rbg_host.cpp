@@ -61,8 +61,9 @@ extern "C" hipError_t rbg_sparse_filter(const SparsePrime *P, const SparseProf *
                                         uint32_t nprof, uint32_t dmax, const uint32_t *planes,
                                         const uint32_t *inv, const uint32_t *tab_val,
                                         const uint16_t *tab_lab, unsigned long long *best,
-                                        uint32_t *failed, unsigned long long *stats, uint32_t grid,
-                                        uint32_t group, hipStream_t stream);
+                                        uint32_t *failed, unsigned long long *stats,
+                                        uint32_t *queue, uint32_t grid, uint32_t group,
+                                        hipStream_t stream);
 extern "C" hipError_t rbg_sparse_finalize(const SparsePrime *P, uint32_t nprimes,
                                           const SparseProf *prof, const unsigned long long *best,
                                           const uint32_t *a4, rbg_record *out, hipStream_t stream);
@@ -196,6 +197,8 @@ struct DevBuf {
 constexpr u32 kSparseMin = 1u << 16;  // smallest prime sent to the sparse engine
 constexpr u32 kCoverQ = 48;           // threshold cap of the cover
 constexpr u32 kBatchChunk = 16384;    // primes per launch chain of a batch
+// filter statistics (3 x u64) and the filter's per-XCD task counters (8 x u32)
+constexpr size_t kQueueOffset = 32, kStatsBytes = kQueueOffset + 8 * 4;
 
 enum Ev { kAll0, kAll1, kDense0, kDense1, kPrep0, kPrep1, kSeed0, kSeed1, kFilt0, kFilt1, kNumEv };
 
@@ -483,7 +486,7 @@ int sparse_run(rbg_ctx *c, rbg_record *recs, u32 *err, std::vector<u32> &failed_
     HIPCK(c, c->sp_best.ensure(nf * 8));
     HIPCK(c, c->sp_failed.ensure(np * 4));
     HIPCK(c, c->sp_a4.ensure(np * 4));
-    HIPCK(c, c->sp_stats.ensure(3 * 8));
+    HIPCK(c, c->sp_stats.ensure(kStatsBytes));
     HIPCK(c, c->sp_segs.ensure((u64)nf * segs_per_prof * rbg_seed_agg_bytes()));
     hipStream_t s = c->stream;
     // A4 by elliptic point counts on the host while the device works
@@ -506,7 +509,7 @@ int sparse_run(rbg_ctx *c, rbg_record *recs, u32 *err, std::vector<u32> &failed_
     HIPCK(c, hipMemcpyAsync(c->sp_lab.ptr, c->h_slab.data(), c->h_slab.size() * 2,
                             hipMemcpyHostToDevice, s));
     HIPCK(c, hipMemsetAsync(c->sp_failed.ptr, 0, np * 4, s));
-    HIPCK(c, hipMemsetAsync(c->sp_stats.ptr, 0, 3 * 8, s));
+    HIPCK(c, hipMemsetAsync(c->sp_stats.ptr, 0, kStatsBytes, s));
     const SparsePrime *dP = (const SparsePrime *)c->sp_primes.ptr;
     SparseProf *dF = (SparseProf *)c->sp_profs.ptr;
     unsigned long long *best = (unsigned long long *)c->sp_best.ptr;
@@ -529,7 +532,9 @@ int sparse_run(rbg_ctx *c, rbg_record *recs, u32 *err, std::vector<u32> &failed_
     HIPCK(c, rbg_sparse_filter(dP, dF, nf, (u32)dmax, (const u32 *)c->sp_planes.ptr,
                                (const u32 *)c->sp_inv.ptr, (const u32 *)c->sp_val.ptr,
                                (const u16 *)c->sp_lab.ptr, best, failed,
-                               (unsigned long long *)c->sp_stats.ptr, grid, xcd_group(c), s));
+                               (unsigned long long *)c->sp_stats.ptr,
+                               (u32 *)((char *)c->sp_stats.ptr + kQueueOffset), grid,
+                               xcd_group(c), s));
     HIPCK(c, hipEventRecord(c->ev[kFilt1], s));
     c->timing.launches += 6;
     std::vector<u32> h_failed(np, 0);

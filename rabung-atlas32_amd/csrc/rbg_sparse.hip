@@ -223,9 +223,13 @@ __device__ u32 colour_direct(const SparsePrime &s, u32 r, const u32 *tab_val,
 constexpr u32 kExtendLimit = 1u << 22;
 
 // Extend the monochromatic window [x, x+t-1] (numerators -b + i d) to its run.
-__device__ void complete_anchor(const SparsePrime &s, const SparseProf &f, const u32 *pl, u32 b,
-                                u32 d, u32 dinv, u32 t, const u32 *tab_val, const u16 *tab_lab,
-                                unsigned long long *best_slot, u32 *failed) {
+// Rare (a few per task), so kept out of line: the filter loop then holds only
+// the two record pointers, not the records.
+__device__ void complete_anchor(
+    const SparsePrime *sg, const SparseProf *fg, const u32 *pl, u32 b, u32 d, u32 dinv, u32 t,
+    const u32 *tab_val, const u16 *tab_lab, unsigned long long *best_slot, u32 *failed) {
+    const SparsePrime s = *sg;
+    const SparseProf f = *fg;
     const u32 p = s.p;
     // dinv is the Montgomery form d^{-1} R, so REDC(b * dinv) = b / d mod p
     const u32 bd = mont_mul(b, dinv, p, s.ninv);
@@ -297,42 +301,51 @@ constexpr u32 kXcds = 8;  // XCDs of an MI355X
 // b + k d and position x+t-1+k reads b - (t-1+k) d; numerators outside
 // [-B, B] are unknown and never reject.  R is kept as a bit-sliced count
 // (5 planes, h <= 23) so that every candidate of the word is screened at once.
-__device__ __forceinline__ u64 match_left(const u32 *pl, u32 wpp, u32 np, const u64 *base,
-                                          u32 b0, u32 B, u32 d, u32 k, u64 &n_windows) {
+template <int NP>
+__device__ __forceinline__ u64 match_left(const u32 *const *plb, const u64 *base, u32 b0, u32 B,
+                                          u32 d, u32 k, u32 &n_windows) {
     const int64_t lo = (int64_t)b0 + (int64_t)k * d, Bs = B;
     if (lo > Bs) return ~0ull;
     u64 mm = 0;
     const u32 idx = (u32)(lo + Bs);
-    for (u32 pb = 0; pb < np; ++pb) mm |= base[pb] ^ window64(pl + (u64)pb * wpp, idx);
-    n_windows += np;
+#pragma unroll
+    for (int pb = 0; pb < NP; ++pb) mm |= base[pb] ^ window64(plb[pb], idx);
+    n_windows += NP;
     const int64_t known = Bs - lo + 1;  // bits 0..known-1 lie in the table
     if (known < 64) mm &= (1ull << known) - 1ull;
     return ~mm;
 }
 
-__device__ __forceinline__ u64 match_right(const u32 *pl, u32 wpp, u32 np, const u64 *base,
-                                           u32 b0, u32 B, u32 d, u32 t, u32 k, u64 &n_windows) {
+template <int NP>
+__device__ __forceinline__ u64 match_right(const u32 *const *plb, const u64 *base, u32 b0, u32 B,
+                                           u32 d, u32 t, u32 k, u32 &n_windows) {
     const int64_t lo = (int64_t)b0 - (int64_t)(t - 1 + k) * d, Bs = B;
     if (lo + 63 < -Bs) return ~0ull;
     u64 mm = 0;
     if (lo >= -Bs) {
         const u32 idx = (u32)(lo + Bs);
-        for (u32 pb = 0; pb < np; ++pb) mm |= base[pb] ^ window64(pl + (u64)pb * wpp, idx);
+#pragma unroll
+        for (int pb = 0; pb < NP; ++pb) mm |= base[pb] ^ window64(plb[pb], idx);
     } else {
         const u32 shift = (u32)(-Bs - lo);  // 1..63 low bits are below -B: unknown
-        for (u32 pb = 0; pb < np; ++pb) mm |= base[pb] ^ (window64(pl + (u64)pb * wpp, 0) << shift);
+#pragma unroll
+        for (int pb = 0; pb < NP; ++pb) mm |= base[pb] ^ (window64(plb[pb], 0) << shift);
         mm &= ~((1ull << shift) - 1ull);
     }
-    n_windows += np;
+    n_windows += NP;
     return ~mm;
 }
 
-__device__ __forceinline__ u64 screen(const u32 *pl, u32 wpp, u32 np, const u64 *base, u32 b0,
-                                      u32 B, u32 d, u32 t, u32 h, u64 &n_windows) {
+// Screen of the candidates `cand` of the word at b0 (the anchors that passed
+// every stage); candidates outside `cand` are never reported, so the counts
+// follow `cand` only and the loops end as soon as those runs end.
+template <int NP>
+__device__ __forceinline__ u64 screen(const u32 *const *plb, const u64 *base, u64 cand, u32 b0,
+                                      u32 B, u32 d, u32 t, u32 h, u32 &n_windows) {
     // R: consecutive right matches per candidate, bit-sliced (r0 = LSB)
-    u64 r0 = 0, r1 = 0, r2 = 0, r3 = 0, r4 = 0, cr = ~0ull;
+    u64 r0 = 0, r1 = 0, r2 = 0, r3 = 0, r4 = 0, cr = cand;
     for (u32 k = 1; k <= h; ++k) {
-        cr &= match_right(pl, wpp, np, base, b0, B, d, t, k, n_windows);
+        cr &= match_right<NP>(plb, base, b0, B, d, t, k, n_windows);
         if (!cr) break;
         u64 c = cr, x;
         x = r0 & c; r0 ^= c; c = x;
@@ -341,10 +354,10 @@ __device__ __forceinline__ u64 screen(const u32 *pl, u32 wpp, u32 np, const u64 
         x = r3 & c; r3 ^= c; c = x;
         r4 ^= c;
     }
-    u64 pass = 0, cl = ~0ull;
+    u64 pass = 0, cl = cand;
     for (u32 k = 0; k <= h; ++k) {
         if (k) {
-            cl &= match_left(pl, wpp, np, base, b0, B, d, k, n_windows);
+            cl &= match_left<NP>(plb, base, b0, B, d, k, n_windows);
             if (!cl) break;
         }
         // candidates with R >= h - k (compare the sliced count with a constant)
@@ -364,40 +377,11 @@ __device__ __forceinline__ u64 screen(const u32 *pl, u32 wpp, u32 np, const u64 
     return pass;
 }
 
-// Anchor test of one 64-candidate word: for j = 1..t-1 the windows at
-// numerators b - j d must equal the base windows.  Stages are issued in
-// batches (8, 4 or 2 stages for 1, 2 or more planes; all loads of a batch
-// before any wait, the base windows with the first batch), so a word costs
-// one memory round trip per batch instead of one per stage; the batch
-// stops early once no candidate is left.  Stages past t-1 re-read stage t-1
-// (harmless, keeps every address inside the table).
-template <int NP>
-__device__ __forceinline__ u64 anchor_test(const u32 *pl, u32 wpp, u64 *base, u32 ib, u32 d, u32 t,
-                                           u64 valid, u64 &n_windows) {
-    constexpr int G = NP == 1 ? 8 : NP == 2 ? 4 : 2;  // stages per batch (register budget)
-    u64 w[G][NP];
-    // the base windows travel with the first batch: one round trip
-#pragma unroll
-    for (int pb = 0; pb < NP; ++pb) base[pb] = window64(pl + (u64)pb * wpp, ib);
-    u64 mism = 0;
-    for (u32 j0 = 1; j0 < t; j0 += G) {
-#pragma unroll
-        for (int g = 0; g < G; ++g) {
-            const u32 j = min(j0 + (u32)g, t - 1u);
-            const u32 io = ib - j * d;
-#pragma unroll
-            for (int pb = 0; pb < NP; ++pb) w[g][pb] = window64(pl + (u64)pb * wpp, io);
-        }
-#pragma unroll
-        for (int g = 0; g < G; ++g)
-#pragma unroll
-            for (int pb = 0; pb < NP; ++pb) mism |= base[pb] ^ w[g][pb];
-        n_windows += (u64)NP * min((u32)G, t - j0);
-        if ((valid & ~mism) == 0) break;
-    }
-    return valid & ~mism;
+__device__ __forceinline__ u32 lanes_below(u64 m) {
+    return __builtin_amdgcn_mbcnt_hi((u32)(m >> 32), __builtin_amdgcn_mbcnt_lo((u32)m, 0u));
 }
 
+// Per-task context (wave-uniform).
 struct FilterCtx {
     const SparsePrime *s;
     const SparseProf *f;
@@ -406,83 +390,277 @@ struct FilterCtx {
     const u16 *tab_lab;
     unsigned long long *best;
     u32 *failed;
-    u32 d, dinv, t, q, B, bmin;
+    u32 d, dinv, t, q, B, bmin, wpp;
 };
 
-// All scheduled words of one (profile, denominator) task for NP planes.
+// Words of one band kind: anchors b = (n0 + band * step) d + c for c in
+// [c_lo, c_hi], split into words of 64 consecutive b; `total` words.
+struct BandKind {
+    u32 c_lo, c_hi, n0, step, W, total;
+    float rW;
+};
+
+// Band layout: kind 0 = c in [1, ca] (2c < d, n = phi mod s), kind 1 =
+// c in [cb, d-1] (2c > d, n = t-2-phi mod s); for d <= 2 one unscheduled band
+// covers [max(bmin, .), B] (reference schedule GPU_METHOD_EN.tex:380-455).
+__device__ __forceinline__ BandKind band_kind(u32 kind, u32 d, u32 t, u32 sp, u32 phi, u32 bmin,
+                                              u32 B) {
+    BandKind k;
+    k.total = 0;
+    k.c_lo = 1;
+    k.c_hi = 0;
+    k.n0 = 0;
+    k.step = 0;
+    k.W = 1;
+    if (d <= 2) {
+        const u32 lo_b = max(bmin, d == 1 ? t : t - 1u);
+        if (kind || lo_b > B) return k;
+        k.c_lo = lo_b;
+        k.c_hi = B;
+        k.W = (B - lo_b + 64u) >> 6;
+        k.total = k.W;
+    } else {
+        const u32 ca = (d - 1u) / 2u, cb = d / 2u + 1u;
+        k.c_lo = kind ? cb : 1u;
+        k.c_hi = kind ? d - 1u : ca;
+        if (k.c_lo > k.c_hi) return k;
+        const u32 res = kind ? (t - 2u - phi) % sp : phi;
+        const u32 n_lo = bmin / d, n_hi = B / d;
+        k.n0 = n_lo + (res + sp - n_lo % sp) % sp;
+        const u32 nb = k.n0 > n_hi ? 0u : (n_hi - k.n0) / sp + 1u;
+        k.step = sp;
+        k.W = (k.c_hi - k.c_lo + 64u) >> 6;
+        k.total = nb * k.W;  // < 2^23: at most B/64 + nb words
+    }
+    k.rW = 1.0f / (float)k.W;
+    return k;
+}
+
+// Screen and complete the queued words (one lane per word).
 template <int NP>
-__device__ void filter_item(const FilterCtx &c, u32 sp, u32 phi, u64 &n_windows, u64 &n_words,
-                            u64 &n_done) {
+__device__ __forceinline__ void drain_queue(const FilterCtx &c, const u32 *const *plb, u32 qn, const u32 *qb,
+                            const u64 *qs, u32 lane, u32 &n_windows, u32 &n_done) {
+    if (lane >= qn) return;
+    const u32 b0 = qb[lane];
+    u64 surv = qs[lane];
+    if (c.q > c.t) {
+        u64 base[NP];
+#pragma unroll
+        for (int pb = 0; pb < NP; ++pb) base[pb] = window64(plb[pb], b0 + c.B);
+        surv &= screen<NP>(plb, base, surv, b0, c.B, c.d, c.t, c.q - c.t, n_windows);
+    }
+    while (surv) {
+        const u32 bit = (u32)__builtin_ctzll(surv);
+        surv &= surv - 1ull;
+        complete_anchor(c.s, c.f, c.pl, b0 + bit, c.d, c.dinv, c.t, c.tab_val, c.tab_lab,
+                        c.best, c.failed);
+        ++n_done;
+    }
+}
+// 64-bit window from the three words at its first bit (shift sh = bit & 31).
+__device__ __forceinline__ u64 window_of(u32 a, u32 b, u32 c, u32 sh) {
+    return ((u64)__builtin_amdgcn_alignbit(c, b, sh) << 32) | __builtin_amdgcn_alignbit(b, a, sh);
+}
+
+__device__ __forceinline__ void wave_lds_fence() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Every scheduled word of one (profile, denominator) task for NP planes.
+//
+// Each lane holds one word (64 consecutive anchors b) and tests it G stages
+// per step: for stages j the 64-bit windows at numerators b - j d must equal
+// the base window at b.  The base words of a new word are loaded when the
+// lane takes it and aligned only after the step's stage loads are issued,
+// so taking a word costs no extra memory round trip.  A lane whose word
+// dies or passes takes the next word at once (words are handed out in order
+// with a ballot), so the lanes stay busy however differently the words end.
+// Passing words (anchors that survived every stage) go to a per-wave LDS
+// queue of 64 entries; the queue is screened and completed with all lanes at
+// once outside the word loop, at the end of the task or when it is full (the
+// lanes whose words did not fit keep them and the word loop stops handing
+// out words until the queue has been drained).  Stages past t-1 re-read
+// stage t-1, which keeps every address inside the table.
+template <int NP>
+__device__ void filter_task(const FilterCtx &c, u32 sp, u32 phi, u32 *qb, u64 *qs, u32 *kinds,
+                            u32 &n_windows, u32 &n_words, u32 &n_done) {
+    constexpr int G = NP == 1 ? 8 : NP == 2 ? 4 : 2;  // stages per step (register budget)
     const u32 lane = threadIdx.x & 63u;
-    const u32 d = c.d, t = c.t, B = c.B, bmin = c.bmin, wpp = c.f->wpp;
-    // band layout: kind 0 = c in [1, ca] (2c < d), kind 1 = c in [cb, d-1]
-    // (2c > d); for d <= 2 one unscheduled band covers [max(bmin, .), B].
-    for (u32 kind = 0; kind < 2; ++kind) {
-        u32 c_lo, c_hi, n0, nb, step;
-        if (d <= 2) {
-            if (kind) break;
-            const u32 lo_b = max(bmin, d == 1 ? t : t - 1u);
-            if (lo_b > B) break;
-            c_lo = lo_b;
-            c_hi = B;
-            n0 = 0;
-            nb = 1;
-            step = 0;
-        } else {
-            const u32 ca = (d - 1u) / 2u, cb = d / 2u + 1u;
-            c_lo = kind ? cb : 1u;
-            c_hi = kind ? d - 1u : ca;
-            if (c_lo > c_hi) continue;
-            const u32 res = kind ? (t - 2u - phi) % sp : phi;
-            const u32 n_lo = bmin / d, n_hi = B / d;
-            n0 = n_lo + (res + sp - n_lo % sp) % sp;
-            nb = n0 > n_hi ? 0u : (n_hi - n0) / sp + 1u;
-            step = sp;
+    const u32 d = c.d, t = c.t, B = c.B, bmin = c.bmin;
+    // the two band kinds go to the wave's LDS slot: a lane reads the record
+    // of its word's kind by address (no per-lane selects, no scalar registers
+    // held over the loop)
+    u32 total0, total;
+    {
+        const BandKind k0 = band_kind(0, d, t, sp, phi, bmin, B);
+        const BandKind k1 = band_kind(1, d, t, sp, phi, bmin, B);
+        total0 = k0.total;
+        total = k0.total + k1.total;
+        if (lane < 2) {
+            const BandKind &k = lane ? k1 : k0;
+            kinds[lane * 8 + 0] = k.c_lo;
+            kinds[lane * 8 + 1] = k.c_hi;
+            kinds[lane * 8 + 2] = k.n0;
+            kinds[lane * 8 + 3] = k.step;
+            kinds[lane * 8 + 4] = k.W;
+            kinds[lane * 8 + 5] = __float_as_uint(k.rW);
+            kinds[lane * 8 + 6] = lane ? k0.total : 0u;  // first word of the kind
         }
-        const u32 width = c_hi - c_lo + 1u;
-        const u32 W = (width + 63u) >> 6;
-        const u32 total = nb * W;  // < 2^32: at most B/64 + nb words
-        // idx -> (band, word) with a float reciprocal and an exact fix-up
-        const float rW = 1.0f / (float)W;
-        for (u32 base_idx = 0; base_idx < total; base_idx += 64u) {
-            const u32 idx = base_idx + lane;
-            if (idx >= total) continue;
-            u32 band = (u32)((float)idx * rW);
-            while (band * W > idx) --band;
-            while ((band + 1u) * W <= idx) ++band;
-            const u32 w = idx - band * W;
-            const u32 nd = (n0 + band * step) * d;  // <= B + d < 2^32
-            const u32 b0 = nd + c_lo + 64u * w;
-            const u32 bend = min(nd + c_hi, B);
-            if (b0 > bend) continue;
-            const u32 nbits = min(64u, bend - b0 + 1u);
-            u64 valid = nbits == 64u ? ~0ull : ((1ull << nbits) - 1ull);
-            if (b0 < bmin) {
-                const u32 below = bmin - b0;
-                if (below >= nbits) continue;
-                valid &= ~0ull << below;
+        wave_lds_fence();
+    }
+    const u32 *plb[NP];
+#pragma unroll
+    for (int pb = 0; pb < NP; ++pb) plb[pb] = c.pl + (u32)pb * c.wpp;
+    const bool even = (d & 1u) == 0, thirds = d % 3u == 0;
+    u32 next = 0, qn = 0;
+    bool pend = false;  // a passing word that did not fit the queue
+    u32 b0 = 0;
+    u64 live = 0;
+    for (;;) {
+        bool stall = false;
+        bool have = false;
+        u32 j = 1, sh = 0;
+        int io = 0, io_last = 0;  // table index of stage j and of stage t-1
+        u32 braw[NP][3];
+#pragma unroll
+        for (int pb = 0; pb < NP; ++pb) braw[pb][0] = braw[pb][1] = braw[pb][2] = 0;
+        for (;;) {
+            const u64 idle = ballot64(!have && !pend);
+            if (!stall && idle && next < total) {
+                if (!have && !pend) {
+                    const u32 my = next + lanes_below(idle);
+                    if (my < total) {
+                        const u32 *kp = kinds + (my >= total0 ? 8u : 0u);
+                        const u32 wi = my - kp[6];
+                        const u32 W = kp[4];
+                        // exact quotient: the float estimate is at most one
+                        // low (|error| < 1/W because total < 2^23)
+                        u32 band = (u32)((float)wi * __uint_as_float(kp[5]));
+                        if ((band + 1u) * W <= wi) ++band;
+                        const u32 w = wi - band * W;
+                        const u32 nd = (kp[2] + band * kp[3]) * d;
+                        b0 = nd + kp[0] + 64u * w;
+                        const u32 bend = min(nd + kp[1], B);
+                        u64 valid = 0;
+                        if (b0 <= bend) {
+                            const u32 nbits = min(64u, bend - b0 + 1u);
+                            valid = nbits == 64u ? ~0ull : ((1ull << nbits) - 1ull);
+                            if (b0 < bmin) {
+                                const u32 below = bmin - b0;
+                                valid = below >= nbits ? 0ull : valid & (~0ull << below);
+                            }
+                            if (even)
+                                valid &= (b0 & 1u) ? 0x5555555555555555ull : 0xAAAAAAAAAAAAAAAAull;
+                            if (thirds) valid &= ~(kThirds << ((3u - b0 % 3u) % 3u));
+                        }
+                        if (valid) {
+                            have = true;
+                            live = valid;
+                            j = 1;
+                            const u32 ib = b0 + B;
+                            io = (int)ib - (int)d;
+                            io_last = (int)ib - (int)((t - 1u) * d);
+                            sh = ib & 31u;
+                            const u32 wo = ib >> 5;
+#pragma unroll
+                            for (int pb = 0; pb < NP; ++pb) {
+                                braw[pb][0] = plb[pb][wo];
+                                braw[pb][1] = plb[pb][wo + 1];
+                                braw[pb][2] = plb[pb][wo + 2];
+                            }
+                            ++n_words;
+                        }
+                    }
+                }
+                next += (u32)__builtin_popcountll(idle);
             }
-            if ((d & 1u) == 0) valid &= (b0 & 1u) ? 0x5555555555555555ull : 0xAAAAAAAAAAAAAAAAull;
-            if (d % 3u == 0) valid &= ~(kThirds << ((3u - b0 % 3u) % 3u));
-            if (!valid) continue;
-            const u32 ib = b0 + B;
-            ++n_words;
-            u64 base[NP];
-            u64 surv = anchor_test<NP>(c.pl, wpp, base, ib, d, t, valid, n_windows);
-            if (surv && c.q > t)
-                surv &= screen(c.pl, wpp, NP, base, b0, B, d, t, c.q - t, n_windows);
-            while (surv) {
-                const u32 bit = (u32)__builtin_ctzll(surv);
-                surv &= surv - 1ull;
-                complete_anchor(*c.s, *c.f, c.pl, b0 + bit, d, c.dinv, t, c.tab_val, c.tab_lab,
-                                c.best, c.failed);
-                ++n_done;
+            const u64 busy = ballot64(have);
+            if (!busy) {
+                if (stall || next >= total) break;
+                continue;
+            }
+            bool fin = false;
+            if (have) {
+                u64 w[G][NP];
+#pragma unroll
+                for (int g = 0; g < G; ++g) {
+                    const u32 ig = (u32)max(io - g * (int)d, io_last);
+#pragma unroll
+                    for (int pb = 0; pb < NP; ++pb) w[g][pb] = window64(plb[pb], ig);
+                }
+                u64 mism = 0;
+#pragma unroll
+                for (int pb = 0; pb < NP; ++pb) {
+                    const u64 base = window_of(braw[pb][0], braw[pb][1], braw[pb][2], sh);
+#pragma unroll
+                    for (int g = 0; g < G; ++g) mism |= base ^ w[g][pb];
+                }
+                n_windows += (u32)NP * min((u32)G, t - j);
+                live &= ~mism;
+                j += G;
+                io -= G * (int)d;
+                if (!live) {
+                    have = false;
+                } else if (j >= t) {
+                    have = false;
+                    fin = true;
+                }
+            }
+            const u64 finm = ballot64(fin);
+            if (finm) {
+                if (fin) {
+                    const u32 slot = qn + lanes_below(finm);
+                    if (slot < 64u) {
+                        qb[slot] = b0;
+                        qs[slot] = live;
+                    } else {
+                        pend = true;
+                    }
+                }
+                const u32 n = (u32)__builtin_popcountll(finm);
+                if (qn + n > 64u) stall = true;
+                qn = min(qn + n, 64u);
             }
         }
+        // no word in flight: screen and complete the queue, then requeue the
+        // lanes that were held back
+        wave_lds_fence();
+        if (qn) drain_queue<NP>(c, plb, qn, qb, qs, lane, n_windows, n_done);
+        wave_lds_fence();
+        const u64 pm = ballot64(pend);
+        qn = 0;
+        if (pm) {
+            if (pend) {
+                const u32 slot = lanes_below(pm);
+                qb[slot] = b0;
+                qs[slot] = live;
+                pend = false;
+            }
+            qn = (u32)__builtin_popcountll(pm);
+            wave_lds_fence();
+            continue;
+        }
+        if (next >= total) break;
     }
 }
 
-__global__ __launch_bounds__(256) void sp_filter(const SparsePrime *__restrict__ P,
+constexpr u32 kFilterWaves = 4;  // waves per filter block (blockDim 256)
+
+constexpr u32 kChunk = 16;  // filter tasks claimed per queue access
+
+// Next task of the wave: the rest of its chunk, then the chunk claimed
+// during the chunk's last task (`claim` is valid in lane 0).
+__device__ __forceinline__ u32 next_task(u32 &k, u32 &chunk, u32 claim) {
+    if (++k == kChunk) {
+        chunk = __builtin_amdgcn_readfirstlane(claim);
+        k = 0;
+    }
+    return chunk + k;
+}
+
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void sp_filter(const SparsePrime *__restrict__ P,
                                                  const SparseProf *__restrict__ prof, u32 nprof,
                                                  u32 dmax, const u32 *__restrict__ planes,
                                                  const u32 *__restrict__ inv,
@@ -491,64 +669,82 @@ __global__ __launch_bounds__(256) void sp_filter(const SparsePrime *__restrict__
                                                  unsigned long long *__restrict__ best,
                                                  u32 *__restrict__ failed,
                                                  unsigned long long *__restrict__ stats,
-                                                 u32 kGroup) {
+                                                 u32 *__restrict__ queue, u32 kGroup) {
     // per-lane work counts: 64-bit window comparisons, candidate words,
     // completed anchors (summed per block, three atomics per block)
     __shared__ unsigned long long s_stats[3];
+    // per-wave queue of passing words: start b0 and surviving anchors
+    __shared__ u32 s_qb[kFilterWaves][64];
+    __shared__ u64 s_qs[kFilterWaves][64];
+    __shared__ u32 s_kinds[kFilterWaves][16];
     if (threadIdx.x < 3) s_stats[threadIdx.x] = 0;
     __syncthreads();
     u64 n_windows = 0, n_words = 0, n_done = 0;
+    u32 *qb = s_qb[threadIdx.x >> 6];
+    u64 *qs = s_qs[threadIdx.x >> 6];
+    u32 *kinds = s_kinds[threadIdx.x >> 6];
     // XCD-aware schedule: workgroups are dispatched round-robin over the 8
     // XCDs, so block b runs on XCD b % 8.  XCD x owns the profiles
     // k = x, x+8, x+16, ... and walks them in groups of kGroup profiles (a
-    // tuning knob, 8 by default),
-    // denominator-major inside a group: the waves of one XCD then touch the
-    // planes of at most kGroup profiles at a time (they stay in its L2),
-    // while each profile's denominators are still spread over the group.
+    // tuning knob), denominator-major inside a group: the waves of one XCD
+    // then touch the planes of at most kGroup profiles at a time (they stay
+    // in its L2), while each profile's denominators are still spread over
+    // the group.  Tasks are taken from a per-XCD counter (a device work
+    // queue, zeroed before the launch) in chunks of kChunk consecutive
+    // tasks, so the tasks in flight on an XCD stay within a short window
+    // however many waves are resident or queued; the next chunk is claimed
+    // while the last task of the current one runs.
     const u32 xcd = blockIdx.x % kXcds;
-    const u32 wpb = blockDim.x >> 6;
-    const u32 wl = (blockIdx.x / kXcds) * wpb + (threadIdx.x >> 6);
-    const u32 waves_x = (gridDim.x / kXcds) * wpb;  // grid is a multiple of 8
+    const u32 lane = threadIdx.x & 63u;
     const u32 nprof_x = nprof > xcd ? (nprof - xcd + kXcds - 1) / kXcds : 0u;
     const u32 ngroups = (nprof_x + kGroup - 1) / kGroup;
     const u32 items = ngroups * kGroup * dmax;  // checked below 2^32 by the host
-    for (u32 item = wl; item < items; item += waves_x) {
+    u32 claim = 0;
+    if (lane == 0) claim = atomicAdd(&queue[xcd], kChunk);
+    u32 chunk = __builtin_amdgcn_readfirstlane(claim), k = 0;
+    for (u32 item = chunk; item < items; item = next_task(k, chunk, claim)) {
+        if (k == kChunk - 1u && lane == 0) claim = atomicAdd(&queue[xcd], kChunk);
         const u32 g = item / (kGroup * dmax);
         const u32 rem = item - g * (kGroup * dmax);
         const u32 dd = __builtin_amdgcn_readfirstlane(1u + rem / kGroup);
         const u32 local = g * kGroup + rem % kGroup;
         if (local >= nprof_x) continue;
         const u32 kk = __builtin_amdgcn_readfirstlane(xcd + local * kXcds);
-        const SparseProf f = prof[kk];
-        const SparsePrime s = P[f.prime];
-        if (dd > s.D) continue;
+        const SparseProf *fg = prof + kk;
+        const SparsePrime *sg = P + fg->prime;
+        if (dd > sg->D) continue;
         const u64 key = __hip_atomic_load(&best[kk], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        const u32 q = min((u32)(key >> 32), s.Q);
+        const u32 q = min((u32)(key >> 32), sg->Q);
         if (q < 3u) continue;
         u32 t = max(3u, (q + 1u) / 2u);
         if (t > q) t = q;
         const u32 sp = q - t + 1u;  // schedule period
         const u32 phi = ((t - 1u) / 2u) % sp;
         FilterCtx c;
-        c.s = &s;
-        c.f = &f;
-        c.pl = planes + f.plane_off;
+        c.s = sg;
+        c.f = fg;
+        c.pl = planes + fg->plane_off;
+        c.wpp = fg->wpp;
         c.tab_val = tab_val;
         c.tab_lab = tab_lab;
         c.best = &best[kk];
         c.failed = failed;
         c.d = dd;
-        c.dinv = inv[(u64)s.inv_off + dd];
+        c.dinv = inv[(u64)sg->inv_off + dd];
         c.t = t;
         c.q = q;
-        c.B = s.B;
+        c.B = sg->B;
         c.bmin = ((t - 1u) * dd + 1u) / 2u;
-        switch (f.np) {
-        case 1: filter_item<1>(c, sp, phi, n_windows, n_words, n_done); break;
-        case 2: filter_item<2>(c, sp, phi, n_windows, n_words, n_done); break;
-        case 3: filter_item<3>(c, sp, phi, n_windows, n_words, n_done); break;
-        default: filter_item<4>(c, sp, phi, n_windows, n_words, n_done); break;
+        u32 tw = 0, tn = 0, td = 0;  // this task's counts (u32 per lane)
+        switch (fg->np) {
+        case 1: filter_task<1>(c, sp, phi, qb, qs, kinds, tw, tn, td); break;
+        case 2: filter_task<2>(c, sp, phi, qb, qs, kinds, tw, tn, td); break;
+        case 3: filter_task<3>(c, sp, phi, qb, qs, kinds, tw, tn, td); break;
+        default: filter_task<4>(c, sp, phi, qb, qs, kinds, tw, tn, td); break;
         }
+        n_windows += tw;
+        n_words += tn;
+        n_done += td;
     }
     atomicAdd(&s_stats[0], (unsigned long long)n_windows);
     atomicAdd(&s_stats[1], (unsigned long long)n_words);
@@ -634,9 +830,10 @@ extern "C" hipError_t rbg_sparse_filter(const SparsePrime *P, const SparseProf *
                                         const uint32_t *inv, const uint32_t *tab_val,
                                         const uint16_t *tab_lab, unsigned long long *best,
                                         uint32_t *failed, unsigned long long *stats,
-                                        uint32_t grid, uint32_t group, hipStream_t stream) {
+                                        uint32_t *queue, uint32_t grid, uint32_t group,
+                                        hipStream_t stream) {
     hipLaunchKernelGGL(rbg::sp_filter, dim3(grid), dim3(256), 0, stream, P, prof, nprof, dmax,
-                       planes, inv, tab_val, tab_lab, best, failed, stats, group);
+                       planes, inv, tab_val, tab_lab, best, failed, stats, queue, group);
     return hipGetLastError();
 }
 
