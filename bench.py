@@ -162,6 +162,33 @@ def cpu_baseline(primes, seconds_budget=30.0):
                           len(sample), ",".join(map(str, sample)), done_profiles, dt)}
 
 
+def calibrate_copy(nbytes=256 << 20, reps=3):
+    """A plain device copy of known size: the reading of rocprof's FETCH_SIZE
+    and WRITE_SIZE for it converts the counters of the filter into bytes."""
+    import torch
+    src = torch.ones(nbytes, dtype=torch.uint8, device="cuda:0")
+    dst = torch.empty_like(src)
+    for _ in range(reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"calibration": "device copy", "bytes_read": nbytes, "bytes_written": nbytes,
+                      "reps": reps}), flush=True)
+    return 0
+
+
+def profiled_traffic(path=ROOT / "profiles" / "sp_filter_top256.md"):
+    """HBM bytes per filter launch on the top-256 workload, as measured by the
+    two rocprof counter runs recorded in the profile summary (FETCH_SIZE and
+    WRITE_SIZE, scaled by the same counters' reading for a device copy)."""
+    try:
+        for line in path.read_text().splitlines():
+            if line.startswith("traffic_bytes_per_launch:"):
+                return int(line.split(":", 1)[1].strip())
+    except OSError:
+        pass
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -173,7 +200,11 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=10.0,
                     help="length of the Poisson serving run (0 skips it)")
     ap.add_argument("--qps", type=float, default=20.0, help="serving arrival rate")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a 256 MiB device copy (rocprof FETCH/WRITE_SIZE calibration)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        return calibrate_copy()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (the path shards across primes with "
               "no exchange; see DESIGN.md); run with --gpus 1", file=sys.stderr)
@@ -255,7 +286,8 @@ def main():
                     "achieved": round(ops / (kern_ms / 1000.0) / 1e12, 3),
                     "peak": round(peak_tops, 3), "unit": "TOP/s",
                     "frac": round(ops / (kern_ms / 1000.0) / 1e12 / peak_tops, 4),
-                    "traffic": None, "avg_launch_ms": round(kern_ms, 3),
+                    "traffic": profiled_traffic() if args.workload == "top256" else None,
+                    "avg_launch_ms": round(kern_ms, 3),
                     "algorithmic_ops_per_launch": ops,
                     "work": {"window_compares": windows, "words": timing["filter_words"],
                              "completions": timing["filter_completions"],
