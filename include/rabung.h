@@ -23,6 +23,7 @@
 #ifndef RABUNG_H
 #define RABUNG_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -160,6 +161,29 @@ int rbg_is_prime(uint32_t p);
 
 /* Library version string. */
 const char *rbg_version(void);
+
+/* ---- pack records (host only; no device needed) ------------------------- */
+
+/* Scan frame of n records, ascending primes in [lo, hi) (lo >= 1): the
+ * reference's pack writer, docs/DATA_FORMAT.md:9-31 (decoder
+ * scripts/atlas_codec.py:20-44), byte-identical for the same zlib level.
+ * witnesses: write the witness varints (flag 1).  out: caller buffer,
+ * *out_len in = capacity, out = frame length; RBG_ECAPACITY (with the needed
+ * length in *out_len) if the buffer is short, RBG_EINVAL if a record fails
+ * the decoder's guards (mask, 1 <= L <= M <= (p-1)/r, 1 <= W <= (p-1)/2, K). */
+int rbg_encode_scan_frame(const rbg_record *recs, uint32_t n, uint64_t lo, uint64_t hi,
+                          int witnesses, int level, uint8_t *out, size_t *out_len);
+
+/* A4 frame of the same records (DATA_FORMAT.md:35-47, decoder
+ * atlas_codec.py:54-60): A4 values as u32 LE, with the block's prime hash and
+ * profile science hash in the header.  Same buffer convention. */
+int rbg_encode_a4_frame(const rbg_record *recs, uint32_t n, uint64_t lo, uint64_t hi, int level,
+                        uint8_t *out, size_t *out_len);
+
+/* Commitments of a block (atlas_codec.py:46-51 and the prime hash of
+ * DATA_FORMAT.md): 32-byte SHA-256 digests. */
+int rbg_science_hash(const rbg_record *recs, uint32_t n, uint8_t out[32]);
+int rbg_prime_hash(const rbg_record *recs, uint32_t n, uint8_t out[32]);
 
 /* ---- reference bridge (baseline_bridge.cu:7,21-27) ---------------------- */
 

@@ -46,7 +46,7 @@ def build(debug=False):
         objs.append(obj)
     tmp = OUT.with_suffix(".so.tmp")
     run([hipcc, "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", str(tmp)]
-        + [str(o) for o in objs] + ["-lpthread"])
+        + [str(o) for o in objs] + ["-lpthread", "-lz", "-lcrypto"])
     os.replace(tmp, OUT)
     worker_src = CSRC / "worker" / "rbg_worker.cpp"
     if worker_src.exists():

@@ -57,7 +57,9 @@ class Config(ctypes.Structure):
 EXPORTS = ["rbg_open", "rbg_profile_block", "rbg_profile_block_device", "rbg_profile_batch",
            "rbg_estimate_bytes", "rbg_autotune", "rbg_last_error",
            "rbg_last_timing", "rbg_device_info", "rbg_close", "rbg_mask", "rbg_is_prime",
-           "rbg_version", "rbg_a4_host", "rbg_primes_from", "baseline_init", "baseline_profile"]
+           "rbg_version", "rbg_a4_host", "rbg_primes_from", "rbg_encode_scan_frame",
+           "rbg_encode_a4_frame", "rbg_science_hash", "rbg_prime_hash", "baseline_init",
+           "baseline_profile"]
 
 
 class RabungError(RuntimeError):
@@ -121,6 +123,15 @@ def load():
     lib.baseline_profile.argtypes = [ctypes.c_uint, ctypes.c_uint, P(ctypes.c_uint),
                                      P(ctypes.c_uint), ctypes.c_int]
     lib.baseline_profile.restype = ctypes.c_int
+    for name in ("rbg_encode_scan_frame", "rbg_encode_a4_frame"):
+        fn = getattr(lib, name)
+        fn.argtypes = [P(Record), ctypes.c_uint32, ctypes.c_uint64, ctypes.c_uint64]
+        fn.argtypes += [ctypes.c_int] if name == "rbg_encode_scan_frame" else []
+        fn.argtypes += [ctypes.c_int, ctypes.c_void_p, P(ctypes.c_size_t)]
+        fn.restype = ctypes.c_int
+    for name in ("rbg_science_hash", "rbg_prime_hash"):
+        getattr(lib, name).argtypes = [P(Record), ctypes.c_uint32, ctypes.c_char_p]
+        getattr(lib, name).restype = ctypes.c_int
     _lib = lib
     return lib
 
@@ -128,6 +139,54 @@ def load():
 def record_to_dict(rec):
     return {"p": rec.p, "mask": rec.mask, "M": list(rec.M), "L": list(rec.L),
             "W": list(rec.W), "K": list(rec.K), "A4": rec.A4}
+
+
+def dict_to_record(d):
+    rec = Record()
+    rec.p, rec.mask, rec.A4 = d["p"], d["mask"], d.get("A4", 0)
+    for k in range(8):
+        rec.M[k], rec.L[k], rec.W[k], rec.K[k] = d["M"][k], d["L"][k], d["W"][k], d["K"][k]
+    return rec
+
+
+def _records_array(records):
+    arr = (Record * max(1, len(records)))()
+    for i, d in enumerate(records):
+        arr[i] = d if isinstance(d, Record) else dict_to_record(d)
+    return arr
+
+
+def _frame(call, *args):
+    size = ctypes.c_size_t(0)
+    rc = call(*args, None, ctypes.byref(size))
+    if rc not in (0, 4):
+        raise RabungError(rc, "frame refused")
+    buf = ctypes.create_string_buffer(size.value)
+    rc = call(*args, buf, ctypes.byref(size))
+    if rc:
+        raise RabungError(rc, "frame refused")
+    return buf.raw[:size.value]
+
+
+def encode_scan_frame(records, lo, hi, witnesses=True, level=3):
+    """Scan frame bytes of library records (dicts or Record), written by the
+    library's own encoder (rbg_encode_scan_frame)."""
+    lib = load()
+    return _frame(lib.rbg_encode_scan_frame, _records_array(records), len(records), lo, hi,
+                  1 if witnesses else 0, level)
+
+
+def encode_a4_frame(records, lo, hi, level=3):
+    lib = load()
+    return _frame(lib.rbg_encode_a4_frame, _records_array(records), len(records), lo, hi, level)
+
+
+def science_hash(records):
+    out = ctypes.create_string_buffer(32)
+    rc = load().rbg_science_hash(_records_array(records), len(records), out)
+    if rc:
+        raise RabungError(rc, "science hash")
+    return out.raw
 
 
 class Engine:

@@ -75,3 +75,50 @@ def test_chains_are_order_sensitive():
     a = codec.chain_scan(codec.chain_scan(z, ph0, sh0, e0["lo"], e0["hi"]), ph1, sh1, e1["lo"], e1["hi"])
     b = codec.chain_scan(codec.chain_scan(z, ph1, sh1, e1["lo"], e1["hi"]), ph0, sh0, e0["lo"], e0["hi"])
     assert a != b and len(a) == 32
+
+
+def _records(rows, values):
+    recs = []
+    for row, a4 in zip(rows, values):
+        rec = {"p": row["p"], "mask": codec.admissible_mask(row["p"]), "M": [0] * 8,
+               "L": [0] * 8, "W": [0] * 8, "K": [0] * 8, "A4": a4}
+        for v in row["profiles"]:
+            k = v["r"] - 2
+            rec["M"][k], rec["L"][k], rec["W"][k], rec["K"][k] = (v["max"], v["prefix"],
+                                                                  v["witness"], v["K"])
+        recs.append(rec)
+    return recs
+
+
+@pytest.mark.parametrize("block", [0, 49900])
+def test_library_encoder_byte_identical(block):
+    """The C-ABI writer (rbg_encode_*_frame) reproduces the sample frames."""
+    import rabung_amd
+    rows, values, entry = golden_data.sample_block(block)
+    recs = _records(rows, values)
+    scan = rabung_amd.encode_scan_frame(recs, entry["lo"], entry["hi"], True,
+                                        entry["zlib_level_scan"])
+    assert scan == (golden_data.GOLDEN / entry["scan_file"]).read_bytes()
+    a4 = rabung_amd.encode_a4_frame(recs, entry["lo"], entry["hi"], entry["zlib_level_a4"])
+    assert hashlib.sha256(a4).hexdigest() == entry["a4_sha256"]
+    assert rabung_amd.science_hash(recs).hex() == entry["science_hash"]
+    nowit = rabung_amd.encode_scan_frame(recs, entry["lo"], entry["hi"], False, 6)
+    assert nowit == codec.encode_scan_frame(rows, entry["lo"], entry["hi"], False, 6)
+
+
+def test_library_encoder_refuses_bad_records():
+    import rabung_amd
+    rows, values, entry = golden_data.sample_block(0)
+    recs = _records(rows[:40], values[:40])
+    lo, hi = entry["lo"], entry["hi"]
+    bad = [dict(r, M=list(r["M"]), L=list(r["L"])) for r in recs]
+    bad[5]["L"][0] = bad[5]["M"][0] + 1  # L > M
+    for args in [(bad, lo, hi), (recs[::-1], lo, hi), (recs, recs[3]["p"], hi),
+                 (recs, lo, recs[-1]["p"])]:
+        with pytest.raises(rabung_amd.RabungError) as e:
+            rabung_amd.encode_scan_frame(*args)
+        assert e.value.status == 1
+    worse = [dict(r) for r in recs]
+    worse[7] = dict(worse[7], A4=worse[7]["p"])  # beyond (p-3)/2
+    with pytest.raises(rabung_amd.RabungError):
+        rabung_amd.encode_a4_frame(worse, lo, hi)
