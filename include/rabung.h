@@ -148,6 +148,14 @@ void rbg_close(rbg_ctx *ctx);
  * non-prime and RBG_ECERTIFY if a count fails its congruence checks. */
 int rbg_a4_host(const uint32_t *primes, uint32_t n, uint32_t *out, uint32_t threads);
 
+/* A4 of one prime with the evidence of its point count, as JSON in the
+ * caller's buffer (same length convention as the frame writers): curve
+ * order, Hasse window, lcm of the point orders found on the curve and its
+ * twist, twist parameter, points drawn (*attempts).  Serves the audit
+ * certificate of the moment worker's protocol, moment_worker.cpp:43-45
+ * (fields after elliptic4_mestre.cpp:68-73; the point list is not kept). */
+int rbg_a4_certificate(uint32_t p, uint32_t *a4, uint64_t *attempts, char *out, size_t *out_len);
+
 /* Admissibility mask of p (bit r-2 iff r | p-1); 0 for p < 3. */
 uint32_t rbg_mask(uint32_t p);
 

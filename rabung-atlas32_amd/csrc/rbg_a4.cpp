@@ -16,6 +16,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <cstdio>
 #include <cstring>
 #include <thread>
 #include <unordered_map>
@@ -164,8 +165,15 @@ u64 gcd64(u64 a, u64 b) {
     return a;
 }
 
+// Evidence of one point count: the Hasse window [lower, upper], the lcm of the
+// orders of the random points found on the curve and on its quadratic twist
+// (parameter `twist`, a non-residue), and the number of points drawn.
+struct CountInfo {
+    u64 lower = 0, upper = 0, lcm_curve = 1, lcm_twist = 1, attempts = 0, twist = 0;
+};
+
 // #E(F_p) for E: y^2 = x(x-1)(x-9); returns 0 on failure.
-u64 curve_order(u64 p) {
+u64 curve_order(u64 p, CountInfo *info = nullptr) {
     const u64 R = isqrt64(4 * p), L = p + 1 - R, H = p + 1 + R, W = H - L;
     u64 m = isqrt64(W + 1);
     if (m * m < W + 1) ++m;
@@ -219,7 +227,17 @@ u64 curve_order(u64 p) {
                 found = N;
                 ++count;
             }
-        if (count == 1) return found;
+        if (count == 1) {
+            if (info) {
+                info->lower = L;
+                info->upper = H;
+                info->lcm_curve = l0;
+                info->lcm_twist = l1;
+                info->attempts = (u64)attempt + 1;
+                info->twist = s;
+            }
+            return found;
+        }
     }
     return 0;
 }
@@ -235,10 +253,11 @@ u32 dense_a4(u32 p) {
 }
 
 // 0 on success; a4 = UINT32_MAX on failure.
-int a4_one(u32 p, u32 *a4) {
+int a4_one(u32 p, u32 *a4, u64 *points = nullptr, CountInfo *info = nullptr) {
     if (p == 2) { *a4 = 0; return 0; }
     if (p < 1000) { *a4 = dense_a4(p); return 0; }
-    const u64 N = curve_order(p);
+    const u64 N = curve_order(p, info);
+    if (points) *points = N;
     if (!N) return 1;
     Field F{p};
     const int64_t bd = 4 * F.chi(2) + 2 * F.chi(6) + 2 * F.chi(3) + 2 * F.chi(p - 1) +
@@ -270,5 +289,38 @@ extern "C" int rbg_a4_host(const uint32_t *primes, uint32_t n, uint32_t *out, ui
     for (auto &th : pool) th.join();
     for (int r : rc)
         if (r) return r;
+    return RBG_OK;
+}
+
+extern "C" int rbg_a4_certificate(uint32_t p, uint32_t *a4, uint64_t *attempts, char *out,
+                                  size_t *out_len) {
+    if (!a4 || !out_len || !rbg_is_prime(p)) return RBG_EINVAL;
+    u64 N = 0;
+    CountInfo info;
+    if (a4_one(p, a4, &N, &info)) return RBG_ECERTIFY;
+    if (attempts) *attempts = info.attempts;
+    char buf[512];
+    int len;
+    if (p < 1000)
+        len = std::snprintf(buf, sizeof buf,
+                            "{\"status\":\"PASS\",\"p\":%u,\"r\":2,\"quadruples\":%u,"
+                            "\"certificate\":\"small_dense\"}",
+                            p, *a4);
+    else
+        len = std::snprintf(
+            buf, sizeof buf,
+            "{\"status\":\"PASS\",\"p\":%u,\"r\":2,\"quadruples\":%u,\"elliptic_points\":%llu,"
+            "\"curve\":\"y^2=x(x-1)(x-9)\",\"hasse_lower\":%llu,\"hasse_upper\":%llu,"
+            "\"order_lcm\":%llu,\"twist_order_lcm\":%llu,\"twist_parameter\":%llu,"
+            "\"attempts\":%llu,\"certificate\":\"unique_lcm_multiple_in_Hasse\"}",
+            p, *a4, (unsigned long long)N, (unsigned long long)info.lower,
+            (unsigned long long)info.upper, (unsigned long long)info.lcm_curve,
+            (unsigned long long)info.lcm_twist, (unsigned long long)info.twist,
+            (unsigned long long)info.attempts);
+    if (len < 0 || (size_t)len >= sizeof buf) return RBG_EINVAL;
+    const size_t cap = *out_len;
+    *out_len = (size_t)len;
+    if (!out || cap < (size_t)len) return RBG_ECAPACITY;
+    std::memcpy(out, buf, (size_t)len);
     return RBG_OK;
 }

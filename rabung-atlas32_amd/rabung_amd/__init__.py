@@ -58,7 +58,8 @@ EXPORTS = ["rbg_open", "rbg_profile_block", "rbg_profile_block_device", "rbg_pro
            "rbg_estimate_bytes", "rbg_autotune", "rbg_last_error",
            "rbg_last_timing", "rbg_device_info", "rbg_close", "rbg_mask", "rbg_is_prime",
            "rbg_version", "rbg_a4_host", "rbg_primes_from", "rbg_encode_scan_frame",
-           "rbg_encode_a4_frame", "rbg_science_hash", "rbg_prime_hash", "baseline_init",
+           "rbg_encode_a4_frame", "rbg_science_hash", "rbg_prime_hash", "rbg_a4_certificate",
+           "baseline_init",
            "baseline_profile"]
 
 
