@@ -40,6 +40,14 @@ extern "C" {
 #define RBG_METHOD_DENSE 0x2u   /* dense scan, one persistent launch over all tiles */
 #define RBG_METHOD_MASK 0xFu
 #define RBG_CERTIFY 0x10u       /* direct powmod check of witness and prefix */
+#define RBG_GRAPH 0x20u         /* record each launch chain (fills, tables, seed,
+                                   filter; dense scans and merge) into a hipGraph
+                                   by stream capture and launch the graph, updating
+                                   the context's cached executable graph in place
+                                   when only launch parameters change; timing then
+                                   reports the sparse chain as the filter stage.  No
+                                   reference counterpart: the reference's optional
+                                   capture, frozen_entry.cu:232-233 */
 
 /* status codes */
 #define RBG_OK 0

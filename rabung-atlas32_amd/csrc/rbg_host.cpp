@@ -45,6 +45,7 @@ extern "C" hipError_t rbg_launch_merge(const PrimeDesc *descs, uint32_t nprimes,
                                        const TileHdr *hdr, const RunSum *runs, rbg_record *out,
                                        uint32_t *err, hipStream_t stream);
 extern "C" int rbg_waves_per_block(void);
+extern "C" hipError_t rbg_fill_u32(uint32_t *p, uint32_t n, uint32_t v, hipStream_t stream);
 extern "C" hipError_t rbg_sparse_prepare(const SparsePrime *P, uint32_t nprimes,
                                          const SparseProf *prof, uint32_t nprof,
                                          uint32_t label_blocks, uint32_t inverse_blocks,
@@ -205,6 +206,10 @@ enum Ev { kAll0, kAll1, kDense0, kDense1, kPrep0, kPrep1, kSeed0, kSeed1, kFilt0
 }  // namespace
 
 struct rbg_ctx {
+    // graph mode of the current call and the cached executable graphs of the
+    // dense chain [0] and the sparse chain [1]
+    bool graph = false;
+    hipGraphExec_t gexec[2] = {nullptr, nullptr};
     int device = 0;
     int cus = 0;
     int clock_khz = 0;
@@ -247,6 +252,43 @@ int hip_fail(rbg_ctx *c, hipError_t e, const char *what) {
     } while (0)
 
 u32 tile_chunks(const rbg_ctx *c) { return c->cfg.tile_chunks ? c->cfg.tile_chunks : 64u; }
+
+// Enqueues a launch chain on the context's stream.  In graph mode the chain
+// is recorded by stream capture and launched as one graph; the cached
+// executable graph of the slot is updated in place when only launch
+// parameters changed and instantiated again otherwise.  The chain must not
+// contain memsets or pageable copies (fills are kernels).
+template <class F>
+int launch_chain(rbg_ctx *c, int slot, F &&enqueue) {
+    if (!c->graph) return enqueue();
+    hipStream_t s = c->stream;
+    HIPCK(c, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    const int rc = enqueue();
+    hipGraph_t g = nullptr;
+    const hipError_t end = hipStreamEndCapture(s, &g);
+    struct GraphGuard {
+        hipGraph_t g;
+        ~GraphGuard() {
+            if (g) (void)hipGraphDestroy(g);
+        }
+    } guard{g};
+    if (rc) return rc;
+    HIPCK(c, end);
+    hipGraphExec_t &x = c->gexec[slot];
+    if (x) {
+        hipGraphNode_t bad = nullptr;
+        hipGraphExecUpdateResult res;
+        if (hipGraphExecUpdate(x, g, &bad, &res) != hipSuccess || res != hipGraphExecUpdateSuccess) {
+            (void)hipGetLastError();
+            (void)hipGraphExecDestroy(x);
+            x = nullptr;
+        }
+    }
+    if (!x) HIPCK(c, hipGraphInstantiate(&x, g, nullptr, nullptr, 0));
+    HIPCK(c, hipGraphLaunch(x, s));
+    ++c->timing.launches;
+    return RBG_OK;
+}
 
 u32 blocks_per_cu(const rbg_ctx *c) { return c->cfg.blocks_per_cu ? c->cfg.blocks_per_cu : 8u; }
 
@@ -362,28 +404,37 @@ int dense_run(rbg_ctx *c, const u32 *primes, const std::vector<u32> &idx, bool p
     HIPCK(c, hipMemcpyAsync(c->tab_code.ptr, c->h_code.data(), c->h_code.size() * 2,
                             hipMemcpyHostToDevice, s));
     u32 *counter = (u32 *)c->flags.ptr;
-    HIPCK(c, hipMemsetAsync(counter, 0, sizeof(u32), s));
     const PrimeDesc *dd = (const PrimeDesc *)c->descs.ptr;
     HIPCK(c, hipEventRecord(c->ev[kDense0], s));
-    if (plain) {
-        for (u32 i = 0; i < np; ++i) {
-            const PrimeDesc &d = c->h_descs[i];
-            HIPCK(c, rbg_launch_scan(dd, np, d.tile_begin, d.tile_begin + d.ntiles, tile_chunks(c),
+    // one chain: clear the tile counter, scan (one launch per prime for the
+    // plain method, one persistent launch otherwise), merge per prime
+    const int rc = launch_chain(c, 0, [&]() -> int {
+        HIPCK(c, rbg_fill_u32(counter, 1, 0u, s));
+        if (plain) {
+            for (u32 i = 0; i < np; ++i) {
+                const PrimeDesc &d = c->h_descs[i];
+                HIPCK(c, rbg_launch_scan(dd, np, d.tile_begin, d.tile_begin + d.ntiles,
+                                         tile_chunks(c), (const u32 *)c->tab_val.ptr,
+                                         (const u16 *)c->tab_code.ptr, (TileHdr *)c->hdr.ptr,
+                                         (RunSum *)c->runs.ptr, counter, err, 0, d.ntiles, s));
+                ++c->timing.launches;
+            }
+        } else {
+            const u32 grid = (u32)std::min<u64>((u64)c->cus * blocks_per_cu(c), tiles);
+            HIPCK(c, rbg_launch_scan(dd, np, 0, (u32)tiles, tile_chunks(c),
                                      (const u32 *)c->tab_val.ptr, (const u16 *)c->tab_code.ptr,
-                                     (TileHdr *)c->hdr.ptr, (RunSum *)c->runs.ptr, counter, err, 0,
-                                     d.ntiles, s));
+                                     (TileHdr *)c->hdr.ptr, (RunSum *)c->runs.ptr, counter, err, 1,
+                                     std::max(1u, grid), s));
             ++c->timing.launches;
         }
-    } else {
-        const u32 grid = (u32)std::min<u64>((u64)c->cus * blocks_per_cu(c), tiles);
-        HIPCK(c, rbg_launch_scan(dd, np, 0, (u32)tiles, tile_chunks(c), (const u32 *)c->tab_val.ptr,
-                                 (const u16 *)c->tab_code.ptr, (TileHdr *)c->hdr.ptr,
-                                 (RunSum *)c->runs.ptr, counter, err, 1, std::max(1u, grid), s));
-        ++c->timing.launches;
-    }
-    HIPCK(c, hipEventRecord(c->ev[kDense1], s));
-    HIPCK(c, rbg_launch_merge(dd, np, (const TileHdr *)c->hdr.ptr, (const RunSum *)c->runs.ptr, recs,
-                              err, s));
+        // the scan time excludes the merge when the chain runs eagerly
+        if (!c->graph) HIPCK(c, hipEventRecord(c->ev[kDense1], s));
+        HIPCK(c, rbg_launch_merge(dd, np, (const TileHdr *)c->hdr.ptr, (const RunSum *)c->runs.ptr,
+                                  recs, err, s));
+        return RBG_OK;
+    });
+    if (rc) return rc;
+    if (c->graph) HIPCK(c, hipEventRecord(c->ev[kDense1], s));
     return RBG_OK;
 }
 
@@ -508,33 +559,42 @@ int sparse_run(rbg_ctx *c, rbg_record *recs, u32 *err, std::vector<u32> &failed_
                             hipMemcpyHostToDevice, s));
     HIPCK(c, hipMemcpyAsync(c->sp_lab.ptr, c->h_slab.data(), c->h_slab.size() * 2,
                             hipMemcpyHostToDevice, s));
-    HIPCK(c, hipMemsetAsync(c->sp_failed.ptr, 0, np * 4, s));
-    HIPCK(c, hipMemsetAsync(c->sp_stats.ptr, 0, kStatsBytes, s));
     const SparsePrime *dP = (const SparsePrime *)c->sp_primes.ptr;
     SparseProf *dF = (SparseProf *)c->sp_profs.ptr;
     unsigned long long *best = (unsigned long long *)c->sp_best.ptr;
     u32 *failed = (u32 *)c->sp_failed.ptr;
-    HIPCK(c, hipEventRecord(c->ev[kPrep0], s));
-    HIPCK(c, rbg_sparse_prepare(dP, np, dF, nf, (u32)lblk, (u32)iblk, (u32)pblk,
-                                (const u32 *)c->sp_val.ptr, (const u16 *)c->sp_lab.ptr,
-                                (u16 *)c->sp_labels.ptr, (u32 *)c->sp_inv.ptr,
-                                (u32 *)c->sp_planes.ptr, err, s));
-    HIPCK(c, hipEventRecord(c->ev[kPrep1], s));
-    HIPCK(c, hipEventRecord(c->ev[kSeed0], s));
-    HIPCK(c, rbg_sparse_seed(dP, dF, nf, (const u16 *)c->sp_labels.ptr, c->sp_segs.ptr,
-                             segs_per_prof, best, failed, s));
-    HIPCK(c, hipEventRecord(c->ev[kSeed1], s));
     // the filter's XCD-aware schedule needs a grid that is a multiple of 8
     const u32 grid = std::max(8u, ((u32)c->cus * blocks_per_cu(c) + 7u) / 8u * 8u);
     if (dmax * (nf + 8ull * xcd_group(c)) > 0xFFFFFFFFull)
         return c->fail(RBG_ECAPACITY, "too many filter tasks");
-    HIPCK(c, hipEventRecord(c->ev[kFilt0], s));
-    HIPCK(c, rbg_sparse_filter(dP, dF, nf, (u32)dmax, (const u32 *)c->sp_planes.ptr,
-                               (const u32 *)c->sp_inv.ptr, (const u32 *)c->sp_val.ptr,
-                               (const u16 *)c->sp_lab.ptr, best, failed,
-                               (unsigned long long *)c->sp_stats.ptr,
-                               (u32 *)((char *)c->sp_stats.ptr + kQueueOffset), grid,
-                               xcd_group(c), s));
+    // one chain: clear flags and counters, tables and planes, seed, filter
+    // (eagerly with an event per stage; in graph mode the whole chain is
+    // timed as the filter)
+    HIPCK(c, hipEventRecord(c->ev[kPrep0], s));
+    if (c->graph)
+        for (int e : {kPrep1, kSeed0, kSeed1, kFilt0}) HIPCK(c, hipEventRecord(c->ev[e], s));
+    const int chain_rc = launch_chain(c, 1, [&]() -> int {
+        HIPCK(c, rbg_fill_u32(failed, np, 0u, s));
+        HIPCK(c, rbg_fill_u32((u32 *)c->sp_stats.ptr, (u32)(kStatsBytes / 4), 0u, s));
+        HIPCK(c, rbg_sparse_prepare(dP, np, dF, nf, (u32)lblk, (u32)iblk, (u32)pblk,
+                                    (const u32 *)c->sp_val.ptr, (const u16 *)c->sp_lab.ptr,
+                                    (u16 *)c->sp_labels.ptr, (u32 *)c->sp_inv.ptr,
+                                    (u32 *)c->sp_planes.ptr, err, s));
+        if (!c->graph) HIPCK(c, hipEventRecord(c->ev[kPrep1], s));
+        if (!c->graph) HIPCK(c, hipEventRecord(c->ev[kSeed0], s));
+        HIPCK(c, rbg_sparse_seed(dP, dF, nf, (const u16 *)c->sp_labels.ptr, c->sp_segs.ptr,
+                                 segs_per_prof, best, failed, s));
+        if (!c->graph) HIPCK(c, hipEventRecord(c->ev[kSeed1], s));
+        if (!c->graph) HIPCK(c, hipEventRecord(c->ev[kFilt0], s));
+        HIPCK(c, rbg_sparse_filter(dP, dF, nf, (u32)dmax, (const u32 *)c->sp_planes.ptr,
+                                   (const u32 *)c->sp_inv.ptr, (const u32 *)c->sp_val.ptr,
+                                   (const u16 *)c->sp_lab.ptr, best, failed,
+                                   (unsigned long long *)c->sp_stats.ptr,
+                                   (u32 *)((char *)c->sp_stats.ptr + kQueueOffset), grid,
+                                   xcd_group(c), s));
+        return RBG_OK;
+    });
+    if (chain_rc) return chain_rc;
     HIPCK(c, hipEventRecord(c->ev[kFilt1], s));
     c->timing.launches += 6;
     std::vector<u32> h_failed(np, 0);
@@ -583,6 +643,7 @@ int run_block(rbg_ctx *c, const u32 *h_primes, u32 n, u32 flags, rbg_record *d_o
     if (n > limit) return c->fail(RBG_EINVAL, "more than 4096 primes in one block");
     c->error.clear();
     c->timing = rbg_timing{};
+    c->graph = (flags & RBG_GRAPH) != 0;
     const u32 method = flags & RBG_METHOD_MASK;
     if (method > RBG_METHOD_DENSE) return c->fail(RBG_EINVAL, "unknown method");
     std::vector<u32> odd, dense, sparse, failed;
@@ -830,6 +891,8 @@ void rbg_close(rbg_ctx *c) {
         b->release();
     for (int i = 0; i < kNumEv; ++i)
         if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
+    for (hipGraphExec_t &x : c->gexec)
+        if (x) (void)hipGraphExecDestroy(x);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }

@@ -281,9 +281,23 @@ __global__ __launch_bounds__(kMergeThreads) void merge_primes(const PrimeDesc *_
     }
 }
 
+// Fills n words with v (used instead of hipMemsetAsync so that a chain can
+// be recorded into a graph).
+__global__ void fill_u32(uint32_t *p, uint32_t n, uint32_t v) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
 }  // namespace rbg
 
 // ---- launch wrappers used by the host orchestration -----------------------
+
+extern "C" hipError_t rbg_fill_u32(uint32_t *p, uint32_t n, uint32_t v, hipStream_t stream) {
+    if (!n) return hipSuccess;
+    const uint32_t blocks = (n + 255u) / 256u < 1024u ? (n + 255u) / 256u : 1024u;
+    hipLaunchKernelGGL(rbg::fill_u32, dim3(blocks), dim3(256), 0, stream, p, n, v);
+    return hipGetLastError();
+}
 
 extern "C" hipError_t rbg_launch_scan(const rbg::PrimeDesc *descs, uint32_t nprimes,
                                       uint32_t tile_first, uint32_t tile_end, uint32_t chunks,

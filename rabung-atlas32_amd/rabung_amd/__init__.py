@@ -24,6 +24,7 @@ METHOD_BATCHED = 0x0  # sparse anchor filter, one persistent launch, dense fallb
 METHOD_PLAIN = 0x1    # dense scan, one launch per prime
 METHOD_DENSE = 0x2    # dense scan, one persistent launch
 CERTIFY = 0x10
+GRAPH = 0x20
 MAX_BLOCK = 4096
 
 STATUS = {0: "ok", 1: "invalid argument", 2: "device error", 3: "verification failed",
@@ -218,25 +219,25 @@ class Engine:
         except Exception:
             pass
 
-    def profile_block(self, primes, method=METHOD_BATCHED, certify=False):
+    def profile_block(self, primes, method=METHOD_BATCHED, certify=False, graph=False):
         """Profiles of up to 4096 primes; returns a list of dicts."""
         primes = [int(p) for p in primes]
         n = len(primes)
         arr = (ctypes.c_uint32 * max(1, n))(*primes)
         out = (Record * max(1, n))()
-        flags = method | (CERTIFY if certify else 0)
+        flags = method | (CERTIFY if certify else 0) | (GRAPH if graph else 0)
         rc = self._lib.rbg_profile_block(self._ctx, arr, n, flags, out)
         if rc:
             raise RabungError(rc, self._lib.rbg_last_error(self._ctx).decode())
         return [record_to_dict(out[i]) for i in range(n)]
 
-    def profile_batch(self, primes, method=METHOD_BATCHED, certify=False):
+    def profile_batch(self, primes, method=METHOD_BATCHED, certify=False, graph=False):
         """Profiles of up to 2^20 primes (many blocks), split by HBM budget."""
         primes = [int(p) for p in primes]
         n = len(primes)
         arr = (ctypes.c_uint32 * max(1, n))(*primes)
         out = (Record * max(1, n))()
-        flags = method | (CERTIFY if certify else 0)
+        flags = method | (CERTIFY if certify else 0) | (GRAPH if graph else 0)
         rc = self._lib.rbg_profile_batch(self._ctx, arr, n, flags, out)
         if rc:
             raise RabungError(rc, self._lib.rbg_last_error(self._ctx).decode())

@@ -204,3 +204,20 @@ def test_batch_across_blocks(engine):
     t = engine.last_timing()
     assert t["sparse_primes"] == len(flat)
     assert got == [r for b in blocks for r in engine.profile_block(b)]
+
+
+@pytest.mark.parametrize("method", [rabung_amd.METHOD_BATCHED, rabung_amd.METHOD_PLAIN,
+                                    rabung_amd.METHOD_DENSE])
+def test_graph_capture_replays(method):
+    """Launch chains recorded into a hipGraph give the eager results on every
+    replay, with eager work of other shapes and syncs between the replays
+    (the cached executable graph is updated or rebuilt each time)."""
+    a = oracle.next_primes(4_000_000_000, 6) + oracle.next_primes(1 << 20, 10)
+    b = oracle.next_primes(3_000_000_000, 9) + [101, 65537]
+    with rabung_amd.Engine() as eng:
+        ea, eb = eng.profile_block(a, method=method), eng.profile_block(b, method=method)
+        for rep in range(4):
+            assert eng.profile_block(a, method=method, graph=True) == ea, rep
+            assert eng.profile_block(b, method=method) == eb
+            assert eng.profile_block(b, method=method, graph=True) == eb, rep
+            assert eng.profile_block(a, method=method, graph=True) == ea, rep
