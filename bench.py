@@ -200,6 +200,8 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=10.0,
                     help="length of the Poisson serving run (0 skips it)")
     ap.add_argument("--qps", type=float, default=20.0, help="serving arrival rate")
+    ap.add_argument("--serve-chunk", type=int, default=64, help="primes per request piece")
+    ap.add_argument("--serve-max", type=int, default=256, help="primes per serving batch")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 256 MiB device copy (rocprof FETCH/WRITE_SIZE calibration)")
     args = ap.parse_args()
@@ -311,14 +313,16 @@ def main():
     if args.serve_seconds > 0:
         from rabung_amd import serve, workloads
         stream = workloads.serving_stream(seed=4, qps=args.qps, seconds=args.serve_seconds)
-        with serve.Server(slots=2) as srv:
+        with serve.Server(slots=2, chunk=args.serve_chunk, max_primes=args.serve_max) as srv:
             serve.run_stream(srv, workloads.serving_stream(seed=5, qps=args.qps, seconds=1.0))
             srv.batches.clear()
             _, summ = serve.run_stream(srv, stream)
         serving = {"workload": "Poisson arrivals at %.1f QPS for %.0f s (seed 4): 80%% single "
                                "primes (next prime >= uniform u32), 20%% intervals of U{16..4096} "
                                "consecutive primes from log-uniform starts in [2^16, 2^32); "
-                               "2 slots, continuous batching" % (args.qps, args.serve_seconds),
+                               "2 slots, continuous batching, pieces of %d primes, batches of "
+                               "at most %d primes" % (args.qps, args.serve_seconds, args.serve_chunk,
+                                                      args.serve_max),
                    **{k: (round(v, 3) if isinstance(v, float) else v) for k, v in summ.items()}}
     cpu = None if args.no_cpu else cpu_baseline(primes)
     line = {

@@ -292,7 +292,7 @@ int launch_chain(rbg_ctx *c, int slot, F &&enqueue) {
 
 u32 blocks_per_cu(const rbg_ctx *c) { return c->cfg.blocks_per_cu ? c->cfg.blocks_per_cu : 8u; }
 
-u32 xcd_group(const rbg_ctx *c) { return c->cfg.xcd_group ? c->cfg.xcd_group : 4u; }
+u32 xcd_group(const rbg_ctx *c) { return c->cfg.xcd_group ? c->cfg.xcd_group : 2u; }
 
 u64 budget(const rbg_ctx *c) { return c->cfg.hbm_budget ? c->cfg.hbm_budget : (16ull << 30); }
 
@@ -809,7 +809,7 @@ int rbg_autotune(rbg_ctx *c, const uint32_t *primes, uint32_t n, uint32_t flags,
         // the filter runs 8 waves per SIMD (8 blocks per CU): fewer blocks
         // leave SIMDs idle, more only queue behind the resident ones
         for (u32 bpc : {8u, 12u, 16u})
-            for (u32 g : {2u, 4u, 8u}) cands.push_back({c->cfg.tile_chunks, bpc, g});
+            for (u32 g : {1u, 2u, 4u}) cands.push_back({c->cfg.tile_chunks, bpc, g});
     } else {
         for (u32 ch : {16u, 32u, 64u, 128u, 256u})
             for (u32 bpc : {4u, 8u}) cands.push_back({ch, bpc, c->cfg.xcd_group});

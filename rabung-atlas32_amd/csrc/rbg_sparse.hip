@@ -483,6 +483,8 @@ __device__ __forceinline__ void wave_lds_fence() {
 // lanes whose words did not fit keep them and the word loop stops handing
 // out words until the queue has been drained).  Stages past t-1 re-read
 // stage t-1, which keeps every address inside the table.
+constexpr u32 kRefill = 48;  // smallest cohort of idle lanes that takes new words
+
 template <int NP>
 __device__ void filter_task(const FilterCtx &c, u32 sp, u32 phi, u32 *qb, u64 *qs, u32 *kinds,
                             u32 &n_windows, u32 &n_words, u32 &n_done) {
@@ -527,8 +529,12 @@ __device__ void filter_task(const FilterCtx &c, u32 sp, u32 phi, u32 *qb, u64 *q
 #pragma unroll
         for (int pb = 0; pb < NP; ++pb) braw[pb][0] = braw[pb][1] = braw[pb][2] = 0;
         for (;;) {
+            // idle lanes take words in cohorts of at least kRefill (or all
+            // 64 at once): a cohort starts together on consecutive words, so
+            // its loads stay on few cache lines
             const u64 idle = ballot64(!have && !pend);
-            if (!stall && idle && next < total) {
+            const u32 nidle = (u32)__builtin_popcountll(idle);
+            if (!stall && idle && next < total && (nidle >= kRefill || nidle == 64u)) {
                 if (!have && !pend) {
                     const u32 my = next + lanes_below(idle);
                     if (my < total) {
