@@ -6,16 +6,20 @@
 
 namespace rbg {
 
-// Montgomery product with 32-bit halves (v_mul_lo_u32 / v_mul_hi_u32); the
-// carry of hi + mh + (lo != 0) is kept because the sum reaches 2p > 2^32 for
-// p > 2^31.
+// Montgomery product with 32-bit halves (v_mul_lo_u32 / v_mul_hi_u32).  The
+// sum u = hi + mh + (lo != 0) reaches 2p > 2^32 for p > 2^31: it is formed
+// with one add-with-carry (carry c1 is bit 32 of u), and u >= p iff c1 or
+// s - p does not borrow, so the reduction is one subtract-with-borrow and one
+// select (8 VALU instructions, 4 of them multiplies).
 __device__ __forceinline__ u32 mont_mul(u32 a, u32 b, u32 p, u32 ninv) {
     const u32 lo = a * b;
     const u32 hi = __umulhi(a, b);
     const u32 m = lo * ninv;
     const u32 mh = __umulhi(m, p);
-    const u64 s = (u64)hi + mh + (lo != 0u);
-    return (u32)(s >= p ? s - p : s);
+    unsigned int c1, borrow;
+    const u32 s = __builtin_addc(hi, mh, lo != 0u ? 1u : 0u, &c1);
+    const u32 t = __builtin_subc(s, p, 0u, &borrow);
+    return (c1 | (borrow ^ 1u)) ? t : s;
 }
 
 __device__ __forceinline__ u32 mont_pow(u32 base, u32 e, u32 p, u32 ninv, u32 one) {
