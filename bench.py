@@ -176,6 +176,18 @@ def calibrate_copy(nbytes=256 << 20, reps=3):
     return 0
 
 
+def microbench():
+    """VALU issue rates (TOP/s) and CU cycles per wave-wide load (one JSON line)."""
+    import rabung_amd
+    names = ["add_or_u32", "mul_lo_or_u32", "mul_hi_or_u32", "alignbit_b32", "montgomery_product",
+             "add_u32", "mul_lo_u32"]
+    rates = {n: round(rabung_amd.microbench(k) / 1e12, 3) for k, n in enumerate(names)}
+    loads = {"%d_dword%s_%s" % (w, "s" if w > 1 else "", "contiguous" if st == 8 else "line_per_lane"):
+             round(rabung_amd.microbench(10 + w, st), 2) for st in (8, 4160) for w in (1, 2, 3, 4)}
+    print(json.dumps({"valu_tops": rates, "cu_cycles_per_wave_load": loads}), flush=True)
+    return 0
+
+
 def profiled_traffic(path=ROOT / "profiles" / "sp_filter_top256.md"):
     """HBM bytes per filter launch on the top-256 workload, as measured by the
     two rocprof counter runs recorded in the profile summary (FETCH_SIZE and
@@ -202,11 +214,15 @@ def main():
     ap.add_argument("--qps", type=float, default=20.0, help="serving arrival rate")
     ap.add_argument("--serve-chunk", type=int, default=64, help="primes per request piece")
     ap.add_argument("--serve-max", type=int, default=256, help="primes per serving batch")
+    ap.add_argument("--microbench", action="store_true",
+                    help="only measure VALU instruction rates and vector-load cost")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 256 MiB device copy (rocprof FETCH/WRITE_SIZE calibration)")
     args = ap.parse_args()
     if args.calibrate_copy:
         return calibrate_copy()
+    if args.microbench:
+        return microbench()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (the path shards across primes with "
               "no exchange; see DESIGN.md); run with --gpus 1", file=sys.stderr)
@@ -254,9 +270,16 @@ def main():
     ms_step = dt * 1000.0 / args.steps
     value = profiles / (dt / args.steps)
 
-    peak_tops = info["cus"] * 64 * info["clock_khz"] * 1e3 / 1e12
-    peak_basis = "%d CUs x 64 lanes x %.0f MHz, INT32 VALU ops" % (info["cus"],
-                                                                   info["clock_khz"] / 1e3)
+    # INT32 VALU peak: each CU retires 128 32-bit lane operations per clock
+    # (the same rate as vector FP32, 157.3 TFLOP/s as FMA); the live probe
+    # below measures what dependent chains of adds, multiplies and whole
+    # Montgomery products reach on this device
+    peak_tops = info["cus"] * 128 * info["clock_khz"] * 1e3 / 1e12
+    peak_basis = "%d CUs x 128 lanes x %.0f MHz, INT32 VALU ops" % (info["cus"],
+                                                                    info["clock_khz"] / 1e3)
+    probe = {"add_u32_tops": round(rabung_amd.microbench(5) / 1e12, 3),
+             "mul_lo_u32_tops": round(rabung_amd.microbench(6) / 1e12, 3),
+             "montgomery_products_per_s": round(rabung_amd.microbench(4), 0)}
     plain = None
     if not args.no_plain:
         psteps = max(1, args.steps // 3)
@@ -276,7 +299,12 @@ def main():
                               "frac": round(ops / scan_s / 1e12 / peak_tops, 4),
                               "algorithmic_ops_per_step": ops,
                               "scan_ms_per_step": round(scan_s * 1000.0, 3),
-                              "peak_basis": peak_basis}}
+                              "peak_basis": peak_basis,
+                              # products/s against the measured product rate
+                              "montgomery_products_per_s": round(ops / 8 / scan_s, 0),
+                              "frac_of_measured_product_rate": round(
+                                  ops / 8 / scan_s / probe["montgomery_products_per_s"], 4),
+                              "probe": probe}}
 
     # dominant kernel of the batched path: the sparse anchor filter (or the
     # dense scan when the whole workload falls below the sparse domain)
@@ -299,7 +327,7 @@ def main():
                                  "filter": round(timing["sparse_filter_ms"], 3),
                                  "dense_fallback": round(timing["dense_scan_ms"], 3),
                                  "call_total": round(timing["total_ms"], 3)},
-                    "peak_basis": peak_basis}
+                    "peak_basis": peak_basis, "probe": probe}
     else:
         ops = int32_ops(primes)
         kern_ms = sum(t["dense_scan_ms"] for t in timings) / len(timings)

@@ -164,6 +164,15 @@ int rbg_a4_host(const uint32_t *primes, uint32_t n, uint32_t *out, uint32_t thre
  * (fields after elliptic4_mestre.cpp:68-73; the point list is not kept). */
 int rbg_a4_certificate(uint32_t p, uint32_t *a4, uint64_t *attempts, char *out, size_t *out_len);
 
+/* Measurement probes for the roofline (no reference counterpart).  kind 0-6:
+ * issue rate, in lane operations per second, of add(x, y|1), mul_lo(x, y|1),
+ * mul_hi(x, y|0x80000001) (each with one v_or_b32), v_alignbit_b32, whole
+ * Montgomery products, plain add and plain v_mul_lo_u32 (8 independent
+ * chains per thread, 8 waves per SIMD); kind 11-14: CU cycles per wave-wide
+ * load of 1-4 dwords per lane with lanes `param` = 8 (contiguous) or 4160
+ * (one line each) bytes apart, from an L2-resident buffer. */
+int rbg_microbench(int device, int kind, uint32_t param, double *out);
+
 /* Admissibility mask of p (bit r-2 iff r | p-1); 0 for p < 3. */
 uint32_t rbg_mask(uint32_t p);
 

@@ -1,0 +1,163 @@
+// Measurement probes for the roofline of the census kernels (gfx950).
+//
+//   * VALU issue rates of the instructions the kernels are made of: 32-bit
+//     add, v_mul_lo_u32, v_mul_hi_u32, v_alignbit_b32, and whole Montgomery
+//     products (the dense scan's unit of work).  Each thread runs 8
+//     independent dependency chains so that latency is hidden and the rate is
+//     the issue rate.
+//   * The cost of one wave-wide vector load: 1 to 4 dwords per lane, lanes
+//     8 bytes apart (one contiguous 512-byte span) or 4160 bytes apart (one
+//     line per lane), from a buffer that stays in L2.
+//
+// No reference counterpart: the reference quotes its dense rate in source
+// comments only (original_scan_gpu.cu:12-14).
+#include <hip/hip_runtime.h>
+
+#include "rabung.h"
+#include "rbg_device.hpp"
+
+namespace rbg {
+
+constexpr int kChains = 8;
+
+template <int OP>
+__global__ __launch_bounds__(256) void valu_rate(u32 seed, u32 iters, u32 p, u32 ninv, u32 *out) {
+    u32 v[kChains];
+#pragma unroll
+    for (int c = 0; c < kChains; ++c) v[c] = seed + threadIdx.x * 977u + c * 131u;
+    for (u32 it = 0; it < iters; ++it) {
+#pragma unroll
+        for (int c = 0; c < kChains; ++c) {
+            if (OP == 0) v[c] = v[c] + (v[(c + 1) % kChains] | 1u);
+            if (OP == 1) v[c] = v[c] * (v[(c + 1) % kChains] | 1u);
+            if (OP == 2) v[c] = __umulhi(v[c], v[(c + 1) % kChains] | 0x80000001u);
+            if (OP == 3) v[c] = __builtin_amdgcn_alignbit(v[c], v[(c + 1) % kChains], v[c]);
+            if (OP == 4) v[c] = mont_mul(v[c], v[c], p, ninv);
+            if (OP == 5) v[c] = v[c] + v[(c + 1) % kChains];
+            if (OP == 6) v[c] = v[c] * v[(c + 1) % kChains];
+        }
+    }
+    u32 x = 0;
+#pragma unroll
+    for (int c = 0; c < kChains; ++c) x ^= v[c];
+    if (x == 0x9E3779B9u) out[0] = x;  // keeps the chains alive
+}
+
+template <int NW>
+__global__ __launch_bounds__(256) void load_rate(const u32 *__restrict__ buf, u32 mask, u32 stride,
+                                                 u32 iters, u32 *out) {
+    const u32 lane = threadIdx.x & 63u;
+    const u32 wave = blockIdx.x * 4u + (threadIdx.x >> 6);
+    u32 acc = 0, off = (wave * 1024u) & mask;
+    for (u32 it = 0; it < iters; ++it) {
+        const u32 a = ((off + lane * stride) & mask) >> 2;  // the buffer has 4 spare dwords
+        if (NW == 1) acc ^= buf[a];
+        if (NW == 2) acc ^= buf[a] ^ buf[a + 1];
+        if (NW == 3) acc ^= buf[a] ^ buf[a + 1] ^ buf[a + 2];
+        if (NW == 4) acc ^= buf[a] ^ buf[a + 1] ^ buf[a + 2] ^ buf[a + 3];
+        off = (off + 4160u) & mask;
+    }
+    if (acc == 0x9E3779B9u) out[0] = acc;
+}
+
+}  // namespace rbg
+
+namespace {
+
+template <class L>
+double time_ms(L launch) {
+    hipEvent_t a, b;
+    if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return -1.0;
+    float best = 1e30f;
+    for (int rep = 0; rep < 3; ++rep) {
+        (void)hipEventRecord(a, 0);
+        launch();
+        (void)hipEventRecord(b, 0);
+        if (hipEventSynchronize(b) != hipSuccess) return -1.0;
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, a, b);
+        if (ms < best) best = ms;
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    return hipGetLastError() == hipSuccess ? best : -1.0;
+}
+
+}  // namespace
+
+// kind 0..6: issue rate, in lane operations per second, of add(x, y|1) /
+// mul_lo(x, y|1) / mul_hi(x, y|0x80000001) / alignbit / Montgomery product /
+// plain add / plain mul_lo (the first three also issue one v_or_b32 per
+// operation).
+// kind 10 + w (w = 1..4): cycles of one CU per wave-wide load of w dwords per
+// lane; param = lane stride in bytes (8: contiguous span, 4160: one line per
+// lane).  *out receives the figure.
+extern "C" int rbg_microbench(int device, int kind, uint32_t param, double *out) {
+    if (!out || hipSetDevice(device) != hipSuccess) return RBG_EDEVICE;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) return RBG_EDEVICE;
+    const double cus = prop.multiProcessorCount, clock_hz = prop.clockRate * 1e3;
+    uint32_t *sink = nullptr;
+    if (hipMalloc(&sink, 64) != hipSuccess) return RBG_EDEVICE;
+    int rc = RBG_OK;
+    const uint32_t blocks = (uint32_t)cus * 8u;  // 8 waves per SIMD
+    if (kind >= 0 && kind <= 6) {
+        const uint32_t iters = 4096, p = 4294967291u;
+        uint32_t ninv = 1;
+        for (int i = 0; i < 5; ++i) ninv *= 2u - p * ninv;
+        ninv = 0u - ninv;
+        double ms = -1.0;
+        switch (kind) {
+#define RBG_VALU_CASE(K)                                                                         \
+    case K:                                                                                      \
+        ms = time_ms([&] {                                                                       \
+            hipLaunchKernelGGL(rbg::valu_rate<K>, dim3(blocks), dim3(256), 0, 0, 12345u, iters, \
+                               p, ninv, sink);                                                   \
+        });                                                                                      \
+        break;
+            RBG_VALU_CASE(0)
+            RBG_VALU_CASE(1)
+            RBG_VALU_CASE(2)
+            RBG_VALU_CASE(3)
+            RBG_VALU_CASE(4)
+            RBG_VALU_CASE(5)
+            RBG_VALU_CASE(6)
+#undef RBG_VALU_CASE
+        }
+        if (ms <= 0) rc = RBG_EDEVICE;
+        else *out = (double)blocks * 256.0 * iters * rbg::kChains / (ms * 1e-3);
+    } else if (kind >= 11 && kind <= 14 && (param == 8u || param == 4160u)) {
+        const uint32_t bytes = 1u << 20, iters = 2000;
+        uint32_t *buf = nullptr;
+        if (hipMalloc(&buf, bytes + 64) != hipSuccess || hipMemset(buf, 1, bytes + 64) != hipSuccess) {
+            rc = RBG_EDEVICE;
+        } else {
+            const int nw = kind - 10;
+            double ms = -1.0;
+            switch (nw) {
+#define RBG_LOAD_CASE(W)                                                                         \
+    case W:                                                                                      \
+        ms = time_ms([&] {                                                                       \
+            hipLaunchKernelGGL(rbg::load_rate<W>, dim3(blocks), dim3(256), 0, 0, buf, bytes - 1, \
+                               param, iters, sink);                                              \
+        });                                                                                      \
+        break;
+                RBG_LOAD_CASE(1)
+                RBG_LOAD_CASE(2)
+                RBG_LOAD_CASE(3)
+                RBG_LOAD_CASE(4)
+#undef RBG_LOAD_CASE
+            }
+            if (ms <= 0) rc = RBG_EDEVICE;
+            else {
+                const double wave_loads_per_cu = (double)blocks * 4.0 * iters / cus;
+                *out = ms * 1e-3 * clock_hz / wave_loads_per_cu;
+            }
+        }
+        if (buf) (void)hipFree(buf);
+    } else {
+        rc = RBG_EINVAL;
+    }
+    (void)hipFree(sink);
+    return rc;
+}

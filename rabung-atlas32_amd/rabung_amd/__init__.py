@@ -59,7 +59,7 @@ EXPORTS = ["rbg_open", "rbg_profile_block", "rbg_profile_block_device", "rbg_pro
            "rbg_estimate_bytes", "rbg_autotune", "rbg_last_error",
            "rbg_last_timing", "rbg_device_info", "rbg_close", "rbg_mask", "rbg_is_prime",
            "rbg_version", "rbg_a4_host", "rbg_primes_from", "rbg_encode_scan_frame",
-           "rbg_encode_a4_frame", "rbg_science_hash", "rbg_prime_hash", "rbg_a4_certificate",
+           "rbg_encode_a4_frame", "rbg_science_hash", "rbg_prime_hash", "rbg_a4_certificate", "rbg_microbench",
            "baseline_init",
            "baseline_profile"]
 
@@ -134,6 +134,9 @@ def load():
     for name in ("rbg_science_hash", "rbg_prime_hash"):
         getattr(lib, name).argtypes = [P(Record), ctypes.c_uint32, ctypes.c_char_p]
         getattr(lib, name).restype = ctypes.c_int
+    lib.rbg_microbench.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_uint32,
+                                   P(ctypes.c_double)]
+    lib.rbg_microbench.restype = ctypes.c_int
     _lib = lib
     return lib
 
@@ -326,3 +329,12 @@ def baseline_profile(p, mask_bits=0, full=True):
 def rows(records):
     """Library records -> codec rows."""
     return codec.records_to_rows(records)
+
+
+def microbench(kind, param=0, device=0):
+    """One measurement probe of rbg_microbench (see include/rabung.h)."""
+    out = ctypes.c_double(0)
+    rc = load().rbg_microbench(device, kind, param, ctypes.byref(out))
+    if rc:
+        raise RabungError(rc, "microbench")
+    return out.value
