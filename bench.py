@@ -89,6 +89,84 @@ def bench_batch64(args):
     return 0
 
 
+# The reference's published batch throughput (RTX 5090): 1024 consecutive
+# primes per window, all admissible r, eight 128-prime batches through its
+# worker protocol, sum of batch wall times (paper/GPU_METHOD_EN.tex:845-848).
+REFERENCE_WINDOWS = {257000003: 0.1461, 1337000003: 0.4461, 2971000001: 0.8447,
+                     4237000001: 1.1118}
+
+
+def bench_holdout(args):
+    """The reference's throughput windows through this build's worker: the
+    same 8 x 128-prime batches of the same line protocol, timed by the driver
+    from sending a batch to its batch_done line (process start excluded);
+    results checked against the windows' committed profiles."""
+    import json as _json
+    import subprocess
+    sys.path.insert(0, str(ROOT / "tests"))
+    import golden_data
+    worker = ROOT / "rabung-atlas32_amd" / "rbg_worker"
+    windows = []
+    for w in golden_data.holdout_windows():
+        if w["lo"] in REFERENCE_WINDOWS:
+            windows.append(w)
+    proc = subprocess.Popen([str(worker), "profile"], stdin=subprocess.PIPE,
+                            stdout=subprocess.PIPE, text=True, bufsize=1)
+    ready = _json.loads(proc.stdout.readline())
+    assert ready["type"] == "ready"
+    batch_id = 0
+
+    def run_window(rows):
+        nonlocal batch_id
+        primes = [r["p"] for r in rows]
+        got, total = [], 0.0
+        for i in range(0, len(primes), 128):
+            t0 = time.perf_counter()
+            proc.stdin.write("%d %s\n" % (batch_id, " ".join(map(str, primes[i:i + 128]))))
+            proc.stdin.flush()
+            while True:
+                item = _json.loads(proc.stdout.readline())
+                if item["type"] == "measurement":
+                    got.append({"p": item["p"], "profiles": item["profiles"]})
+                elif item["type"] == "batch_done":
+                    break
+            total += time.perf_counter() - t0
+            batch_id += 1
+        if got != rows:
+            raise SystemExit("bench: worker results differ from the window's profiles")
+        return total
+
+    out, ref_total, our_total, profiles = [], 0.0, 0.0, 0
+    for w in windows:
+        run_window(w["rows"])  # warm-up pass
+        reps = [run_window(w["rows"]) for _ in range(max(1, args.steps))]
+        mean = sum(reps) / len(reps)
+        nprof = sum(len(r["profiles"]) for r in w["rows"])
+        ref = REFERENCE_WINDOWS[w["lo"]]
+        out.append({"window_start": w["lo"], "primes": len(w["rows"]), "profiles": nprof,
+                    "seconds": round(mean, 4), "profiles_per_s": round(nprof / mean, 1),
+                    "reference_seconds": ref, "speedup_vs_reference": round(ref / mean, 2)})
+        ref_total += ref
+        our_total += mean
+        profiles += nprof
+    proc.stdin.write("QUIT\n")
+    proc.stdin.flush()
+    proc.wait(timeout=60)
+    line = {"metric": METRIC, "value": round(profiles / our_total, 3), "unit": "profiles/s",
+            "n_gpus": 1, "steps": args.steps, "warmup": 1,
+            "ms_per_step": round(our_total * 1000.0, 3), "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": round(ref_total / our_total, 3), "dtype": "u32",
+            "data": "reference throughput windows (committed profiles)",
+            "config": {"workload": "reference batch-throughput windows: 1024 consecutive primes "
+                                   "from 0.257e9, 1.337e9, 2.971e9 and 4.237e9, all admissible r, "
+                                   "8 x 128-prime batches through rbg_worker's line protocol "
+                                   "(certified), sum of batch wall times per window",
+                       "windows": out,
+                       "baseline": "paper/GPU_METHOD_EN.tex:845-848, RTX 5090, selected engine"}}
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def mm_products(e):
     if e < 2:
         return 0
@@ -206,7 +284,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="top256", choices=sorted(WORKLOADS) + ["batch64"])
+    ap.add_argument("--workload", default="top256",
+                    choices=sorted(WORKLOADS) + ["batch64", "holdout"])
     ap.add_argument("--no-plain", action="store_true", help="skip timing the plain method")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--serve-seconds", type=float, default=10.0,
@@ -229,6 +308,8 @@ def main():
         return 2
     if args.workload == "batch64":
         return bench_batch64(args)
+    if args.workload == "holdout":
+        return bench_holdout(args)
     import torch
     import rabung_amd
     from rabung_amd import Record, RECORD_SIZE

@@ -74,7 +74,7 @@ typedef struct rbg_config {
     uint32_t tile_chunks;    /* dense scan: 64-position chunks per wave segment; 0 = 64 */
     uint32_t blocks_per_cu;  /* persistent blocks (4 waves) per CU; 0 = 8 */
     uint64_t hbm_budget;     /* bytes of device workspace allowed; 0 = 16 GiB */
-    uint32_t xcd_group;      /* sparse filter: profiles one XCD works on at a time; 0 = 2 */
+    uint32_t xcd_group;      /* sparse filter: profiles one XCD works on at a time; 0 = 1 */
     uint32_t reserved;
 } rbg_config;
 

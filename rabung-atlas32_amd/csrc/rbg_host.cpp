@@ -292,7 +292,7 @@ int launch_chain(rbg_ctx *c, int slot, F &&enqueue) {
 
 u32 blocks_per_cu(const rbg_ctx *c) { return c->cfg.blocks_per_cu ? c->cfg.blocks_per_cu : 8u; }
 
-u32 xcd_group(const rbg_ctx *c) { return c->cfg.xcd_group ? c->cfg.xcd_group : 2u; }
+u32 xcd_group(const rbg_ctx *c) { return c->cfg.xcd_group ? c->cfg.xcd_group : 1u; }
 
 u64 budget(const rbg_ctx *c) { return c->cfg.hbm_budget ? c->cfg.hbm_budget : (16ull << 30); }
 
