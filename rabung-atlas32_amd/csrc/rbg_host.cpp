@@ -806,9 +806,9 @@ int rbg_autotune(rbg_ctx *c, const uint32_t *primes, uint32_t n, uint32_t flags,
     };
     std::vector<Cand> cands;
     if (method == RBG_METHOD_BATCHED) {
-        // the filter runs 8 waves per SIMD (8 blocks per CU): fewer blocks
-        // leave SIMDs idle, more only queue behind the resident ones
-        for (u32 bpc : {8u, 12u, 16u})
+        // the filter runs 6 waves per SIMD (6 blocks per CU resident): fewer
+        // blocks leave SIMDs idle, more only queue behind the resident ones
+        for (u32 bpc : {6u, 8u, 12u})
             for (u32 g : {1u, 2u, 4u}) cands.push_back({c->cfg.tile_chunks, bpc, g});
     } else {
         for (u32 ch : {16u, 32u, 64u, 128u, 256u})

@@ -589,19 +589,28 @@ __device__ void filter_task(const FilterCtx &c, u32 sp, u32 phi, u32 *qb, u64 *q
             }
             bool fin = false;
             if (have) {
-                u64 w[G][NP];
+                // every load of the step is issued before the first use
+                u32 raw[G][NP][3], shg[G];
 #pragma unroll
                 for (int g = 0; g < G; ++g) {
                     const u32 ig = (u32)max(io - g * (int)d, io_last);
+                    shg[g] = ig & 31u;
 #pragma unroll
-                    for (int pb = 0; pb < NP; ++pb) w[g][pb] = window64(plb[pb], ig);
+                    for (int pb = 0; pb < NP; ++pb) {
+                        const u32 *q = plb[pb] + (ig >> 5);
+                        raw[g][pb][0] = q[0];
+                        raw[g][pb][1] = q[1];
+                        raw[g][pb][2] = q[2];
+                    }
                 }
+                __builtin_amdgcn_sched_barrier(0);
                 u64 mism = 0;
 #pragma unroll
                 for (int pb = 0; pb < NP; ++pb) {
                     const u64 base = window_of(braw[pb][0], braw[pb][1], braw[pb][2], sh);
 #pragma unroll
-                    for (int g = 0; g < G; ++g) mism |= base ^ w[g][pb];
+                    for (int g = 0; g < G; ++g)
+                        mism |= base ^ window_of(raw[g][pb][0], raw[g][pb][1], raw[g][pb][2], shg[g]);
                 }
                 n_windows += (u32)NP * min((u32)G, t - j);
                 live &= ~mism;
@@ -666,7 +675,7 @@ __device__ __forceinline__ u32 next_task(u32 &k, u32 &chunk, u32 claim) {
     return chunk + k;
 }
 
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void sp_filter(const SparsePrime *__restrict__ P,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void sp_filter(const SparsePrime *__restrict__ P,
                                                  const SparseProf *__restrict__ prof, u32 nprof,
                                                  u32 dmax, const u32 *__restrict__ planes,
                                                  const u32 *__restrict__ inv,
