@@ -221,3 +221,27 @@ def test_graph_capture_replays(method):
             assert eng.profile_block(b, method=method) == eb
             assert eng.profile_block(b, method=method, graph=True) == eb, rep
             assert eng.profile_block(a, method=method, graph=True) == ea, rep
+
+
+def test_random_primes_sparse_vs_dense(engine):
+    """Seeded primes drawn log-uniformly over the sparse domain [2^16, 2^32)
+    (and near its lower edge): the sparse engine and the dense scan are
+    independent algorithms and must agree bit for bit; the oracle pins the
+    smaller ones."""
+    rng = random.Random(20261203)
+    primes = set()
+    while len(primes) < 40:
+        x = int(2.0 ** rng.uniform(16, 32))
+        got = rabung_amd.primes_from(x, 1)
+        if got:
+            primes.add(got[0])
+    primes |= set(rabung_amd.primes_from(65520, 6))  # both sides of the sparse threshold
+    primes = sorted(primes)
+    sparse = engine.profile_block(primes)
+    t = engine.last_timing()
+    assert t["sparse_primes"] >= sum(p >= 1 << 16 for p in primes) - t["fallbacks"]
+    dense = engine.profile_block(primes, method=rabung_amd.METHOD_DENSE)
+    for s, d in zip(sparse, dense):
+        assert s == d, (s["p"], s, d)
+    small = [p for p in primes if p < 1 << 24]
+    assert engine.profile_block(small) == oracle.profile_many(small)
