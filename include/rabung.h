@@ -69,6 +69,19 @@ typedef struct rbg_record {
     uint32_t status;
 } rbg_record;
 
+/* The same record for the wider domain p < 2^33 (M, L, W, A4 still fit 32
+ * bits: W <= (p-1)/2 and A4 <= (p-3)/2). */
+typedef struct rbg_record_wide {
+    uint64_t p;
+    uint32_t mask;
+    uint32_t M[8];
+    uint32_t L[8];
+    uint32_t W[8];
+    uint64_t K[8];
+    uint32_t A4;
+    uint32_t status;
+} rbg_record_wide;
+
 typedef struct rbg_config {
     int32_t device;          /* HIP device ordinal */
     uint32_t tile_chunks;    /* dense scan: 64-position chunks per wave segment; 0 = 64 */
@@ -121,6 +134,21 @@ int rbg_estimate_bytes(const rbg_ctx *ctx, const uint32_t *primes, uint32_t n,
 int rbg_profile_block_device(rbg_ctx *ctx, const uint32_t *d_primes, uint32_t n,
                              uint32_t flags, rbg_record *d_out);
 
+/* The wider arithmetic domain: primes 1000 <= p < 2^33 (and any smaller
+ * prime).  Replaces the reference's 33-bit worker and engine
+ * (reproducibility/gpu_method/source/worker33.cu:19-41 calculate_prime /
+ * main, engine33/src/frozen_entry.cu, engine33/src/math33.hpp:9-72): the
+ * same census with residues of 33 bits (Montgomery products with R = 2^32,
+ * two conditional subtractions).  Primes below 2^32 of the same call take
+ * the 32-bit kernels.  Same flags, limits and all-or-nothing contract as
+ * rbg_profile_block / rbg_profile_batch / rbg_estimate_bytes. */
+int rbg_profile_block_wide(rbg_ctx *ctx, const uint64_t *primes, uint32_t n,
+                           uint32_t flags, rbg_record_wide *out);
+int rbg_profile_batch_wide(rbg_ctx *ctx, const uint64_t *primes, uint32_t n,
+                           uint32_t flags, rbg_record_wide *out);
+int rbg_estimate_bytes_wide(const rbg_ctx *ctx, const uint64_t *primes, uint32_t n,
+                            uint32_t flags, uint64_t *bytes);
+
 /* Message for the last nonzero status of this context ("" if none). */
 const char *rbg_last_error(const rbg_ctx *ctx);
 
@@ -156,6 +184,10 @@ void rbg_close(rbg_ctx *ctx);
  * non-prime and RBG_ECERTIFY if a count fails its congruence checks. */
 int rbg_a4_host(const uint32_t *primes, uint32_t n, uint32_t *out, uint32_t threads);
 
+/* Same for primes below 2^33 (the reference's point count accepts
+ * 1000 <= p < 2^33, src/census/theorie/elliptic4_mestre.cpp:75). */
+int rbg_a4_host_wide(const uint64_t *primes, uint32_t n, uint32_t *out, uint32_t threads);
+
 /* A4 of one prime with the evidence of its point count, as JSON in the
  * caller's buffer (same length convention as the frame writers): curve
  * order, Hasse window, lcm of the point orders found on the curve and its
@@ -173,6 +205,15 @@ int rbg_a4_certificate(uint32_t p, uint32_t *a4, uint64_t *attempts, char *out, 
  * (one line each) bytes apart, from an L2-resident buffer. */
 int rbg_microbench(int device, int kind, uint32_t param, double *out);
 
+/* Test hook for the wide residue arithmetic: on `device`, kind 0 writes
+ * out[i] = a[i] b[i] 2^-32 mod p (the kernels' 33-bit Montgomery product),
+ * kind 1 the Montgomery power of a[i] to b[i] (R = 2^32 mod p = 2^32), for
+ * odd 2^32 < p < 2^33 and a[i] (and for kind 0 b[i]) < p.  No reference
+ * counterpart (the reference checks its wide multiply by the reciprocal
+ * lemma, paper/GPU_METHOD_EN.tex:1001-1025). */
+int rbg_check_wide(int device, const uint64_t *a, const uint64_t *b, uint32_t n, uint64_t p,
+                   int kind, uint64_t *out);
+
 /* Admissibility mask of p (bit r-2 iff r | p-1); 0 for p < 3. */
 uint32_t rbg_mask(uint32_t p);
 
@@ -183,6 +224,13 @@ uint32_t rbg_primes_from(uint64_t start, uint32_t count, uint32_t *out);
 
 /* Deterministic primality for 32-bit inputs. */
 int rbg_is_prime(uint32_t p);
+
+/* Wider domain helpers (p < 2^33; the reference's wide33::prime and
+ * prime_sieve bounds, engine33/src/math33.hpp:72, src/census/prime_sieve.cpp:11):
+ * primality (0 for p >= 2^33), admissibility mask, consecutive primes. */
+int rbg_is_prime_wide(uint64_t p);
+uint32_t rbg_mask_wide(uint64_t p);
+uint32_t rbg_primes_from_wide(uint64_t start, uint32_t count, uint64_t *out);
 
 /* Library version string. */
 const char *rbg_version(void);

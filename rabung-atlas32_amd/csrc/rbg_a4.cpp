@@ -31,7 +31,7 @@ typedef uint32_t u32;
 
 struct Field {
     u64 p;
-    u64 mul(u64 a, u64 b) const { return a * b % p; }  // a, b < p < 2^32
+    u64 mul(u64 a, u64 b) const { return (u64)((unsigned __int128)a * b % p); }  // p < 2^33
     u64 add(u64 a, u64 b) const { u64 s = a + b; return s >= p ? s - p : s; }
     u64 sub(u64 a, u64 b) const { return a >= b ? a - b : a + p - b; }
     u64 pow(u64 a, u64 e) const {
@@ -253,9 +253,9 @@ u32 dense_a4(u32 p) {
 }
 
 // 0 on success; a4 = UINT32_MAX on failure.
-int a4_one(u32 p, u32 *a4, u64 *points = nullptr, CountInfo *info = nullptr) {
+int a4_one(u64 p, u32 *a4, u64 *points = nullptr, CountInfo *info = nullptr) {
     if (p == 2) { *a4 = 0; return 0; }
-    if (p < 1000) { *a4 = dense_a4(p); return 0; }
+    if (p < 1000) { *a4 = dense_a4((u32)p); return 0; }
     const u64 N = curve_order(p, info);
     if (points) *points = N;
     if (!N) return 1;
@@ -271,12 +271,8 @@ int a4_one(u32 p, u32 *a4, u64 *points = nullptr, CountInfo *info = nullptr) {
     return 0;
 }
 
-}  // namespace
-
-extern "C" int rbg_a4_host(const uint32_t *primes, uint32_t n, uint32_t *out, uint32_t threads) {
-    if ((!primes || !out) && n) return RBG_EINVAL;
-    for (u32 i = 0; i < n; ++i)
-        if (!rbg_is_prime(primes[i])) return RBG_EINVAL;
+template <class P>
+int a4_many(const P *primes, uint32_t n, uint32_t *out, uint32_t threads) {
     if (!threads) threads = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
     threads = std::min<u32>(threads, std::max(1u, n));
     std::vector<int> rc(threads, 0);
@@ -290,6 +286,23 @@ extern "C" int rbg_a4_host(const uint32_t *primes, uint32_t n, uint32_t *out, ui
     for (int r : rc)
         if (r) return r;
     return RBG_OK;
+}
+
+}  // namespace
+
+extern "C" int rbg_a4_host(const uint32_t *primes, uint32_t n, uint32_t *out, uint32_t threads) {
+    if ((!primes || !out) && n) return RBG_EINVAL;
+    for (u32 i = 0; i < n; ++i)
+        if (!rbg_is_prime(primes[i])) return RBG_EINVAL;
+    return a4_many(primes, n, out, threads);
+}
+
+extern "C" int rbg_a4_host_wide(const uint64_t *primes, uint32_t n, uint32_t *out,
+                                uint32_t threads) {
+    if ((!primes || !out) && n) return RBG_EINVAL;
+    for (u32 i = 0; i < n; ++i)
+        if (!rbg_is_prime_wide(primes[i])) return RBG_EINVAL;
+    return a4_many(primes, n, out, threads);
 }
 
 extern "C" int rbg_a4_certificate(uint32_t p, uint32_t *a4, uint64_t *attempts, char *out,

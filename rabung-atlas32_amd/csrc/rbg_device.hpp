@@ -40,7 +40,8 @@ __device__ __forceinline__ u32 add_mod(u32 a, u32 b, u32 p) {
 
 // Index of y in the sorted table vals[0..m) (binary search, fixed trip count
 // for a given m so that a wave stays converged).
-__device__ __forceinline__ u32 table_index(const u32 *vals, u32 m, u32 y) {
+template <class V>
+__device__ __forceinline__ u32 table_index(const V *vals, u32 m, V y) {
     u32 lo = 0, n = m;
     while (n > 1) {
         const u32 half = n >> 1;
@@ -49,5 +50,65 @@ __device__ __forceinline__ u32 table_index(const u32 *vals, u32 m, u32 y) {
     }
     return lo;
 }
+
+// Montgomery product for primes 2^32 < p < 2^33, still with R = 2^32 (now
+// R < p).  Residues a, b < p are a low word and a high bit:
+// a b = al bl + (ah bl + bh al) 2^32 + ah bh 2^64.  With m = lo(al bl) n' and
+// p = 2^32 + pl, the low word of a b + m p is lo(al bl) + lo(m pl) = 0 mod
+// 2^32 (carry [lo(al bl) != 0]) and the exact quotient (a b + m p) / 2^32 is
+//     hi(al bl) + hi(m pl) + carry + ah bl + bh al + ah bh 2^32 + m,
+// which is below p (p / 2^32 + 1) < 3p: two conditional subtractions
+// reduce it (4 multiplies).
+__device__ __forceinline__ u64 mont_mul33(u64 a, u64 b, u64 p, u32 ninv) {
+    const u32 al = (u32)a, bl = (u32)b, pl = (u32)p;
+    const u32 ah = (u32)(a >> 32), bh = (u32)(b >> 32);  // 0 or 1
+    const u32 lo = al * bl, hi = __umulhi(al, bl);
+    const u32 m = lo * ninv;
+    const u32 mh = __umulhi(m, pl);
+    u64 u = (u64)hi + mh + m + (lo != 0u ? 1u : 0u);
+    u += (u64)(bl & (0u - ah)) + (al & (0u - bh)) + ((u64)(ah & bh) << 32);
+    u = u >= p ? u - p : u;
+    return u >= p ? u - p : u;
+}
+
+__device__ __forceinline__ u64 mont_pow33(u64 base, u64 e, u64 p, u32 ninv, u64 one) {
+    if (e == 0) return one;
+    u64 y = base;
+    const int top = 63 - __builtin_clzll(e);
+    for (int bit = top - 1; bit >= 0; --bit) {
+        y = mont_mul33(y, y, p, ninv);
+        if ((e >> bit) & 1ull) y = mont_mul33(y, base, p, ninv);
+    }
+    return y;
+}
+
+// Residue arithmetic of one prime, as a template argument of the kernels
+// that work for both domains: Dom32 (p < 2^32, u32 residues) and Dom33
+// (2^32 < p < 2^33, u64 residues).  Both use R = 2^32.
+struct Dom32 {
+    typedef u32 V;
+    static constexpr bool wide = false;
+    u32 p, ninv;
+    __device__ __forceinline__ V mul(V a, V b) const { return mont_mul(a, b, p, ninv); }
+    __device__ __forceinline__ V add(V a, V b) const { return add_mod(a, b, p); }
+    __device__ __forceinline__ V pow(V base, u64 e, V one) const {
+        return mont_pow(base, (u32)e, p, ninv, one);
+    }
+};
+
+struct Dom33 {
+    typedef u64 V;
+    static constexpr bool wide = true;
+    u64 p;
+    u32 ninv;
+    __device__ __forceinline__ V mul(V a, V b) const { return mont_mul33(a, b, p, ninv); }
+    __device__ __forceinline__ V add(V a, V b) const {
+        const u64 s = a + b;
+        return s >= p ? s - p : s;
+    }
+    __device__ __forceinline__ V pow(V base, u64 e, V one) const {
+        return mont_pow33(base, e, p, ninv, one);
+    }
+};
 
 }  // namespace rbg

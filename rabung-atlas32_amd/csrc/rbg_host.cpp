@@ -37,9 +37,9 @@ using rbg::u32;
 using rbg::u64;
 
 extern "C" hipError_t rbg_launch_scan(const PrimeDesc *descs, uint32_t nprimes, uint32_t tile_first,
-                                      uint32_t tile_end, uint32_t chunks, const uint32_t *tab_val,
+                                      uint32_t tile_end, uint32_t chunks, const void *tab_val,
                                       const uint16_t *tab_code, TileHdr *hdr, RunSum *runs,
-                                      uint32_t *counter, uint32_t *err, int persistent,
+                                      uint32_t *counter, uint32_t *err, int persistent, int wide,
                                       uint32_t grid, hipStream_t stream);
 extern "C" hipError_t rbg_launch_merge(const PrimeDesc *descs, uint32_t nprimes,
                                        const TileHdr *hdr, const RunSum *runs, rbg_record *out,
@@ -50,7 +50,8 @@ extern "C" hipError_t rbg_sparse_prepare(const SparsePrime *P, uint32_t nprimes,
                                          const SparseProf *prof, uint32_t nprof,
                                          uint32_t label_blocks, uint32_t inverse_blocks,
                                          uint32_t plane_blocks, const uint32_t *tab_val,
-                                         const uint16_t *tab_lab, uint16_t *labels, uint32_t *inv,
+                                         const uint64_t *tab_val64, const uint16_t *tab_lab,
+                                         uint16_t *labels, uint32_t *inv, uint64_t *inv64,
                                          uint32_t *planes, uint32_t *err, hipStream_t stream);
 extern "C" uint32_t rbg_seed_segments(uint32_t bmax);
 extern "C" size_t rbg_seed_agg_bytes(void);
@@ -60,10 +61,10 @@ extern "C" hipError_t rbg_sparse_seed(const SparsePrime *P, SparseProf *prof, ui
                                       hipStream_t stream);
 extern "C" hipError_t rbg_sparse_filter(const SparsePrime *P, const SparseProf *prof,
                                         uint32_t nprof, uint32_t dmax, const uint32_t *planes,
-                                        const uint32_t *inv, const uint32_t *tab_val,
+                                        const void *inv, const void *tab_val,
                                         const uint16_t *tab_lab, unsigned long long *best,
                                         uint32_t *failed, unsigned long long *stats,
-                                        uint32_t *queue, uint32_t grid, uint32_t group,
+                                        uint32_t *queue, int wide, uint32_t grid, uint32_t group,
                                         hipStream_t stream);
 extern "C" hipError_t rbg_sparse_finalize(const SparsePrime *P, uint32_t nprimes,
                                           const SparseProf *prof, const unsigned long long *best,
@@ -99,7 +100,56 @@ bool prime_u32(u32 n) {
     return true;
 }
 
-u32 mask_of(u32 p) {
+// The wider domain 2^32 < p < 2^33 (reference worker33 and engine33,
+// reproducibility/gpu_method/source/worker33.cu:30;
+// engine33/src/math33.hpp:9-72): exact host arithmetic through 128-bit
+// products.
+constexpr u64 kWideLimit = 1ull << 33;
+
+inline u64 mulmod_w(u64 a, u64 b, u64 m) { return (u64)((unsigned __int128)a * b % m); }
+
+u64 powmod_w(u64 a, u64 e, u64 m) {
+    u64 v = 1 % m, x = a % m;
+    while (e) {
+        if (e & 1) v = mulmod_w(v, x, m);
+        x = mulmod_w(x, x, m);
+        e >>= 1;
+    }
+    return v;
+}
+
+// Deterministic Miller-Rabin with the first twelve prime bases, exact for
+// n < 3.18e23 (Sorenson and Webster, Math. Comp. 86 (2017), Theorem 1.1; the
+// reference's exact_prime64, src/census/theorie/bounded_prime.hpp:5-17, rests
+// on the same theorem).
+bool prime_u64(u64 n) {
+    if (n < (1ull << 32)) return prime_u32((u32)n);
+    static const u64 bases[] = {2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37};
+    for (u64 q : bases)
+        if (n % q == 0) return false;
+    u64 d = n - 1;
+    int s = 0;
+    while (!(d & 1)) {
+        d >>= 1;
+        ++s;
+    }
+    for (u64 a : bases) {
+        u64 x = powmod_w(a, d, n);
+        if (x == 1 || x == n - 1) continue;
+        bool ok = false;
+        for (int i = 1; i < s; ++i) {
+            x = mulmod_w(x, x, n);
+            if (x == n - 1) {
+                ok = true;
+                break;
+            }
+        }
+        if (!ok) return false;
+    }
+    return true;
+}
+
+u32 mask_of(u64 p) {
     if (p < 3) return 0;
     u32 m = 0;
     for (u32 r = 2; r <= 9; ++r)
@@ -116,13 +166,13 @@ u32 lcm_of(u32 mask) {
 
 // An element of exact multiplicative order m: z = a^((p-1)/m) for the least
 // a >= 2 with z^(m/q) != 1 for each prime q | m (GPU_METHOD_EN.tex:229-244).
-u32 root_of_order(u32 p, u32 m) {
-    const u32 e = (p - 1) / m;
-    for (u32 a = 2; a < p; ++a) {
-        u32 z = rbg::powmod64(a, e, p);
+u64 root_of_order(u64 p, u32 m) {
+    const u64 e = (p - 1) / m;
+    for (u64 a = 2; a < p; ++a) {
+        const u64 z = powmod_w(a, e, p);
         bool ok = true;
         for (u32 q : {2u, 3u, 5u, 7u})
-            if (m % q == 0 && rbg::powmod64(z, m / q, p) == 1) {
+            if (m % q == 0 && powmod_w(z, m / q, p) == 1) {
                 ok = false;
                 break;
             }
@@ -131,24 +181,44 @@ u32 root_of_order(u32 p, u32 m) {
     return 0;
 }
 
-// Sorted Montgomery images z^j R mod p with their label j (and colour code).
-bool root_table(u32 p, u32 m, std::vector<u32> &vals, std::vector<u16> &codes,
+// Sorted Montgomery images z^j R mod p (R = 2^32) with their label j (and
+// colour code); V = u32 below 2^32, u64 for the wider primes.
+template <class V>
+bool root_table(u64 p, u32 m, std::vector<V> &vals, std::vector<u16> &codes,
                 std::vector<u16> *labs) {
-    const u32 z = root_of_order(p, m);
+    const u64 z = root_of_order(p, m);
     if (!z) return false;
-    std::vector<std::pair<u32, u32>> tab(m);
-    u32 cur = 1;
+    std::vector<std::pair<u64, u32>> tab(m);
+    u64 cur = 1;
     for (u32 j = 0; j < m; ++j) {
-        tab[j] = {(u32)(((u64)cur << 32) % p), j};
-        cur = rbg::mulmod64(cur, z, p);
+        tab[j] = {(u64)(((unsigned __int128)cur << 32) % p), j};
+        cur = mulmod_w(cur, z, p);
     }
     std::sort(tab.begin(), tab.end());
     for (auto &t : tab) {
-        vals.push_back(t.first);
+        vals.push_back((V)t.first);
         codes.push_back((u16)rbg::colour_code(t.second, m));
         if (labs) labs->push_back((u16)t.second);
     }
     return true;
+}
+
+// Montgomery constants of p for R = 2^32: one = R mod p, r2 = R^2 mod p and
+// the form of 64; each as a low word and bit 32 (set only for p > 2^32).
+struct MontConsts {
+    u32 ninv, one, r2, step, hi;
+};
+
+MontConsts mont_consts(u64 p) {
+    const u64 one = (1ull << 32) % p;
+    const u64 r2 = mulmod_w(one, one, p), step = mulmod_w(64, one, p);
+    MontConsts k;
+    k.ninv = rbg::neg_inverse32((u32)p);
+    k.one = (u32)one;
+    k.r2 = (u32)r2;
+    k.step = (u32)step;
+    k.hi = (u32)(p >> 32) | (u32)(one >> 32) << 1 | (u32)(r2 >> 32) << 2 | (u32)(step >> 32) << 3;
+    return k;
 }
 
 u64 isqrt64(u64 n) {
@@ -161,7 +231,7 @@ u64 isqrt64(u64 n) {
 // Certified clipped cover for threshold cap Q (GPU_METHOD_EN.tex:254-326;
 // reference vdw_clipped::from_denominator): D ~ sqrt(p/H), least B with
 // 4BD - HD^2 > 4p, all checked in integers.
-bool choose_cover(u32 p, u32 Q, u32 &B, u32 &D) {
+bool choose_cover(u64 p, u32 Q, u32 &B, u32 &D) {
     const u64 H = Q - 1;
     u64 d = isqrt64(p / H);
     while (d > 1 && H * d * d > 4ull * p) --d;
@@ -198,8 +268,9 @@ struct DevBuf {
 constexpr u32 kSparseMin = 1u << 16;  // smallest prime sent to the sparse engine
 constexpr u32 kCoverQ = 48;           // threshold cap of the cover
 constexpr u32 kBatchChunk = 16384;    // primes per launch chain of a batch
-// filter statistics (3 x u64) and the filter's per-XCD task counters (8 x u32)
-constexpr size_t kQueueOffset = 32, kStatsBytes = kQueueOffset + 8 * 4;
+// filter statistics (3 x u64) and the filter's per-XCD task counters (8 x u32
+// per launch: primes below 2^32, wider primes)
+constexpr size_t kQueueOffset = 32, kStatsBytes = kQueueOffset + 16 * 4;
 
 enum Ev { kAll0, kAll1, kDense0, kDense1, kPrep0, kPrep1, kSeed0, kSeed1, kFilt0, kFilt1, kNumEv };
 
@@ -219,17 +290,20 @@ struct rbg_ctx {
     rbg_config cfg{};
     std::string error;
     // dense path
-    DevBuf descs, tab_val, tab_code, hdr, runs, recs, flags;
+    DevBuf descs, tab_val, tab_val64, tab_code, hdr, runs, recs, flags;
     std::vector<PrimeDesc> h_descs;
     std::vector<u32> h_val;
+    std::vector<u64> h_val64;  // root tables of primes above 2^32
     std::vector<u16> h_code;
     // sparse path
-    DevBuf sp_primes, sp_profs, sp_val, sp_lab, sp_labels, sp_inv, sp_planes, sp_best, sp_failed,
-        sp_a4, sp_stats, sp_segs;
+    DevBuf sp_primes, sp_profs, sp_val, sp_val64, sp_lab, sp_labels, sp_inv, sp_inv64, sp_planes,
+        sp_best, sp_failed, sp_a4, sp_stats, sp_segs;
     std::vector<SparsePrime> h_sp;
     std::vector<SparseProf> h_prof;
     std::vector<u32> h_sval;
+    std::vector<u64> h_sval64;
     std::vector<u16> h_scode, h_slab;
+    u32 nprof32 = 0;  // profiles of primes below 2^32 (they come first)
     // timing of the last call
     rbg_timing timing{};
 
@@ -305,21 +379,21 @@ double elapsed(rbg_ctx *c, int a, int b) {
 // Host verification of one profile by direct powers (reference
 // certify_witness, VBB/src/advance.cpp:7-13, and the prefix loop of
 // src/census/worker32.cu:37-38).
-bool certify(u32 p, u32 r, u32 M, u32 L, u32 W) {
+bool certify(u64 p, u32 r, u32 M, u32 L, u32 W) {
     const u64 e = (p - 1) / r;
     if (M == 0 || W == 0 || (u64)W + M > p || L == 0) return false;
-    const u32 colour = rbg::powmod64(W, e, p);
+    const u64 colour = powmod_w(W, e, p);
     for (u32 j = 1; j < M; ++j)
-        if (rbg::powmod64(W + j, e, p) != colour) return false;
-    if (W > 1 && rbg::powmod64(W - 1, e, p) == colour) return false;
-    if ((u64)W + M < p && rbg::powmod64(W + M, e, p) == colour) return false;
+        if (powmod_w((u64)W + j, e, p) != colour) return false;
+    if (W > 1 && powmod_w(W - 1, e, p) == colour) return false;
+    if ((u64)W + M < p && powmod_w((u64)W + M, e, p) == colour) return false;
     for (u32 x = 1; x <= L; ++x)
-        if (rbg::powmod64(x, e, p) != 1) return false;
-    if ((u64)L + 1 < p && rbg::powmod64(L + 1, e, p) == 1) return false;
+        if (powmod_w(x, e, p) != 1) return false;
+    if ((u64)L + 1 < p && powmod_w((u64)L + 1, e, p) == 1) return false;
     return true;
 }
 
-int certify_all(rbg_ctx *c, const rbg_record *out, u32 n) {
+int certify_all(rbg_ctx *c, const rbg_record *out, const u64 *primes, u32 n) {
     unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
     std::vector<int> bad(nt, -1);
     std::vector<std::thread> th;
@@ -329,7 +403,7 @@ int certify_all(rbg_ctx *c, const rbg_record *out, u32 n) {
                 const rbg_record &rec = out[i];
                 for (u32 r = 2; r <= 9; ++r)
                     if (rec.mask & (1u << (r - 2)))
-                        if (!certify(rec.p, r, rec.M[r - 2], rec.L[r - 2], rec.W[r - 2])) {
+                        if (!certify(primes[i], r, rec.M[r - 2], rec.L[r - 2], rec.W[r - 2])) {
                             bad[t] = (int)i;
                             return;
                         }
@@ -339,7 +413,8 @@ int certify_all(rbg_ctx *c, const rbg_record *out, u32 n) {
     for (int b : bad)
         if (b >= 0) {
             char msg[128];
-            snprintf(msg, sizeof msg, "direct verification failed for p=%u", out[b].p);
+            snprintf(msg, sizeof msg, "direct verification failed for p=%llu",
+                     (unsigned long long)primes[b]);
             return c->fail(RBG_ECERTIFY, msg);
         }
     return RBG_OK;
@@ -347,26 +422,43 @@ int certify_all(rbg_ctx *c, const rbg_record *out, u32 n) {
 
 // ---- dense scan ----------------------------------------------------------
 
-int dense_run(rbg_ctx *c, const u32 *primes, const std::vector<u32> &idx, bool plain,
+// Primes below 2^32 first, then the wider ones (each domain is one
+// contiguous range of descriptors, tiles and launches).
+std::vector<u32> by_domain(const u64 *primes, const std::vector<u32> &idx) {
+    std::vector<u32> out;
+    for (u32 i : idx)
+        if (primes[i] < (1ull << 32)) out.push_back(i);
+    for (u32 i : idx)
+        if (primes[i] >= (1ull << 32)) out.push_back(i);
+    return out;
+}
+
+int dense_run(rbg_ctx *c, const u64 *primes, const std::vector<u32> &idx_in, bool plain,
               rbg_record *recs, u32 *err) {
     const u32 tile_pos = (u32)rbg_waves_per_block() * tile_chunks(c) * 64u;
+    const std::vector<u32> idx = by_domain(primes, idx_in);
     c->h_descs.clear();
     c->h_val.clear();
+    c->h_val64.clear();
     c->h_code.clear();
-    u64 tiles = 0, runs = 0;
+    u64 tiles = 0, runs = 0, tiles32 = 0;
     for (u32 i : idx) {
-        const u32 p = primes[i];
+        const u64 p = primes[i];
+        const bool wide = p >= (1ull << 32);
+        const MontConsts k = mont_consts(p);
         PrimeDesc d{};
-        d.p = p;
-        d.nh = (p - 1) / 2;
+        d.p = (u32)p;
+        d.nh = (u32)((p - 1) / 2);
         d.mask = mask_of(p);
         d.m = lcm_of(d.mask);
-        d.e = (p - 1) / d.m;
-        d.ninv = rbg::neg_inverse32(p);
-        d.one = (u32)((1ull << 32) % p);
-        d.r2 = (u32)((u64)d.one * d.one % p);
-        d.step = (u32)((u64)64 * d.one % p);
-        d.tab_off = (u32)c->h_val.size();
+        d.e = (u32)((p - 1) / d.m);
+        d.ninv = k.ninv;
+        d.one = k.one;
+        d.r2 = k.r2;
+        d.step = k.step;
+        d.hi = k.hi;
+        d.tab_off = (u32)c->h_code.size();
+        d.tab64 = (u32)c->h_val64.size();
         d.planes = 0;
         for (u32 r = 2; r <= 9; ++r)
             if (d.mask & (1u << (r - 2))) d.planes |= rbg::plane_bits(r);
@@ -378,54 +470,72 @@ int dense_run(rbg_ctx *c, const u32 *primes, const std::vector<u32> &idx, bool p
         if (tiles + d.ntiles > 0xFFFFFFFFull || runs + (u64)d.ntiles * d.nr > 0xFFFFFFFFull)
             return c->fail(RBG_ECAPACITY, "block too large for one call");
         tiles += d.ntiles;
+        if (!wide) tiles32 = tiles;
         runs += (u64)d.ntiles * d.nr;
         c->timing.positions += d.nh;
         c->timing.products += (u64)d.nh * rbg::montpow_products(d.e);
-        if (!root_table(p, d.m, c->h_val, c->h_code, nullptr))
-            return c->fail(RBG_EINVAL, "no root of the required order");
+        const bool ok = wide ? root_table(p, d.m, c->h_val64, c->h_code, nullptr)
+                             : root_table(p, d.m, c->h_val, c->h_code, nullptr);
+        if (!ok) return c->fail(RBG_EINVAL, "no root of the required order");
         c->h_descs.push_back(d);
     }
     const u32 np = (u32)c->h_descs.size();
     c->timing.dense_primes = np;
     if (!np) return RBG_OK;
-    const u64 bytes = np * sizeof(PrimeDesc) + c->h_val.size() * 6 + tiles * sizeof(TileHdr) +
-                      runs * sizeof(RunSum);
+    const u64 bytes = np * sizeof(PrimeDesc) + c->h_val.size() * 4 + c->h_val64.size() * 8 +
+                      c->h_code.size() * 2 + tiles * sizeof(TileHdr) + runs * sizeof(RunSum);
     if (bytes > budget(c)) return c->fail(RBG_ECAPACITY, "dense workspace exceeds the HBM budget");
     HIPCK(c, c->descs.ensure(np * sizeof(PrimeDesc)));
     HIPCK(c, c->tab_val.ensure(c->h_val.size() * 4));
+    HIPCK(c, c->tab_val64.ensure(c->h_val64.size() * 8));
     HIPCK(c, c->tab_code.ensure(c->h_code.size() * 2));
     HIPCK(c, c->hdr.ensure(tiles * sizeof(TileHdr)));
     HIPCK(c, c->runs.ensure(runs * sizeof(RunSum)));
     hipStream_t s = c->stream;
     HIPCK(c, hipMemcpyAsync(c->descs.ptr, c->h_descs.data(), np * sizeof(PrimeDesc),
                             hipMemcpyHostToDevice, s));
-    HIPCK(c, hipMemcpyAsync(c->tab_val.ptr, c->h_val.data(), c->h_val.size() * 4,
-                            hipMemcpyHostToDevice, s));
+    if (!c->h_val.empty())
+        HIPCK(c, hipMemcpyAsync(c->tab_val.ptr, c->h_val.data(), c->h_val.size() * 4,
+                                hipMemcpyHostToDevice, s));
+    if (!c->h_val64.empty())
+        HIPCK(c, hipMemcpyAsync(c->tab_val64.ptr, c->h_val64.data(), c->h_val64.size() * 8,
+                                hipMemcpyHostToDevice, s));
     HIPCK(c, hipMemcpyAsync(c->tab_code.ptr, c->h_code.data(), c->h_code.size() * 2,
                             hipMemcpyHostToDevice, s));
-    u32 *counter = (u32 *)c->flags.ptr;
+    // tile counters of the two persistent launches (flags[0], flags[2])
+    u32 *counter = (u32 *)c->flags.ptr, *counter64 = (u32 *)c->flags.ptr + 2;
     const PrimeDesc *dd = (const PrimeDesc *)c->descs.ptr;
     HIPCK(c, hipEventRecord(c->ev[kDense0], s));
     // one chain: clear the tile counter, scan (one launch per prime for the
     // plain method, one persistent launch otherwise), merge per prime
     const int rc = launch_chain(c, 0, [&]() -> int {
         HIPCK(c, rbg_fill_u32(counter, 1, 0u, s));
+        HIPCK(c, rbg_fill_u32(counter64, 1, 0u, s));
         if (plain) {
             for (u32 i = 0; i < np; ++i) {
                 const PrimeDesc &d = c->h_descs[i];
+                const int wide = (int)(d.hi & 1u);
                 HIPCK(c, rbg_launch_scan(dd, np, d.tile_begin, d.tile_begin + d.ntiles,
-                                         tile_chunks(c), (const u32 *)c->tab_val.ptr,
+                                         tile_chunks(c), wide ? c->tab_val64.ptr : c->tab_val.ptr,
                                          (const u16 *)c->tab_code.ptr, (TileHdr *)c->hdr.ptr,
-                                         (RunSum *)c->runs.ptr, counter, err, 0, d.ntiles, s));
+                                         (RunSum *)c->runs.ptr, counter, err, 0, wide, d.ntiles,
+                                         s));
                 ++c->timing.launches;
             }
         } else {
-            const u32 grid = (u32)std::min<u64>((u64)c->cus * blocks_per_cu(c), tiles);
-            HIPCK(c, rbg_launch_scan(dd, np, 0, (u32)tiles, tile_chunks(c),
-                                     (const u32 *)c->tab_val.ptr, (const u16 *)c->tab_code.ptr,
-                                     (TileHdr *)c->hdr.ptr, (RunSum *)c->runs.ptr, counter, err, 1,
-                                     std::max(1u, grid), s));
-            ++c->timing.launches;
+            // one persistent launch per domain over its contiguous tile range
+            const u64 ranges[2][2] = {{0, tiles32}, {tiles32, tiles}};
+            for (int wide = 0; wide < 2; ++wide) {
+                const u64 t0 = ranges[wide][0], t1 = ranges[wide][1];
+                if (t1 == t0) continue;
+                const u32 grid = (u32)std::min<u64>((u64)c->cus * blocks_per_cu(c), t1 - t0);
+                HIPCK(c, rbg_launch_scan(dd, np, (u32)t0, (u32)t1, tile_chunks(c),
+                                         wide ? c->tab_val64.ptr : c->tab_val.ptr,
+                                         (const u16 *)c->tab_code.ptr, (TileHdr *)c->hdr.ptr,
+                                         (RunSum *)c->runs.ptr, wide ? counter64 : counter, err, 1,
+                                         wide, std::max(1u, grid), s));
+                ++c->timing.launches;
+            }
         }
         // the scan time excludes the merge when the chain runs eagerly
         if (!c->graph) HIPCK(c, hipEventRecord(c->ev[kDense1], s));
@@ -442,47 +552,55 @@ int dense_run(rbg_ctx *c, const u32 *primes, const std::vector<u32> &idx, bool p
 
 // Plans the sparse primes; indices that the sparse engine cannot take are
 // appended to `rest`.
-int sparse_plan(rbg_ctx *c, const u32 *primes, const std::vector<u32> &idx,
+int sparse_plan(rbg_ctx *c, const u64 *primes, const std::vector<u32> &idx_in,
                 std::vector<u32> &rest) {
+    const std::vector<u32> idx = by_domain(primes, idx_in);
     c->h_sp.clear();
     c->h_prof.clear();
     c->h_sval.clear();
+    c->h_sval64.clear();
     c->h_scode.clear();
     c->h_slab.clear();
-    u64 lab = 0, inv = 0, words = 0, lblk = 0, iblk = 0, pblk = 0;
+    c->nprof32 = 0;
+    u64 lab = 0, inv = 0, inv64 = 0, words = 0, lblk = 0, iblk = 0, pblk = 0;
     for (u32 i : idx) {
-        const u32 p = primes[i];
+        const u64 p = primes[i];
+        const bool wide = p >= (1ull << 32);
         u32 B = 0, D = 0;
         if (p < kSparseMin || !choose_cover(p, kCoverQ, B, D)) {
             rest.push_back(i);
             continue;
         }
+        const MontConsts k = mont_consts(p);
         SparsePrime s{};
-        s.p = p;
+        s.p = (u32)p;
         s.mask = mask_of(p);
         s.m = lcm_of(s.mask);
-        s.e = (p - 1) / s.m;
-        s.ninv = rbg::neg_inverse32(p);
-        s.one = (u32)((1ull << 32) % p);
-        s.r2 = (u32)((u64)s.one * s.one % p);
+        s.e = (u32)((p - 1) / s.m);
+        s.ninv = k.ninv;
+        s.one = k.one;
+        s.r2 = k.r2;
+        s.hi = k.hi & 7u;
         s.index = i;
         s.B = B;
         s.D = D;
         s.Q = kCoverQ;
         s.jneg = (s.e & 1u) ? s.m / 2 : 0;  // (-1)^e = z^(m/2) when e is odd
-        s.tab_off = (u32)c->h_sval.size();
+        s.tab_off = (u32)c->h_slab.size();
+        s.tab64 = (u32)c->h_sval64.size();
         s.lab_off = (u32)lab;
-        s.inv_off = (u32)inv;
+        s.inv_off = (u32)(wide ? inv64 : inv);
         s.prof_begin = (u32)c->h_prof.size();
         s.lblk = (u32)lblk;
         s.iblk = (u32)iblk;
         s.pblk = (u32)pblk;
         lab += B + 1ull;
-        inv += D + 1ull;
+        (wide ? inv64 : inv) += D + 1ull;
         lblk += (B + 2047ull) / 2048ull;
         iblk += (D + 1ull + 255ull) / 256ull;
-        if (!root_table(p, s.m, c->h_sval, c->h_scode, &c->h_slab))
-            return c->fail(RBG_EINVAL, "no root of the required order");
+        const bool ok = wide ? root_table(p, s.m, c->h_sval64, c->h_scode, &c->h_slab)
+                             : root_table(p, s.m, c->h_sval, c->h_scode, &c->h_slab);
+        if (!ok) return c->fail(RBG_EINVAL, "no root of the required order");
         for (u32 r = 2; r <= 9; ++r)
             if (s.mask & (1u << (r - 2))) {
                 SparseProf f{};
@@ -497,6 +615,7 @@ int sparse_plan(rbg_ctx *c, const u32 *primes, const std::vector<u32> &idx,
                 pblk += ((f.wpp + 1ull) / 2ull + 63ull) / 64ull;
                 c->h_prof.push_back(f);
                 ++s.nprof;
+                if (!wide) ++c->nprof32;
             }
         c->h_sp.push_back(s);
         if (lab > 0xFFFFFFFFull || words > 0xFFFFFFFFull || pblk > 0x7FFFFFFFull)
@@ -510,13 +629,15 @@ int sparse_plan(rbg_ctx *c, const u32 *primes, const std::vector<u32> &idx,
 int sparse_run(rbg_ctx *c, rbg_record *recs, u32 *err, std::vector<u32> &failed_idx) {
     const u32 np = (u32)c->h_sp.size(), nf = (u32)c->h_prof.size();
     if (!np) return RBG_OK;
-    u64 lab = 0, inv = 0, words = 0, lblk = 0, iblk = 0, pblk = 0, dmax = 0, bmax = 0;
+    u64 lab = 0, inv = 0, inv64 = 0, words = 0, lblk = 0, iblk = 0, pblk = 0, bmax = 0;
+    u64 dmax[2] = {0, 0};  // largest denominator per domain
     for (const SparsePrime &s : c->h_sp) {
+        const bool wide = s.hi & 1u;
         lab += s.B + 1ull;
-        inv += s.D + 1ull;
+        (wide ? inv64 : inv) += s.D + 1ull;
         lblk += (s.B + 2047ull) / 2048ull;
         iblk += (s.D + 1ull + 255ull) / 256ull;
-        dmax = std::max<u64>(dmax, s.D);
+        dmax[wide] = std::max<u64>(dmax[wide], s.D);
         bmax = std::max<u64>(bmax, s.B);
     }
     const u32 segs_per_prof = rbg_seed_segments((u32)bmax);
@@ -525,14 +646,17 @@ int sparse_run(rbg_ctx *c, rbg_record *recs, u32 *err, std::vector<u32> &failed_
         pblk += ((f.wpp + 1ull) / 2ull + 63ull) / 64ull;
     }
     const u64 bytes = np * sizeof(SparsePrime) + nf * sizeof(SparseProf) + c->h_sval.size() * 4 +
-                      c->h_slab.size() * 2 + lab * 2 + inv * 4 + words * 4 + nf * 8 + np * 8;
+                      c->h_sval64.size() * 8 + c->h_slab.size() * 2 + lab * 2 + inv * 4 +
+                      inv64 * 8 + words * 4 + nf * 8 + np * 8;
     if (bytes > budget(c)) return c->fail(RBG_ECAPACITY, "sparse workspace exceeds the HBM budget");
     HIPCK(c, c->sp_primes.ensure(np * sizeof(SparsePrime)));
     HIPCK(c, c->sp_profs.ensure(nf * sizeof(SparseProf)));
     HIPCK(c, c->sp_val.ensure(c->h_sval.size() * 4));
+    HIPCK(c, c->sp_val64.ensure(c->h_sval64.size() * 8));
     HIPCK(c, c->sp_lab.ensure(c->h_slab.size() * 2));
     HIPCK(c, c->sp_labels.ensure(lab * 2));
     HIPCK(c, c->sp_inv.ensure(inv * 4));
+    HIPCK(c, c->sp_inv64.ensure(inv64 * 8));
     HIPCK(c, c->sp_planes.ensure(words * 4));
     HIPCK(c, c->sp_best.ensure(nf * 8));
     HIPCK(c, c->sp_failed.ensure(np * 4));
@@ -541,10 +665,11 @@ int sparse_run(rbg_ctx *c, rbg_record *recs, u32 *err, std::vector<u32> &failed_
     HIPCK(c, c->sp_segs.ensure((u64)nf * segs_per_prof * rbg_seed_agg_bytes()));
     hipStream_t s = c->stream;
     // A4 by elliptic point counts on the host while the device works
-    std::vector<u32> sp_p(np), a4(np, 0);
-    for (u32 i = 0; i < np; ++i) sp_p[i] = c->h_sp[i].p;
+    std::vector<u64> sp_p(np);
+    std::vector<u32> a4(np, 0);
+    for (u32 i = 0; i < np; ++i) sp_p[i] = c->h_sp[i].p | ((u64)(c->h_sp[i].hi & 1u) << 32);
     int a4_rc = RBG_OK;
-    std::thread a4_thread([&] { a4_rc = rbg_a4_host(sp_p.data(), np, a4.data(), 0); });
+    std::thread a4_thread([&] { a4_rc = rbg_a4_host_wide(sp_p.data(), np, a4.data(), 0); });
     struct Joiner {
         std::thread &t;
         ~Joiner() {
@@ -555,8 +680,12 @@ int sparse_run(rbg_ctx *c, rbg_record *recs, u32 *err, std::vector<u32> &failed_
                             hipMemcpyHostToDevice, s));
     HIPCK(c, hipMemcpyAsync(c->sp_profs.ptr, c->h_prof.data(), nf * sizeof(SparseProf),
                             hipMemcpyHostToDevice, s));
-    HIPCK(c, hipMemcpyAsync(c->sp_val.ptr, c->h_sval.data(), c->h_sval.size() * 4,
-                            hipMemcpyHostToDevice, s));
+    if (!c->h_sval.empty())
+        HIPCK(c, hipMemcpyAsync(c->sp_val.ptr, c->h_sval.data(), c->h_sval.size() * 4,
+                                hipMemcpyHostToDevice, s));
+    if (!c->h_sval64.empty())
+        HIPCK(c, hipMemcpyAsync(c->sp_val64.ptr, c->h_sval64.data(), c->h_sval64.size() * 8,
+                                hipMemcpyHostToDevice, s));
     HIPCK(c, hipMemcpyAsync(c->sp_lab.ptr, c->h_slab.data(), c->h_slab.size() * 2,
                             hipMemcpyHostToDevice, s));
     const SparsePrime *dP = (const SparsePrime *)c->sp_primes.ptr;
@@ -565,7 +694,7 @@ int sparse_run(rbg_ctx *c, rbg_record *recs, u32 *err, std::vector<u32> &failed_
     u32 *failed = (u32 *)c->sp_failed.ptr;
     // the filter's XCD-aware schedule needs a grid that is a multiple of 8
     const u32 grid = std::max(8u, ((u32)c->cus * blocks_per_cu(c) + 7u) / 8u * 8u);
-    if (dmax * (nf + 8ull * xcd_group(c)) > 0xFFFFFFFFull)
+    if (std::max(dmax[0], dmax[1]) * (nf + 8ull * xcd_group(c)) > 0xFFFFFFFFull)
         return c->fail(RBG_ECAPACITY, "too many filter tasks");
     // one chain: clear flags and counters, tables and planes, seed, filter
     // (eagerly with an event per stage; in graph mode the whole chain is
@@ -577,8 +706,9 @@ int sparse_run(rbg_ctx *c, rbg_record *recs, u32 *err, std::vector<u32> &failed_
         HIPCK(c, rbg_fill_u32(failed, np, 0u, s));
         HIPCK(c, rbg_fill_u32((u32 *)c->sp_stats.ptr, (u32)(kStatsBytes / 4), 0u, s));
         HIPCK(c, rbg_sparse_prepare(dP, np, dF, nf, (u32)lblk, (u32)iblk, (u32)pblk,
-                                    (const u32 *)c->sp_val.ptr, (const u16 *)c->sp_lab.ptr,
-                                    (u16 *)c->sp_labels.ptr, (u32 *)c->sp_inv.ptr,
+                                    (const u32 *)c->sp_val.ptr, (const u64 *)c->sp_val64.ptr,
+                                    (const u16 *)c->sp_lab.ptr, (u16 *)c->sp_labels.ptr,
+                                    (u32 *)c->sp_inv.ptr, (u64 *)c->sp_inv64.ptr,
                                     (u32 *)c->sp_planes.ptr, err, s));
         if (!c->graph) HIPCK(c, hipEventRecord(c->ev[kPrep1], s));
         if (!c->graph) HIPCK(c, hipEventRecord(c->ev[kSeed0], s));
@@ -586,12 +716,21 @@ int sparse_run(rbg_ctx *c, rbg_record *recs, u32 *err, std::vector<u32> &failed_
                                  segs_per_prof, best, failed, s));
         if (!c->graph) HIPCK(c, hipEventRecord(c->ev[kSeed1], s));
         if (!c->graph) HIPCK(c, hipEventRecord(c->ev[kFilt0], s));
-        HIPCK(c, rbg_sparse_filter(dP, dF, nf, (u32)dmax, (const u32 *)c->sp_planes.ptr,
-                                   (const u32 *)c->sp_inv.ptr, (const u32 *)c->sp_val.ptr,
-                                   (const u16 *)c->sp_lab.ptr, best, failed,
-                                   (unsigned long long *)c->sp_stats.ptr,
-                                   (u32 *)((char *)c->sp_stats.ptr + kQueueOffset), grid,
-                                   xcd_group(c), s));
+        // one filter launch per domain, each over its own profiles (the
+        // profiles of the primes below 2^32 come first)
+        const u32 nfd[2] = {c->nprof32, nf - c->nprof32};
+        for (int wide = 0; wide < 2; ++wide) {
+            if (!nfd[wide]) continue;
+            const u32 f0 = wide ? c->nprof32 : 0u;
+            HIPCK(c, rbg_sparse_filter(dP, dF + f0, nfd[wide], (u32)dmax[wide],
+                                       (const u32 *)c->sp_planes.ptr,
+                                       wide ? c->sp_inv64.ptr : c->sp_inv.ptr,
+                                       wide ? c->sp_val64.ptr : c->sp_val.ptr,
+                                       (const u16 *)c->sp_lab.ptr, best + f0, failed,
+                                       (unsigned long long *)c->sp_stats.ptr,
+                                       (u32 *)((char *)c->sp_stats.ptr + kQueueOffset) + 8 * wide,
+                                       wide, grid, xcd_group(c), s));
+        }
         return RBG_OK;
     });
     if (chain_rc) return chain_rc;
@@ -621,14 +760,15 @@ int sparse_run(rbg_ctx *c, rbg_record *recs, u32 *err, std::vector<u32> &failed_
 
 // Device workspace one prime needs under `flags` (the planning rules of
 // sparse_plan and dense_run).
-u64 prime_bytes(u32 p, u32 flags, u32 tile_pos) {
+u64 prime_bytes(u64 p, u32 flags, u32 tile_pos) {
     if (p < 3) return sizeof(rbg_record);
     const u32 mask = mask_of(p), m = lcm_of(mask), nr = (u32)__builtin_popcount(mask);
-    u64 bytes = sizeof(rbg_record) + (u64)m * 8 + 64;
+    const u64 vbytes = p >= (1ull << 32) ? 8 : 4;  // root-table and inverse values
+    u64 bytes = sizeof(rbg_record) + (u64)m * (vbytes + 4) + 64;
     u32 B = 0, D = 0;
     if ((flags & RBG_METHOD_MASK) == RBG_METHOD_BATCHED && p >= kSparseMin &&
         choose_cover(p, kCoverQ, B, D)) {
-        bytes += 2ull * (B + 1) + 4ull * (D + 1) + 80;
+        bytes += 2ull * (B + 1) + vbytes * (D + 1) + 88;
         for (u32 r = 2; r <= 9; ++r)
             if (mask & (1u << (r - 2)))
                 bytes += 4ull * (32u - (u32)__builtin_clz(r - 1)) * ((2u * B + 192u) / 32u + 2u) + 48;
@@ -638,8 +778,11 @@ u64 prime_bytes(u32 p, u32 flags, u32 tile_pos) {
     return bytes + tiles * (sizeof(TileHdr) + nr * sizeof(RunSum));
 }
 
-int run_block(rbg_ctx *c, const u32 *h_primes, u32 n, u32 flags, rbg_record *d_out,
-              rbg_record *h_out, u32 limit = RBG_MAX_BLOCK) {
+// Profiles a block of primes below `pmax` (2^32 for the 32-bit entry points,
+// 2^33 for the wide ones).  Records keep the low word of p; the wide entry
+// points widen them.
+int run_block(rbg_ctx *c, const u64 *h_primes, u32 n, u32 flags, rbg_record *d_out,
+              rbg_record *h_out, u32 limit = RBG_MAX_BLOCK, u64 pmax = 1ull << 32) {
     if (n > limit) return c->fail(RBG_EINVAL, "more than 4096 primes in one block");
     c->error.clear();
     c->timing = rbg_timing{};
@@ -648,16 +791,17 @@ int run_block(rbg_ctx *c, const u32 *h_primes, u32 n, u32 flags, rbg_record *d_o
     if (method > RBG_METHOD_DENSE) return c->fail(RBG_EINVAL, "unknown method");
     std::vector<u32> odd, dense, sparse, failed;
     for (u32 i = 0; i < n; ++i) {
-        const u32 p = h_primes[i];
-        if (!prime_u32(p)) {
-            char msg[96];
-            snprintf(msg, sizeof msg, "not a prime below 2^32: %u (index %u)", p, i);
+        const u64 p = h_primes[i];
+        if (p >= pmax || !prime_u64(p)) {
+            char msg[128];
+            snprintf(msg, sizeof msg, "not a prime below 2^%d: %llu (index %u)",
+                     pmax > (1ull << 32) ? 33 : 32, (unsigned long long)p, i);
             return c->fail(RBG_EINVAL, msg);
         }
         if (p != 2) odd.push_back(i);
     }
     HIPCK(c, hipSetDevice(c->device));
-    HIPCK(c, c->flags.ensure(2 * sizeof(u32)));
+    HIPCK(c, c->flags.ensure(3 * sizeof(u32)));  // tile counter, error flags, tile counter
     rbg_record *recs = d_out;
     if (!recs) {
         HIPCK(c, c->recs.ensure((size_t)std::max(1u, n) * sizeof(rbg_record)));
@@ -697,8 +841,66 @@ int run_block(rbg_ctx *c, const u32 *h_primes, u32 n, u32 flags, rbg_record *d_o
     if (c->timing.dense_primes) c->timing.dense_scan_ms = elapsed(c, kDense0, kDense1);
     if (h_err & 1u) return c->fail(RBG_EDEVICE, "character value missing from the root table");
     if (h_err & 2u) return c->fail(RBG_EDEVICE, "incomplete coverage of the half interval");
-    if ((flags & RBG_CERTIFY) && h_out) return certify_all(c, h_out, n);
+    if ((flags & RBG_CERTIFY) && h_out) return certify_all(c, h_out, h_primes, n);
     return RBG_OK;
+}
+
+std::vector<u64> widen(const uint32_t *primes, u32 n) { return std::vector<u64>(primes, primes + n); }
+
+// rbg_profile_batch over 64-bit primes: greedy sub-batches under the HBM
+// budget and the chunk limit, records into out (low word of p).
+int run_batch(rbg_ctx *c, const u64 *primes, u32 n, u32 flags, rbg_record *out, u64 pmax) {
+    const u32 tile_pos = (u32)rbg_waves_per_block() * tile_chunks(c) * 64u;
+    const u64 cap = budget(c) / 10 * 9;
+    rbg_timing sum{};
+    u32 first = 0;
+    while (first < n) {
+        u32 last = first;
+        u64 bytes = 0;
+        while (last < n && last - first < kBatchChunk) {
+            const u64 b = prime_bytes(primes[last], flags, tile_pos);
+            if (last > first && bytes + b > cap) break;
+            bytes += b;
+            ++last;
+        }
+        int rc = run_block(c, primes + first, last - first, flags, nullptr, out + first,
+                           kBatchChunk, pmax);
+        if (rc) return rc;
+        for (u32 i = first; i < last; ++i) out[i].status = 0;
+        const rbg_timing &t = c->timing;
+        sum.total_ms += t.total_ms;
+        sum.dense_scan_ms += t.dense_scan_ms;
+        sum.sparse_prep_ms += t.sparse_prep_ms;
+        sum.sparse_seed_ms += t.sparse_seed_ms;
+        sum.sparse_filter_ms += t.sparse_filter_ms;
+        sum.dense_primes += t.dense_primes;
+        sum.sparse_primes += t.sparse_primes;
+        sum.sparse_profiles += t.sparse_profiles;
+        sum.fallbacks += t.fallbacks;
+        sum.positions += t.positions;
+        sum.products += t.products;
+        sum.launches += t.launches;
+        sum.filter_windows += t.filter_windows;
+        sum.filter_words += t.filter_words;
+        sum.filter_completions += t.filter_completions;
+        first = last;
+    }
+    c->timing = sum;
+    return RBG_OK;
+}
+
+void widen_records(const u64 *primes, const rbg_record *in, u32 n, rbg_record_wide *out) {
+    for (u32 i = 0; i < n; ++i) {
+        rbg_record_wide &w = out[i];
+        w.p = primes[i];
+        w.mask = in[i].mask;
+        std::memcpy(w.M, in[i].M, sizeof w.M);
+        std::memcpy(w.L, in[i].L, sizeof w.L);
+        std::memcpy(w.W, in[i].W, sizeof w.W);
+        std::memcpy(w.K, in[i].K, sizeof w.K);
+        w.A4 = in[i].A4;
+        w.status = in[i].status;
+    }
 }
 
 }  // namespace
@@ -735,9 +937,48 @@ rbg_ctx *rbg_open(const rbg_config *cfg) {
 int rbg_profile_block(rbg_ctx *c, const uint32_t *primes, uint32_t n, uint32_t flags,
                       rbg_record *out) {
     if (!c || (!primes && n) || (!out && n)) return RBG_EINVAL;
-    int rc = run_block(c, primes, n, flags, nullptr, out);
+    const std::vector<u64> wp = widen(primes, n);
+    int rc = run_block(c, wp.data(), n, flags, nullptr, out);
     if (rc && out) memset(out, 0, sizeof(rbg_record) * n);
     return rc;
+}
+
+int rbg_profile_block_wide(rbg_ctx *c, const uint64_t *primes, uint32_t n, uint32_t flags,
+                           rbg_record_wide *out) {
+    if (!c || (!primes && n) || (!out && n)) return RBG_EINVAL;
+    if (n > RBG_MAX_BLOCK) return c->fail(RBG_EINVAL, "more than 4096 primes in one block");
+    std::vector<rbg_record> recs(n);
+    int rc = run_block(c, primes, n, flags, nullptr, recs.data(), RBG_MAX_BLOCK, kWideLimit);
+    if (rc) {
+        if (out) memset(out, 0, sizeof(rbg_record_wide) * n);
+        return rc;
+    }
+    widen_records(primes, recs.data(), n, out);
+    return RBG_OK;
+}
+
+int rbg_profile_batch_wide(rbg_ctx *c, const uint64_t *primes, uint32_t n, uint32_t flags,
+                           rbg_record_wide *out) {
+    if (!c || (!primes && n) || (!out && n)) return RBG_EINVAL;
+    if (n > RBG_MAX_BATCH) return c->fail(RBG_EINVAL, "batch larger than RBG_MAX_BATCH");
+    std::vector<rbg_record> recs(n);
+    int rc = run_batch(c, primes, n, flags, recs.data(), kWideLimit);
+    if (rc) {
+        if (out) memset(out, 0, sizeof(rbg_record_wide) * n);
+        return rc;
+    }
+    widen_records(primes, recs.data(), n, out);
+    return RBG_OK;
+}
+
+int rbg_estimate_bytes_wide(const rbg_ctx *c, const uint64_t *primes, uint32_t n, uint32_t flags,
+                            uint64_t *bytes) {
+    if (!c || (!primes && n) || !bytes) return RBG_EINVAL;
+    const u32 tile_pos = (u32)rbg_waves_per_block() * tile_chunks(c) * 64u;
+    u64 total = 0;
+    for (u32 i = 0; i < n; ++i) total += prime_bytes(primes[i], flags, tile_pos);
+    *bytes = total;
+    return RBG_OK;
 }
 
 int rbg_estimate_bytes(const rbg_ctx *c, const uint32_t *primes, uint32_t n, uint32_t flags,
@@ -754,47 +995,10 @@ int rbg_profile_batch(rbg_ctx *c, const uint32_t *primes, uint32_t n, uint32_t f
                       rbg_record *out) {
     if (!c || (!primes && n) || (!out && n)) return RBG_EINVAL;
     if (n > RBG_MAX_BATCH) return c->fail(RBG_EINVAL, "batch larger than RBG_MAX_BATCH");
-    const u32 tile_pos = (u32)rbg_waves_per_block() * tile_chunks(c) * 64u;
-    const u64 cap = budget(c) / 10 * 9;
-    rbg_timing sum{};
-    u32 first = 0;
-    while (first < n) {
-        // greedy sub-batch under the HBM budget and the task-count limit
-        u32 last = first;
-        u64 bytes = 0;
-        while (last < n && last - first < kBatchChunk) {
-            const u64 b = prime_bytes(primes[last], flags, tile_pos);
-            if (last > first && bytes + b > cap) break;
-            bytes += b;
-            ++last;
-        }
-        int rc = run_block(c, primes + first, last - first, flags, nullptr, out + first,
-                           kBatchChunk);
-        if (rc) {
-            memset(out, 0, sizeof(rbg_record) * n);
-            return rc;
-        }
-        for (u32 i = first; i < last; ++i) out[i].status = 0;
-        const rbg_timing &t = c->timing;
-        sum.total_ms += t.total_ms;
-        sum.dense_scan_ms += t.dense_scan_ms;
-        sum.sparse_prep_ms += t.sparse_prep_ms;
-        sum.sparse_seed_ms += t.sparse_seed_ms;
-        sum.sparse_filter_ms += t.sparse_filter_ms;
-        sum.dense_primes += t.dense_primes;
-        sum.sparse_primes += t.sparse_primes;
-        sum.sparse_profiles += t.sparse_profiles;
-        sum.fallbacks += t.fallbacks;
-        sum.positions += t.positions;
-        sum.products += t.products;
-        sum.launches += t.launches;
-        sum.filter_windows += t.filter_windows;
-        sum.filter_words += t.filter_words;
-        sum.filter_completions += t.filter_completions;
-        first = last;
-    }
-    c->timing = sum;
-    return RBG_OK;
+    const std::vector<u64> wp = widen(primes, n);
+    int rc = run_batch(c, wp.data(), n, flags, out, 1ull << 32);
+    if (rc) memset(out, 0, sizeof(rbg_record) * n);
+    return rc;
 }
 
 int rbg_autotune(rbg_ctx *c, const uint32_t *primes, uint32_t n, uint32_t flags,
@@ -815,6 +1019,7 @@ int rbg_autotune(rbg_ctx *c, const uint32_t *primes, uint32_t n, uint32_t flags,
             for (u32 bpc : {4u, 8u}) cands.push_back({ch, bpc, c->cfg.xcd_group});
     }
     const rbg_config saved = c->cfg;
+    const std::vector<u64> wp = widen(primes, n);
     std::vector<rbg_record> ref(n), got(n);
     double best_ms = 1e300;
     Cand best = cands[0];
@@ -825,7 +1030,7 @@ int rbg_autotune(rbg_ctx *c, const uint32_t *primes, uint32_t n, uint32_t flags,
         c->cfg.xcd_group = k.group;
         double ms = 1e300;
         for (int rep = 0; rep < 2; ++rep) {
-            int rc = run_block(c, primes, n, method, nullptr, got.data());
+            int rc = run_block(c, wp.data(), n, method, nullptr, got.data());
             if (rc) {
                 c->cfg = saved;
                 return rc;
@@ -862,7 +1067,8 @@ int rbg_profile_block_device(rbg_ctx *c, const uint32_t *d_primes, uint32_t n, u
     if (n > RBG_MAX_BLOCK) return c->fail(RBG_EINVAL, "more than 4096 primes in one block");
     std::vector<u32> h(n);
     if (n) HIPCK(c, hipMemcpy(h.data(), d_primes, n * 4, hipMemcpyDeviceToHost));
-    return run_block(c, h.data(), n, flags & ~RBG_CERTIFY, d_out, nullptr);
+    const std::vector<u64> wp = widen(h.data(), n);
+    return run_block(c, wp.data(), n, flags & ~RBG_CERTIFY, d_out, nullptr);
 }
 
 const char *rbg_last_error(const rbg_ctx *c) { return c ? c->error.c_str() : "no context"; }
@@ -884,10 +1090,10 @@ int rbg_device_info(const rbg_ctx *c, uint32_t *cus, uint32_t *clock_khz, uint64
 void rbg_close(rbg_ctx *c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
-    for (DevBuf *b : {&c->descs, &c->tab_val, &c->tab_code, &c->hdr, &c->runs, &c->recs, &c->flags,
-                      &c->sp_primes, &c->sp_profs, &c->sp_val, &c->sp_lab, &c->sp_labels,
-                      &c->sp_inv, &c->sp_planes, &c->sp_best, &c->sp_failed, &c->sp_a4,
-                      &c->sp_stats, &c->sp_segs})
+    for (DevBuf *b : {&c->descs, &c->tab_val, &c->tab_val64, &c->tab_code, &c->hdr, &c->runs,
+                      &c->recs, &c->flags, &c->sp_primes, &c->sp_profs, &c->sp_val, &c->sp_val64,
+                      &c->sp_lab, &c->sp_labels, &c->sp_inv, &c->sp_inv64, &c->sp_planes,
+                      &c->sp_best, &c->sp_failed, &c->sp_a4, &c->sp_stats, &c->sp_segs})
         b->release();
     for (int i = 0; i < kNumEv; ++i)
         if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
@@ -908,6 +1114,18 @@ uint32_t rbg_primes_from(uint64_t start, uint32_t count, uint32_t *out) {
 }
 
 int rbg_is_prime(uint32_t p) { return prime_u32(p) ? 1 : 0; }
+
+int rbg_is_prime_wide(uint64_t p) { return p < kWideLimit && prime_u64(p) ? 1 : 0; }
+
+uint32_t rbg_mask_wide(uint64_t p) { return mask_of(p); }
+
+uint32_t rbg_primes_from_wide(uint64_t start, uint32_t count, uint64_t *out) {
+    if (!out) return 0;
+    u32 k = 0;
+    for (u64 x = start < 2 ? 2 : start; k < count && x < kWideLimit; ++x)
+        if (prime_u64(x)) out[k++] = x;
+    return k;
+}
 
 const char *rbg_version(void) { return "rabung-mi355x 0.2 (gfx950 dense scan + sparse anchor filter)"; }
 

@@ -60,6 +60,16 @@ __global__ __launch_bounds__(256) void load_rate(const u32 *__restrict__ buf, u3
     if (acc == 0x9E3779B9u) out[0] = acc;
 }
 
+// Wide residue arithmetic on given operands (2^32 < p < 2^33): kind 0
+// out[i] = a[i] b[i] 2^-32 mod p (one Montgomery product), kind 1
+// out[i] = Montgomery power of a[i] to the exponent b[i].
+__global__ void wide_check(const u64 *a, const u64 *b, u32 n, u64 p, u32 ninv, u64 one, int kind,
+                           u64 *out) {
+    const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    out[i] = kind == 0 ? mont_mul33(a[i], b[i], p, ninv) : mont_pow33(a[i], b[i], p, ninv, one);
+}
+
 }  // namespace rbg
 
 namespace {
@@ -159,5 +169,35 @@ extern "C" int rbg_microbench(int device, int kind, uint32_t param, double *out)
         rc = RBG_EINVAL;
     }
     (void)hipFree(sink);
+    return rc;
+}
+
+// Device check of the wide residue arithmetic (a test hook: the kernels'
+// 33-bit Montgomery product and power on caller operands, host arrays).
+extern "C" int rbg_check_wide(int device, const uint64_t *a, const uint64_t *b, uint32_t n,
+                              uint64_t p, int kind, uint64_t *out) {
+    if (!a || !b || !out || !n || p <= (1ull << 32) || p >= (1ull << 33) || !(p & 1) ||
+        (kind != 0 && kind != 1))
+        return RBG_EINVAL;
+    for (uint32_t i = 0; i < n; ++i)
+        if (a[i] >= p || (kind == 0 && b[i] >= p)) return RBG_EINVAL;
+    if (hipSetDevice(device) != hipSuccess) return RBG_EDEVICE;
+    uint32_t ninv = (uint32_t)p;  // -p^{-1} mod 2^32 (Newton)
+    for (int i = 0; i < 5; ++i) ninv *= 2u - (uint32_t)p * ninv;
+    ninv = 0u - ninv;
+    uint64_t *d = nullptr;
+    if (hipMalloc(&d, 3ull * n * 8) != hipSuccess) return RBG_EDEVICE;
+    int rc = RBG_OK;
+    if (hipMemcpy(d, a, n * 8ull, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(d + n, b, n * 8ull, hipMemcpyHostToDevice) != hipSuccess) {
+        rc = RBG_EDEVICE;
+    } else {
+        hipLaunchKernelGGL(rbg::wide_check, dim3((n + 255) / 256), dim3(256), 0, 0, d, d + n, n, p,
+                           ninv, 1ull << 32, kind, d + 2ull * n);
+        if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess ||
+            hipMemcpy(out, d + 2ull * n, n * 8ull, hipMemcpyDeviceToHost) != hipSuccess)
+            rc = RBG_EDEVICE;
+    }
+    (void)hipFree(d);
     return rc;
 }

@@ -34,12 +34,32 @@
 
 namespace rbg {
 
+// Residue constants of a prime in the kernel's domain (the 33rd bits of the
+// wide primes are kept in d.hi).
+template <class D>
+struct PrimeConsts {
+    D dom;
+    typename D::V one, r2, step;
+};
+
+__device__ __forceinline__ PrimeConsts<Dom32> consts_of(const PrimeDesc &d, Dom32) {
+    return {{d.p, d.ninv}, d.one, d.r2, d.step};
+}
+
+__device__ __forceinline__ PrimeConsts<Dom33> consts_of(const PrimeDesc &d, Dom33) {
+    const auto w = [&](u32 lo, int bit) { return (u64)lo | ((u64)((d.hi >> bit) & 1u) << 32); };
+    return {{w(d.p, 0), d.ninv}, w(d.one, 1), w(d.r2, 2), w(d.step, 3)};
+}
+
 // Scan one segment [seg_start, seg_start+seg_len) with the calling wave.
 // Writes per-order summaries (positions relative to seg_start) into out[r-2].
-__device__ void wave_scan(const PrimeDesc &d, u32 seg_start, u32 seg_len, const u32 *s_val,
-                          const u16 *s_code, RunSum *out, u32 &codes_out, u32 *err) {
+template <class D>
+__device__ void wave_scan(const PrimeDesc &d, const PrimeConsts<D> &k, u32 seg_start, u32 seg_len,
+                          const typename D::V *s_val, const u16 *s_code, RunSum *out,
+                          u32 &codes_out, u32 *err) {
+    typedef typename D::V V;
     const u32 lane = threadIdx.x & 63u;
-    const u32 p = d.p, ninv = d.ninv, e = d.e, one = d.one, m = d.m, mask = d.mask;
+    const u32 e = d.e, m = d.m, mask = d.mask;
     OrderState st[8];
 #pragma unroll
     for (int k = 0; k < 8; ++k) order_init(st[k]);
@@ -47,7 +67,7 @@ __device__ void wave_scan(const PrimeDesc &d, u32 seg_start, u32 seg_len, const 
     u64 prevN = 0;
     u32 carry_code = 0, first_code = 0, last_code = 0;
     // Montgomery form of this lane's first position
-    u32 xm = mont_mul(seg_start + lane, d.r2, p, ninv);
+    V xm = k.dom.mul((V)(seg_start + lane), k.r2);
     const u32 nchunks = (seg_len + 63u) >> 6;
     u32 bad = 0;
     for (u32 c = 0; c < nchunks; ++c) {
@@ -55,7 +75,7 @@ __device__ void wave_scan(const PrimeDesc &d, u32 seg_start, u32 seg_len, const 
         const u32 valid = min(64u, seg_len - base);
         const u64 vmask = valid == 64u ? ~0ull : ((1ull << valid) - 1ull);
         // colour code of x: y = x^e, then its index in the sorted root table
-        const u32 y = mont_pow(xm, e, p, ninv, one);
+        const V y = k.dom.pow(xm, e, k.one);
         const u32 lo = table_index(s_val, m, y);
         bad |= (s_val[lo] != y) & (lane < valid);
         u32 code = s_code[lo];
@@ -79,7 +99,7 @@ __device__ void wave_scan(const PrimeDesc &d, u32 seg_start, u32 seg_len, const 
             }
             order_update(st[k], C, N, base);
         }
-        xm = add_mod(xm, d.step, p);
+        xm = k.dom.add(xm, k.step);
     }
     if (__builtin_amdgcn_ballot_w64(bad != 0)) atomicOr(err, 1u);
 #pragma unroll
@@ -102,11 +122,15 @@ __device__ void wave_scan(const PrimeDesc &d, u32 seg_start, u32 seg_len, const 
 
 constexpr int kWavesPerBlock = 4;
 
+// D = Dom32 for primes below 2^32, Dom33 for 2^32 < p < 2^33 (the tiles of
+// one launch belong to primes of one domain).
+template <class D>
 __global__ __launch_bounds__(kWavesPerBlock * 64) void scan_tiles(
     const PrimeDesc *__restrict__ descs, u32 nprimes, u32 tile_first, u32 tile_end, u32 chunks,
-    const u32 *__restrict__ tab_val, const u16 *__restrict__ tab_code, TileHdr *__restrict__ hdr,
-    RunSum *__restrict__ runs, u32 *__restrict__ counter, u32 *__restrict__ err, int persistent) {
-    __shared__ u32 s_val[kMaxM];
+    const typename D::V *__restrict__ tab_val, const u16 *__restrict__ tab_code,
+    TileHdr *__restrict__ hdr, RunSum *__restrict__ runs, u32 *__restrict__ counter,
+    u32 *__restrict__ err, int persistent) {
+    __shared__ typename D::V s_val[kMaxM];
     __shared__ u16 s_code[kMaxM];
     __shared__ RunSum s_run[kWavesPerBlock][8];
     __shared__ u32 s_codes[kWavesPerBlock];
@@ -144,8 +168,9 @@ __global__ __launch_bounds__(kWavesPerBlock * 64) void scan_tiles(
         const u32 pi = __builtin_amdgcn_readfirstlane(s_pi);
         const PrimeDesc d = descs[pi];
         if (pi != cur_pi) {
+            const u32 voff = D::wide ? d.tab64 : d.tab_off;
             for (u32 j = threadIdx.x; j < d.m; j += blockDim.x) {
-                s_val[j] = tab_val[d.tab_off + j];
+                s_val[j] = tab_val[voff + j];
                 s_code[j] = tab_code[d.tab_off + j];
             }
             cur_pi = pi;
@@ -157,7 +182,8 @@ __global__ __launch_bounds__(kWavesPerBlock * 64) void scan_tiles(
         const u32 seg_len =
             seg_start > d.nh ? 0u : (u32)min((u64)seg_positions, (u64)d.nh - seg_start + 1ull);
         u32 codes = 0;
-        wave_scan(d, (u32)seg_start, seg_len, s_val, s_code, s_run[wave], codes, err);
+        wave_scan<D>(d, consts_of(d, D{}), (u32)seg_start, seg_len, s_val, s_code, s_run[wave],
+                     codes, err);
         if ((threadIdx.x & 63u) == 0) {
             s_codes[wave] = codes;
             s_first[wave] = (u32)seg_start;
@@ -263,11 +289,12 @@ __global__ __launch_bounds__(kMergeThreads) void merge_primes(const PrimeDesc *_
         if (t == 0) {
             const Agg S = s_agg[0];
             if (S.len != d.nh || S.first_x != 1u) atomicOr(err, 2u);
-            const bool sigma0 = (((d.p - 1u) / 2u) % r) == 0u;
+            const u64 p = (u64)d.p | ((u64)(d.hi & 1u) << 32);
+            const bool sigma0 = (((p - 1u) / 2u) % r) == 0u;
             u32 M = S.best, W = S.bstart;
             if (sigma0) take(2u * S.suffix, d.nh - S.suffix + 1u, M, W);  // central junction
             u32 L = S.prefix;
-            if (sigma0 && S.prefix == d.nh) L = d.p - 1u;
+            if (sigma0 && S.prefix == d.nh) L = (u32)(p - 1u);
             const u64 bound = sigma0 ? 2ull * L + 2ull : (u64)L + 2ull;
             rec->M[k] = M;
             rec->L[k] = L;
@@ -299,15 +326,24 @@ extern "C" hipError_t rbg_fill_u32(uint32_t *p, uint32_t n, uint32_t v, hipStrea
     return hipGetLastError();
 }
 
+// wide = 0: tiles of primes below 2^32 (tab_val: u32 table values);
+// wide = 1: tiles of primes between 2^32 and 2^33 (u64 table values).
 extern "C" hipError_t rbg_launch_scan(const rbg::PrimeDesc *descs, uint32_t nprimes,
                                       uint32_t tile_first, uint32_t tile_end, uint32_t chunks,
-                                      const uint32_t *tab_val, const uint16_t *tab_code,
+                                      const void *tab_val, const uint16_t *tab_code,
                                       rbg::TileHdr *hdr, rbg::RunSum *runs, uint32_t *counter,
-                                      uint32_t *err, int persistent, uint32_t grid,
+                                      uint32_t *err, int persistent, int wide, uint32_t grid,
                                       hipStream_t stream) {
-    hipLaunchKernelGGL(rbg::scan_tiles, dim3(grid), dim3(rbg::kWavesPerBlock * 64), 0, stream,
-                       descs, nprimes, tile_first, tile_end, chunks, tab_val, tab_code, hdr, runs,
-                       counter, err, persistent);
+    if (wide)
+        hipLaunchKernelGGL(rbg::scan_tiles<rbg::Dom33>, dim3(grid), dim3(rbg::kWavesPerBlock * 64),
+                           0, stream, descs, nprimes, tile_first, tile_end, chunks,
+                           (const uint64_t *)tab_val, tab_code, hdr, runs, counter, err,
+                           persistent);
+    else
+        hipLaunchKernelGGL(rbg::scan_tiles<rbg::Dom32>, dim3(grid), dim3(rbg::kWavesPerBlock * 64),
+                           0, stream, descs, nprimes, tile_first, tile_end, chunks,
+                           (const uint32_t *)tab_val, tab_code, hdr, runs, counter, err,
+                           persistent);
     return hipGetLastError();
 }
 

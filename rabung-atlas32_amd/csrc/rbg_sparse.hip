@@ -49,27 +49,42 @@ __device__ __forceinline__ u32 find_prime(const SparsePrime *P, u32 n, u32 blk, 
 
 constexpr u32 kLabelsPerThread = 8;
 
-// labels[lab_off + y] = j(y) for y in [1, B]; slot 0 holds 0.
-__global__ __launch_bounds__(256) void sp_labels(const SparsePrime *__restrict__ P, u32 nprimes,
-                                                 const u32 *__restrict__ tab_val,
-                                                 const u16 *__restrict__ tab_lab,
-                                                 u16 *__restrict__ labels, u32 *__restrict__ err) {
-    __shared__ u32 s_val[kMaxM];
-    __shared__ u16 s_lab[kMaxM];
-    const u32 pi = __builtin_amdgcn_readfirstlane(find_prime(P, nprimes, blockIdx.x, 0));
-    const SparsePrime d = P[pi];
+// Residue domain and constants of a sparse prime (the 33rd bits of the wide
+// primes are kept in s.hi).
+template <class D>
+struct SpConsts {
+    D dom;
+    typename D::V one, r2;
+};
+
+__device__ __forceinline__ SpConsts<Dom32> sp_consts(const SparsePrime &s, Dom32) {
+    return {{s.p, s.ninv}, s.one, s.r2};
+}
+
+__device__ __forceinline__ SpConsts<Dom33> sp_consts(const SparsePrime &s, Dom33) {
+    const auto w = [&](u32 lo, int bit) { return (u64)lo | ((u64)((s.hi >> bit) & 1u) << 32); };
+    return {{w(s.p, 0), s.ninv}, w(s.one, 1), w(s.r2, 2)};
+}
+
+template <class D>
+__device__ __forceinline__ void labels_of(const SparsePrime &d, const typename D::V *tab_val,
+                                          const u16 *tab_lab, u16 *labels, u32 *err,
+                                          typename D::V *s_val, u16 *s_lab) {
+    typedef typename D::V V;
+    const SpConsts<D> k = sp_consts(d, D{});
+    const u32 voff = D::wide ? d.tab64 : d.tab_off;
     for (u32 j = threadIdx.x; j < d.m; j += blockDim.x) {
-        s_val[j] = tab_val[d.tab_off + j];
+        s_val[j] = tab_val[voff + j];
         s_lab[j] = tab_lab[d.tab_off + j];
     }
     __syncthreads();
     const u32 y0 = 1u + (blockIdx.x - d.lblk) * (256u * kLabelsPerThread);
     if (blockIdx.x == d.lblk && threadIdx.x == 0) labels[d.lab_off] = 0;
     u32 bad = 0;
-    for (u32 k = 0; k < kLabelsPerThread; ++k) {
-        const u32 y = y0 + k * 256u + threadIdx.x;
+    for (u32 i = 0; i < kLabelsPerThread; ++i) {
+        const u32 y = y0 + i * 256u + threadIdx.x;
         if (y > d.B) break;
-        const u32 v = mont_pow(mont_mul(y, d.r2, d.p, d.ninv), d.e, d.p, d.ninv, d.one);
+        const V v = k.dom.pow(k.dom.mul((V)y, k.r2), d.e, k.one);
         const u32 idx = table_index(s_val, d.m, v);
         bad |= s_val[idx] != v;
         labels[(u64)d.lab_off + y] = s_lab[idx];
@@ -77,16 +92,39 @@ __global__ __launch_bounds__(256) void sp_labels(const SparsePrime *__restrict__
     if (bad) atomicOr(err, 1u);
 }
 
-// inv[inv_off + d] = Montgomery form d^{-1} R mod p for d in [1, D] (d^(p-2)).
+// labels[lab_off + y] = j(y) for y in [1, B]; slot 0 holds 0.  tab_val holds
+// the root tables of the primes below 2^32, tab_val64 those of the wider
+// primes.
+__global__ __launch_bounds__(256) void sp_labels(const SparsePrime *__restrict__ P, u32 nprimes,
+                                                 const u32 *__restrict__ tab_val,
+                                                 const u64 *__restrict__ tab_val64,
+                                                 const u16 *__restrict__ tab_lab,
+                                                 u16 *__restrict__ labels, u32 *__restrict__ err) {
+    __shared__ u64 s_val[kMaxM];
+    __shared__ u16 s_lab[kMaxM];
+    const u32 pi = __builtin_amdgcn_readfirstlane(find_prime(P, nprimes, blockIdx.x, 0));
+    const SparsePrime d = P[pi];
+    if (d.hi & 1u)
+        labels_of<Dom33>(d, tab_val64, tab_lab, labels, err, s_val, s_lab);
+    else
+        labels_of<Dom32>(d, tab_val, tab_lab, labels, err, (u32 *)s_val, s_lab);
+}
+
+// inv[inv_off + d] = Montgomery form d^{-1} R mod p for d in [1, D] (d^(p-2));
+// the wider primes write inv64.
 __global__ __launch_bounds__(256) void sp_inverses(const SparsePrime *__restrict__ P, u32 nprimes,
-                                                   u32 *__restrict__ inv) {
+                                                   u32 *__restrict__ inv, u64 *__restrict__ inv64) {
     const u32 pi = __builtin_amdgcn_readfirstlane(find_prime(P, nprimes, blockIdx.x, 1));
     const SparsePrime s = P[pi];
     const u32 d = (blockIdx.x - s.iblk) * 256u + threadIdx.x;
     if (d == 0 || d > s.D) return;
-    const u32 dm = mont_mul(d, s.r2, s.p, s.ninv);
-    const u32 im = mont_pow(dm, s.p - 2, s.p, s.ninv, s.one);
-    inv[(u64)s.inv_off + d] = im;
+    if (s.hi & 1u) {
+        const SpConsts<Dom33> k = sp_consts(s, Dom33{});
+        inv64[(u64)s.inv_off + d] = k.dom.pow(k.dom.mul(d, k.r2), k.dom.p - 2u, k.one);
+    } else {
+        const u32 dm = mont_mul(d, s.r2, s.p, s.ninv);
+        inv[(u64)s.inv_off + d] = mont_pow(dm, s.p - 2, s.p, s.ninv, s.one);
+    }
 }
 
 constexpr u32 kPlanePairsPerWave = 16;
@@ -207,16 +245,11 @@ __device__ __forceinline__ u32 colour_at(const u32 *pl, u32 wpp, u32 np, int64_t
 }
 
 // Absolute colour of position z by a direct power (past the table).
-__device__ u32 colour_direct(const SparsePrime &s, u32 r, const u32 *tab_val,
-                             const u16 *tab_lab, u32 z) {
-    const u32 v = mont_pow(mont_mul(z, s.r2, s.p, s.ninv), s.e, s.p, s.ninv, s.one);
-    u32 lo = 0, n = s.m;
-    const u32 *vals = tab_val + s.tab_off;
-    while (n > 1) {
-        const u32 half = n >> 1;
-        lo = (vals[lo + half] <= v) ? lo + half : lo;
-        n -= half;
-    }
+template <class D>
+__device__ u32 colour_direct(const SparsePrime &s, const SpConsts<D> &k, u32 r,
+                             const typename D::V *tab_val, const u16 *tab_lab, typename D::V z) {
+    const typename D::V v = k.dom.pow(k.dom.mul(z, k.r2), s.e, k.one);
+    const u32 lo = table_index(tab_val + (D::wide ? s.tab64 : s.tab_off), s.m, v);
     return tab_lab[s.tab_off + lo] % r;
 }
 
@@ -225,21 +258,25 @@ constexpr u32 kExtendLimit = 1u << 22;
 // Extend the monochromatic window [x, x+t-1] (numerators -b + i d) to its run.
 // Rare (a few per task), so kept out of line: the filter loop then holds only
 // the two record pointers, not the records.
+template <class D>
 __device__ void complete_anchor(
-    const SparsePrime *sg, const SparseProf *fg, const u32 *pl, u32 b, u32 d, u32 dinv, u32 t,
-    const u32 *tab_val, const u16 *tab_lab, unsigned long long *best_slot, u32 *failed) {
+    const SparsePrime *sg, const SparseProf *fg, const u32 *pl, u32 b, u32 d, typename D::V dinv,
+    u32 t, const typename D::V *tab_val, const u16 *tab_lab, unsigned long long *best_slot,
+    u32 *failed) {
+    typedef typename D::V V;
     const SparsePrime s = *sg;
     const SparseProf f = *fg;
-    const u32 p = s.p;
+    const SpConsts<D> k = sp_consts(s, D{});
+    const V p = k.dom.p;
     // dinv is the Montgomery form d^{-1} R, so REDC(b * dinv) = b / d mod p
-    const u32 bd = mont_mul(b, dinv, p, s.ninv);
-    const u32 x = p - bd;  // bd != 0 because 0 < b <= B < p
+    const V bd = k.dom.mul((V)b, dinv);
+    const V x = p - bd;  // bd != 0 because 0 < b <= B < p
     const int64_t B = s.B;
     const u32 c0 = colour_at(pl, f.wpp, f.np, -(int64_t)b, s.B);
     u32 abs0 = 0xFFFFFFFFu;
     u32 steps = 0;
     // left
-    u32 lo = x;
+    V lo = x;
     int64_t a = -(int64_t)b - d;
     bool table = true;
     while (lo > 1) {
@@ -248,15 +285,15 @@ __device__ void complete_anchor(
             same = colour_at(pl, f.wpp, f.np, a, s.B) == c0;
         } else {
             table = false;
-            if (abs0 == 0xFFFFFFFFu) abs0 = colour_direct(s, f.r, tab_val, tab_lab, x);
-            same = colour_direct(s, f.r, tab_val, tab_lab, lo - 1) == abs0;
+            if (abs0 == 0xFFFFFFFFu) abs0 = colour_direct<D>(s, k, f.r, tab_val, tab_lab, x);
+            same = colour_direct<D>(s, k, f.r, tab_val, tab_lab, lo - 1u) == abs0;
         }
         if (!same || ++steps > kExtendLimit) break;
         --lo;
         a -= d;
     }
     // right
-    u32 hi = x + t - 1u;
+    V hi = x + t - 1u;
     a = -(int64_t)b + (int64_t)t * d;
     table = true;
     while (hi < p - 1u) {
@@ -265,8 +302,8 @@ __device__ void complete_anchor(
             same = colour_at(pl, f.wpp, f.np, a, s.B) == c0;
         } else {
             table = false;
-            if (abs0 == 0xFFFFFFFFu) abs0 = colour_direct(s, f.r, tab_val, tab_lab, x);
-            same = colour_direct(s, f.r, tab_val, tab_lab, hi + 1u) == abs0;
+            if (abs0 == 0xFFFFFFFFu) abs0 = colour_direct<D>(s, k, f.r, tab_val, tab_lab, x);
+            same = colour_direct<D>(s, k, f.r, tab_val, tab_lab, hi + 1u) == abs0;
         }
         if (!same || ++steps > kExtendLimit) break;
         ++hi;
@@ -276,8 +313,9 @@ __device__ void complete_anchor(
         atomicOr(&failed[f.prime], 2u);
         return;
     }
-    const u32 len = hi - lo + 1u;
-    const u32 w = lo < p - hi ? lo : p - hi;
+    // the run lies in [1, p-1] and its reflected start is at most (p-1)/2
+    const u32 len = (u32)(hi - lo + 1u);
+    const u32 w = (u32)(lo < p - hi ? lo : p - hi);
     atomicMax(best_slot, ((unsigned long long)len << 32) | (0xFFFFFFFFu - w));
 }
 
@@ -382,15 +420,17 @@ __device__ __forceinline__ u32 lanes_below(u64 m) {
 }
 
 // Per-task context (wave-uniform).
+template <class D>
 struct FilterCtx {
     const SparsePrime *s;
     const SparseProf *f;
     const u32 *pl;
-    const u32 *tab_val;
+    const typename D::V *tab_val;
     const u16 *tab_lab;
     unsigned long long *best;
     u32 *failed;
-    u32 d, dinv, t, q, B, bmin, wpp;
+    typename D::V dinv;
+    u32 d, t, q, B, bmin, wpp;
 };
 
 // Words of one band kind: anchors b = (n0 + band * step) d + c for c in
@@ -437,8 +477,8 @@ __device__ __forceinline__ BandKind band_kind(u32 kind, u32 d, u32 t, u32 sp, u3
 }
 
 // Screen and complete the queued words (one lane per word).
-template <int NP>
-__device__ __forceinline__ void drain_queue(const FilterCtx &c, const u32 *const *plb, u32 qn, const u32 *qb,
+template <class D, int NP>
+__device__ __forceinline__ void drain_queue(const FilterCtx<D> &c, const u32 *const *plb, u32 qn, const u32 *qb,
                             const u64 *qs, u32 lane, u32 &n_windows, u32 &n_done) {
     if (lane >= qn) return;
     const u32 b0 = qb[lane];
@@ -452,7 +492,7 @@ __device__ __forceinline__ void drain_queue(const FilterCtx &c, const u32 *const
     while (surv) {
         const u32 bit = (u32)__builtin_ctzll(surv);
         surv &= surv - 1ull;
-        complete_anchor(c.s, c.f, c.pl, b0 + bit, c.d, c.dinv, c.t, c.tab_val, c.tab_lab,
+        complete_anchor<D>(c.s, c.f, c.pl, b0 + bit, c.d, c.dinv, c.t, c.tab_val, c.tab_lab,
                         c.best, c.failed);
         ++n_done;
     }
@@ -485,8 +525,8 @@ __device__ __forceinline__ void wave_lds_fence() {
 // stage t-1, which keeps every address inside the table.
 constexpr u32 kRefill = 48;  // smallest cohort of idle lanes that takes new words
 
-template <int NP>
-__device__ void filter_task(const FilterCtx &c, u32 sp, u32 phi, u32 *qb, u64 *qs, u32 *kinds,
+template <class D, int NP>
+__device__ void filter_task(const FilterCtx<D> &c, u32 sp, u32 phi, u32 *qb, u64 *qs, u32 *kinds,
                             u32 &n_windows, u32 &n_words, u32 &n_done) {
     constexpr int G = NP == 1 ? 8 : NP == 2 ? 4 : 2;  // stages per step (register budget)
     const u32 lane = threadIdx.x & 63u;
@@ -642,7 +682,7 @@ __device__ void filter_task(const FilterCtx &c, u32 sp, u32 phi, u32 *qb, u64 *q
         // no word in flight: screen and complete the queue, then requeue the
         // lanes that were held back
         wave_lds_fence();
-        if (qn) drain_queue<NP>(c, plb, qn, qb, qs, lane, n_windows, n_done);
+        if (qn) drain_queue<D, NP>(c, plb, qn, qb, qs, lane, n_windows, n_done);
         wave_lds_fence();
         const u64 pm = ballot64(pend);
         qn = 0;
@@ -675,11 +715,14 @@ __device__ __forceinline__ u32 next_task(u32 &k, u32 &chunk, u32 claim) {
     return chunk + k;
 }
 
+// D = Dom32 for the profiles of primes below 2^32, Dom33 for the wider
+// primes (one launch per domain, each over its own range of profiles).
+template <class D>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void sp_filter(const SparsePrime *__restrict__ P,
                                                  const SparseProf *__restrict__ prof, u32 nprof,
                                                  u32 dmax, const u32 *__restrict__ planes,
-                                                 const u32 *__restrict__ inv,
-                                                 const u32 *__restrict__ tab_val,
+                                                 const typename D::V *__restrict__ inv,
+                                                 const typename D::V *__restrict__ tab_val,
                                                  const u16 *__restrict__ tab_lab,
                                                  unsigned long long *__restrict__ best,
                                                  u32 *__restrict__ failed,
@@ -735,7 +778,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void s
         if (t > q) t = q;
         const u32 sp = q - t + 1u;  // schedule period
         const u32 phi = ((t - 1u) / 2u) % sp;
-        FilterCtx c;
+        FilterCtx<D> c;
         c.s = sg;
         c.f = fg;
         c.pl = planes + fg->plane_off;
@@ -752,10 +795,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void s
         c.bmin = ((t - 1u) * dd + 1u) / 2u;
         u32 tw = 0, tn = 0, td = 0;  // this task's counts (u32 per lane)
         switch (fg->np) {
-        case 1: filter_task<1>(c, sp, phi, qb, qs, kinds, tw, tn, td); break;
-        case 2: filter_task<2>(c, sp, phi, qb, qs, kinds, tw, tn, td); break;
-        case 3: filter_task<3>(c, sp, phi, qb, qs, kinds, tw, tn, td); break;
-        default: filter_task<4>(c, sp, phi, qb, qs, kinds, tw, tn, td); break;
+        case 1: filter_task<D, 1>(c, sp, phi, qb, qs, kinds, tw, tn, td); break;
+        case 2: filter_task<D, 2>(c, sp, phi, qb, qs, kinds, tw, tn, td); break;
+        case 3: filter_task<D, 3>(c, sp, phi, qb, qs, kinds, tw, tn, td); break;
+        default: filter_task<D, 4>(c, sp, phi, qb, qs, kinds, tw, tn, td); break;
         }
         n_windows += tw;
         n_words += tn;
@@ -810,13 +853,13 @@ extern "C" hipError_t rbg_sparse_prepare(const SparsePrime *P, uint32_t nprimes,
                                          const SparseProf *prof, uint32_t nprof,
                                          uint32_t label_blocks, uint32_t inverse_blocks,
                                          uint32_t plane_blocks, const uint32_t *tab_val,
-                                         const uint16_t *tab_lab, uint16_t *labels,
-                                         uint32_t *inv, uint32_t *planes, uint32_t *err,
-                                         hipStream_t stream) {
+                                         const uint64_t *tab_val64, const uint16_t *tab_lab,
+                                         uint16_t *labels, uint32_t *inv, uint64_t *inv64,
+                                         uint32_t *planes, uint32_t *err, hipStream_t stream) {
     hipLaunchKernelGGL(rbg::sp_labels, dim3(label_blocks), dim3(256), 0, stream, P, nprimes,
-                       tab_val, tab_lab, labels, err);
+                       tab_val, tab_val64, tab_lab, labels, err);
     hipLaunchKernelGGL(rbg::sp_inverses, dim3(inverse_blocks), dim3(256), 0, stream, P, nprimes,
-                       inv);
+                       inv, inv64);
     hipLaunchKernelGGL(rbg::sp_planes, dim3(plane_blocks), dim3(256), 0, stream, P, nprimes, prof,
                        nprof, labels, planes);
     return hipGetLastError();
@@ -840,15 +883,24 @@ extern "C" hipError_t rbg_sparse_seed(const SparsePrime *P, SparseProf *prof, ui
     return hipGetLastError();
 }
 
+// prof / best: the first profile of the launch and its key; wide = 0 for
+// profiles of primes below 2^32 (inv and tab_val hold u32 values), 1 for the
+// wider primes (u64 values).
 extern "C" hipError_t rbg_sparse_filter(const SparsePrime *P, const SparseProf *prof,
                                         uint32_t nprof, uint32_t dmax, const uint32_t *planes,
-                                        const uint32_t *inv, const uint32_t *tab_val,
+                                        const void *inv, const void *tab_val,
                                         const uint16_t *tab_lab, unsigned long long *best,
                                         uint32_t *failed, unsigned long long *stats,
-                                        uint32_t *queue, uint32_t grid, uint32_t group,
+                                        uint32_t *queue, int wide, uint32_t grid, uint32_t group,
                                         hipStream_t stream) {
-    hipLaunchKernelGGL(rbg::sp_filter, dim3(grid), dim3(256), 0, stream, P, prof, nprof, dmax,
-                       planes, inv, tab_val, tab_lab, best, failed, stats, queue, group);
+    if (wide)
+        hipLaunchKernelGGL(rbg::sp_filter<rbg::Dom33>, dim3(grid), dim3(256), 0, stream, P, prof,
+                           nprof, dmax, planes, (const uint64_t *)inv, (const uint64_t *)tab_val,
+                           tab_lab, best, failed, stats, queue, group);
+    else
+        hipLaunchKernelGGL(rbg::sp_filter<rbg::Dom32>, dim3(grid), dim3(256), 0, stream, P, prof,
+                           nprof, dmax, planes, (const uint32_t *)inv, (const uint32_t *)tab_val,
+                           tab_lab, best, failed, stats, queue, group);
     return hipGetLastError();
 }
 

@@ -11,6 +11,11 @@ struct SparsePrime {
     u32 B, D, Q, jneg;       // certified cover; label of -1 (0 or m/2)
     u32 tab_off, lab_off, inv_off, prof_begin;
     u32 nprof, lblk, iblk, pblk;  // profiles; first label / inverse / plane block
+    // 2^32 < p < 2^33: bit 32 of p, one, r2 (bits 0-2; the fields above hold
+    // the low words), the offset of the root-table values in the u64 value
+    // array (labels stay at tab_off), and inv_off counts in the u64 inverse
+    // array
+    u32 hi, tab64;
 };
 
 // Per (prime, admissible order).

@@ -45,7 +45,7 @@ RBG_HD u32 colour_code(u32 j, u32 m) {
     return c;
 }
 
-// Per-prime constants, one 64-byte record per prime of the block.
+// Per-prime constants, one record per prime of the block.
 struct PrimeDesc {
     u32 p;          // the prime
     u32 nh;         // (p-1)/2, length of the half interval scanned
@@ -63,6 +63,10 @@ struct PrimeDesc {
     u32 sum_off;    // first RunSum slot of this prime
     u32 nr;         // number of admissible orders
     u32 index;      // position in the caller's block
+    u32 hi;         // 2^32 < p < 2^33: bit 32 of p, one, r2, step (bits 0-3);
+                    // p, one, r2 and step above hold the low words
+    u32 tab64;      // 2^32 < p: offset of the root-table values in the u64
+                    // value array (codes stay at tab_off)
 };
 
 // Per (tile, admissible order) run summary of a contiguous stretch of the

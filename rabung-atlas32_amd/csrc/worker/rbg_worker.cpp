@@ -2,9 +2,12 @@
 // process protocols so external drivers see the same worker.
 //
 //   rbg_worker profile [--method batched|plain|dense] [--max-batch N]
-//                      [--no-certify] [--device D]
+//                      [--no-certify] [--device D] [--domain-bits 32|33]
 //     stdin : "<id> p1 ... pk" lines (ids consecutive from 0, 1 <= k <= N,
-//             default N = 128, primes < 2^32), then "QUIT";
+//             default N = 128, primes < 2^32; with --domain-bits 33 primes
+//             1000 <= p < 2^33, the reference's wider worker,
+//             reproducibility/gpu_method/source/worker33.cu:24-41), then
+//             "QUIT";
 //     stdout: JSON lines {"type":"ready"}, one {"type":"measurement"} per
 //             prime in input order, {"type":"batch_done"} per batch,
 //             {"type":"bye"}.  Reference: src/census/worker32.cu:10-45 and
@@ -65,12 +68,15 @@ struct ProfileOptions {
     uint32_t max_batch = 128;
     bool certify = true;
     int device = 0;
+    int domain_bits = 32;
 };
 
 // Profiles `primes` into `out`, replaying halves on a capacity refusal.
-int profile_replay(rbg_ctx *ctx, const uint32_t *primes, uint32_t n, uint32_t flags,
-                   rbg_record *out, Clock::time_point deadline, std::string &why) {
-    const int rc = rbg_profile_block(ctx, primes, n, flags, out);
+// (The wide entry point takes both domains; primes below 2^32 run the
+// 32-bit kernels.)
+int profile_replay(rbg_ctx *ctx, const uint64_t *primes, uint32_t n, uint32_t flags,
+                   rbg_record_wide *out, Clock::time_point deadline, std::string &why) {
+    const int rc = rbg_profile_block_wide(ctx, primes, n, flags, out);
     if (rc == RBG_OK) return RBG_OK;
     why = rbg_last_error(ctx);
     if (rc != RBG_ECAPACITY || n == 1 || Clock::now() > deadline) return rc;
@@ -86,12 +92,13 @@ int run_profile(const ProfileOptions &o) {
     cfg.device = o.device;
     rbg_ctx *ctx = rbg_open(&cfg);
     if (!ctx) fail(tag, "device context");
-    std::cout << "{\"type\":\"ready\",\"domain_bits\":32,\"engine\":\"" << rbg_version()
-              << "\"}\n"
+    std::cout << "{\"type\":\"ready\",\"domain_bits\":" << o.domain_bits << ",\"engine\":\""
+              << rbg_version() << "\"}\n"
               << std::flush;
+    const bool wide = o.domain_bits == 33;
     std::string line;
     uint64_t expected = 0;
-    std::vector<rbg_record> recs;
+    std::vector<rbg_record_wide> recs;
     while (std::getline(std::cin, line)) {
         if (line == "QUIT") {
             std::cout << "{\"type\":\"bye\"}\n" << std::flush;
@@ -102,16 +109,17 @@ int run_profile(const ProfileOptions &o) {
         std::string tok;
         uint64_t id = 0, v = 0;
         if (!(in >> tok) || !parse_u64(tok, id) || id != expected) fail(tag, "batch sequence");
-        std::vector<uint32_t> primes;
+        std::vector<uint64_t> primes;
         while (in >> tok) {
             if (!parse_u64(tok, v)) fail(tag, "prime token");
-            if (v >= (1ull << 32)) fail(tag, "worker prime range");
-            if (!rbg_is_prime((uint32_t)v)) fail(tag, "worker primality");
-            primes.push_back((uint32_t)v);
+            if (wide ? (v < 1000 || v >= (1ull << 33)) : v >= (1ull << 32))
+                fail(tag, "worker prime range");
+            if (!rbg_is_prime_wide(v)) fail(tag, "worker primality");
+            primes.push_back(v);
         }
         if (primes.empty() || primes.size() > o.max_batch) fail(tag, "worker batch size");
         const auto start = Clock::now();
-        recs.assign(primes.size(), rbg_record{});
+        recs.assign(primes.size(), rbg_record_wide{});
         std::string why;
         const uint32_t flags = o.method | (o.certify ? RBG_CERTIFY : 0u);
         const int rc = profile_replay(ctx, primes.data(), (uint32_t)primes.size(), flags,
@@ -120,7 +128,7 @@ int run_profile(const ProfileOptions &o) {
         const double wall = ms_since(start);
         std::ostringstream outs;
         outs.precision(12);
-        for (const rbg_record &r : recs) {
+        for (const rbg_record_wide &r : recs) {
             outs << "{\"type\":\"measurement\",\"status\":\"PASS\",\"p\":" << r.p
                  << ",\"mask\":" << r.mask << ",\"total_ms\":" << wall / recs.size()
                  << ",\"A4\":" << r.A4 << ",\"profiles\":[";
@@ -225,7 +233,7 @@ int run_a4() {
 int main(int argc, char **argv) {
     if (argc < 2) {
         std::cerr << "usage: rbg_worker profile [--method batched|plain|dense] [--max-batch N] "
-                     "[--no-certify] [--device D] | rbg_worker a4\n";
+                     "[--no-certify] [--device D] [--domain-bits 32|33] | rbg_worker a4\n";
         return 2;
     }
     const std::string mode = argv[1];
@@ -251,6 +259,10 @@ int main(int argc, char **argv) {
             o.certify = false;
         } else if (a == "--device" && i + 1 < argc) {
             o.device = std::atoi(argv[++i]);
+        } else if (a == "--domain-bits" && i + 1 < argc) {
+            o.domain_bits = std::atoi(argv[++i]);
+            if (o.domain_bits != 32 && o.domain_bits != 33)
+                fail("VDW_CAMPAIGN_FAILURE", "domain bits must be 32 or 33");
         } else {
             fail("VDW_CAMPAIGN_FAILURE", "unknown option " + a);
         }

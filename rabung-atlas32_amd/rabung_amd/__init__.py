@@ -38,6 +38,17 @@ class Record(ctypes.Structure):
                 ("A4", ctypes.c_uint32), ("status", ctypes.c_uint32)]
 
 
+class RecordWide(ctypes.Structure):
+    """rbg_record_wide: the record of the wider domain (p < 2^33)."""
+    _fields_ = [("p", ctypes.c_uint64), ("mask", ctypes.c_uint32),
+                ("M", ctypes.c_uint32 * 8), ("L", ctypes.c_uint32 * 8),
+                ("W", ctypes.c_uint32 * 8), ("K", ctypes.c_uint64 * 8),
+                ("A4", ctypes.c_uint32), ("status", ctypes.c_uint32)]
+
+
+WIDE_LIMIT = 1 << 33  # primes up to here take the wide entry points
+
+
 class Timing(ctypes.Structure):
     _fields_ = [("total_ms", ctypes.c_double), ("dense_scan_ms", ctypes.c_double),
                 ("sparse_prep_ms", ctypes.c_double), ("sparse_seed_ms", ctypes.c_double),
@@ -60,6 +71,9 @@ EXPORTS = ["rbg_open", "rbg_profile_block", "rbg_profile_block_device", "rbg_pro
            "rbg_last_timing", "rbg_device_info", "rbg_close", "rbg_mask", "rbg_is_prime",
            "rbg_version", "rbg_a4_host", "rbg_primes_from", "rbg_encode_scan_frame",
            "rbg_encode_a4_frame", "rbg_science_hash", "rbg_prime_hash", "rbg_a4_certificate", "rbg_microbench",
+           "rbg_profile_block_wide", "rbg_profile_batch_wide", "rbg_estimate_bytes_wide",
+           "rbg_a4_host_wide", "rbg_is_prime_wide", "rbg_mask_wide", "rbg_primes_from_wide",
+           "rbg_check_wide",
            "baseline_init",
            "baseline_profile"]
 
@@ -137,6 +151,26 @@ def load():
     lib.rbg_microbench.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_uint32,
                                    P(ctypes.c_double)]
     lib.rbg_microbench.restype = ctypes.c_int
+    for name in ("rbg_profile_block_wide", "rbg_profile_batch_wide"):
+        getattr(lib, name).argtypes = [ctypes.c_void_p, P(ctypes.c_uint64), ctypes.c_uint32,
+                                       ctypes.c_uint32, P(RecordWide)]
+        getattr(lib, name).restype = ctypes.c_int
+    lib.rbg_estimate_bytes_wide.argtypes = [ctypes.c_void_p, P(ctypes.c_uint64), ctypes.c_uint32,
+                                            ctypes.c_uint32, P(ctypes.c_uint64)]
+    lib.rbg_estimate_bytes_wide.restype = ctypes.c_int
+    lib.rbg_a4_host_wide.argtypes = [P(ctypes.c_uint64), ctypes.c_uint32, P(ctypes.c_uint32),
+                                     ctypes.c_uint32]
+    lib.rbg_a4_host_wide.restype = ctypes.c_int
+    lib.rbg_is_prime_wide.argtypes = [ctypes.c_uint64]
+    lib.rbg_is_prime_wide.restype = ctypes.c_int
+    lib.rbg_mask_wide.argtypes = [ctypes.c_uint64]
+    lib.rbg_mask_wide.restype = ctypes.c_uint32
+    lib.rbg_primes_from_wide.argtypes = [ctypes.c_uint64, ctypes.c_uint32, P(ctypes.c_uint64)]
+    lib.rbg_primes_from_wide.restype = ctypes.c_uint32
+    lib.rbg_check_wide.argtypes = [ctypes.c_int, P(ctypes.c_uint64), P(ctypes.c_uint64),
+                                   ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int,
+                                   P(ctypes.c_uint64)]
+    lib.rbg_check_wide.restype = ctypes.c_int
     _lib = lib
     return lib
 
@@ -222,29 +256,31 @@ class Engine:
         except Exception:
             pass
 
-    def profile_block(self, primes, method=METHOD_BATCHED, certify=False, graph=False):
-        """Profiles of up to 4096 primes; returns a list of dicts."""
+    def _profile(self, name, primes, method, certify, graph):
         primes = [int(p) for p in primes]
         n = len(primes)
-        arr = (ctypes.c_uint32 * max(1, n))(*primes)
-        out = (Record * max(1, n))()
         flags = method | (CERTIFY if certify else 0) | (GRAPH if graph else 0)
-        rc = self._lib.rbg_profile_block(self._ctx, arr, n, flags, out)
+        # primes above 2^32 take the wide entry points (rbg_record_wide)
+        wide = any(p >= (1 << 32) for p in primes)
+        if wide:
+            arr = (ctypes.c_uint64 * max(1, n))(*primes)
+            out = (RecordWide * max(1, n))()
+            rc = getattr(self._lib, name + "_wide")(self._ctx, arr, n, flags, out)
+        else:
+            arr = (ctypes.c_uint32 * max(1, n))(*primes)
+            out = (Record * max(1, n))()
+            rc = getattr(self._lib, name)(self._ctx, arr, n, flags, out)
         if rc:
             raise RabungError(rc, self._lib.rbg_last_error(self._ctx).decode())
         return [record_to_dict(out[i]) for i in range(n)]
 
+    def profile_block(self, primes, method=METHOD_BATCHED, certify=False, graph=False):
+        """Profiles of up to 4096 primes below 2^33; returns a list of dicts."""
+        return self._profile("rbg_profile_block", primes, method, certify, graph)
+
     def profile_batch(self, primes, method=METHOD_BATCHED, certify=False, graph=False):
         """Profiles of up to 2^20 primes (many blocks), split by HBM budget."""
-        primes = [int(p) for p in primes]
-        n = len(primes)
-        arr = (ctypes.c_uint32 * max(1, n))(*primes)
-        out = (Record * max(1, n))()
-        flags = method | (CERTIFY if certify else 0) | (GRAPH if graph else 0)
-        rc = self._lib.rbg_profile_batch(self._ctx, arr, n, flags, out)
-        if rc:
-            raise RabungError(rc, self._lib.rbg_last_error(self._ctx).decode())
-        return [record_to_dict(out[i]) for i in range(n)]
+        return self._profile("rbg_profile_batch", primes, method, certify, graph)
 
     def autotune(self, primes, method=METHOD_BATCHED):
         """Pick the fastest launch shape on these sample primes; returns it."""
@@ -259,9 +295,15 @@ class Engine:
 
     def estimate_bytes(self, primes, method=METHOD_BATCHED):
         primes = [int(p) for p in primes]
-        arr = (ctypes.c_uint32 * max(1, len(primes)))(*primes)
         out = ctypes.c_uint64()
-        rc = self._lib.rbg_estimate_bytes(self._ctx, arr, len(primes), method, ctypes.byref(out))
+        if any(p >= (1 << 32) for p in primes):
+            arr = (ctypes.c_uint64 * max(1, len(primes)))(*primes)
+            rc = self._lib.rbg_estimate_bytes_wide(self._ctx, arr, len(primes), method,
+                                                   ctypes.byref(out))
+        else:
+            arr = (ctypes.c_uint32 * max(1, len(primes)))(*primes)
+            rc = self._lib.rbg_estimate_bytes(self._ctx, arr, len(primes), method,
+                                              ctypes.byref(out))
         if rc:
             raise RabungError(rc, "estimate failed")
         return out.value
@@ -290,11 +332,36 @@ RECORD_SIZE = ctypes.sizeof(Record)
 
 
 def mask(p):
-    return load().rbg_mask(p)
+    p = int(p)
+    return load().rbg_mask(p) if p < (1 << 32) else load().rbg_mask_wide(p)
 
 
 def is_prime(p):
-    return bool(load().rbg_is_prime(p))
+    """Deterministic primality below 2^33 (False from 2^33 on)."""
+    p = int(p)
+    if p < 0:
+        return False
+    return bool(load().rbg_is_prime(p) if p < (1 << 32) else load().rbg_is_prime_wide(p))
+
+
+def primes_from_wide(start, count):
+    """Up to `count` consecutive primes >= start below 2^33."""
+    out = (ctypes.c_uint64 * max(1, count))()
+    k = load().rbg_primes_from_wide(start, count, out)
+    return list(out[:k])
+
+
+def check_wide(a, b, p, kind, device=0):
+    """Device results of the wide residue arithmetic (kind 0: Montgomery
+    product a b 2^-32 mod p; kind 1: Montgomery power of a to b)."""
+    n = len(a)
+    A = (ctypes.c_uint64 * n)(*a)
+    Bv = (ctypes.c_uint64 * n)(*b)
+    out = (ctypes.c_uint64 * n)()
+    rc = load().rbg_check_wide(device, A, Bv, n, p, kind, out)
+    if rc:
+        raise RabungError(rc, "rbg_check_wide")
+    return list(out)
 
 
 def primes_from(start, count):
@@ -309,9 +376,13 @@ def a4_host(primes, threads=0):
     lib = load()
     primes = [int(p) for p in primes]
     n = len(primes)
-    arr = (ctypes.c_uint32 * max(1, n))(*primes)
     out = (ctypes.c_uint32 * max(1, n))()
-    rc = lib.rbg_a4_host(arr, n, out, threads)
+    if any(p >= (1 << 32) for p in primes):
+        arr = (ctypes.c_uint64 * max(1, n))(*primes)
+        rc = lib.rbg_a4_host_wide(arr, n, out, threads)
+    else:
+        arr = (ctypes.c_uint32 * max(1, n))(*primes)
+        rc = lib.rbg_a4_host(arr, n, out, threads)
     if rc:
         raise RabungError(rc, "A4 point count failed")
     return list(out[:n])

@@ -15,6 +15,10 @@ Outputs:
   holdout32.json.gz                             profiles of 4096 primes in four
                                                 windows up to 4.237e9
   night32.json.gz                               151 primes 1.16e8 .. 4294967291
+  holdout33.json.gz                             profiles of 4096 primes in four
+                                                windows 5.389e9 .. 8.317e9 (wide
+                                                arithmetic path, p < 2^33)
+  night33.json.gz                               16 primes 4.4e9 .. 8.6e9
 """
 import gzip
 import hashlib
@@ -108,6 +112,34 @@ def main():
         json.dump({"source": "reproducibility/gpu_method/night/MESURES_FINALES_32_BITS.json",
                    "rows": compact_rows(rows)}, f, separators=(",", ":"))
     print("holdout", sum(len(w["rows"]) for w in windows), "night", len(rows))
+
+    # wide path: the final 33-bit cohort (the paper's throughput windows
+    # 6.023e9 and 8.041e9) and the earlier 33-bit windows 5.389e9, 8.317e9
+    windows33 = []
+    for name in ("holdout-33-selected_selected_binary48.json", "holdout-33-mr_selected.json"):
+        hold = json.loads((gm / "throughput" / name).read_text())
+        assert hold["bits"] == 33
+        for w in hold["windows"]:
+            rows = w["profiles_full"]
+            for row in rows:
+                assert (1 << 32) < row["p"] < (1 << 33)
+                for v in row["profiles"]:
+                    assert v["K"] == codec.threshold_k(row["p"], v["r"], v["max"], v["prefix"])
+            windows33.append({"lo": w["lo"], "hi": w["hi"], "source": name,
+                              "rows": compact_rows(rows)})
+    windows33.sort(key=lambda w: w["lo"])
+    with gzip.open(HERE / "holdout33.json.gz", "wt") as f:
+        json.dump({"source": "reproducibility/gpu_method/throughput/holdout-33-*.json",
+                   "windows": windows33}, f, separators=(",", ":"))
+    night = json.loads((gm / "night/MESURES_FINALES_33_BITS.json").read_text())
+    rows = sorted(night["per_prime"], key=lambda r: r["p"])
+    for row in rows:
+        for v in row["profiles"]:
+            assert v["K"] == codec.threshold_k(row["p"], v["r"], v["max"], v["prefix"])
+    with gzip.open(HERE / "night33.json.gz", "wt") as f:
+        json.dump({"source": "reproducibility/gpu_method/night/MESURES_FINALES_33_BITS.json",
+                   "rows": compact_rows(rows)}, f, separators=(",", ":"))
+    print("holdout33", sum(len(w["rows"]) for w in windows33), "night33", len(rows))
 
 
 if __name__ == "__main__":

@@ -35,3 +35,17 @@ def night_rows():
     with gzip.open(GOLDEN / "night32.json.gz", "rt") as f:
         data = json.load(f)
     return _expand(data["rows"])
+
+
+def holdout33_windows():
+    """Four windows of 1024 consecutive primes between 2^32 and 2^33."""
+    with gzip.open(GOLDEN / "holdout33.json.gz", "rt") as f:
+        data = json.load(f)
+    return [{"lo": w["lo"], "hi": w["hi"], "source": w["source"], "rows": _expand(w["rows"])}
+            for w in data["windows"]]
+
+
+def night33_rows():
+    with gzip.open(GOLDEN / "night33.json.gz", "rt") as f:
+        data = json.load(f)
+    return _expand(data["rows"])
