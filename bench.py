@@ -91,9 +91,11 @@ def bench_batch64(args):
 
 # The reference's published batch throughput (RTX 5090): 1024 consecutive
 # primes per window, all admissible r, eight 128-prime batches through its
-# worker protocol, sum of batch wall times (paper/GPU_METHOD_EN.tex:845-848).
+# worker protocol, sum of batch wall times (paper/GPU_METHOD_EN.tex:839-857:
+# four windows on the 32-bit path, two on the 33-bit path).
 REFERENCE_WINDOWS = {257000003: 0.1461, 1337000003: 0.4461, 2971000001: 0.8447,
                      4237000001: 1.1118}
+REFERENCE_WINDOWS_33 = {6023000001: 1.1628, 8041000003: 1.4750}
 
 
 def bench_holdout(args):
@@ -106,23 +108,28 @@ def bench_holdout(args):
     sys.path.insert(0, str(ROOT / "tests"))
     import golden_data
     worker = ROOT / "rabung-atlas32_amd" / "rbg_worker"
-    windows = []
+    windows = []  # (window, reference seconds, worker domain bits)
     for w in golden_data.holdout_windows():
         if w["lo"] in REFERENCE_WINDOWS:
-            windows.append(w)
-    proc = subprocess.Popen([str(worker), "profile"], stdin=subprocess.PIPE,
-                            stdout=subprocess.PIPE, text=True, bufsize=1)
-    ready = _json.loads(proc.stdout.readline())
-    assert ready["type"] == "ready"
-    batch_id = 0
+            windows.append((w, REFERENCE_WINDOWS[w["lo"]], 32))
+    for w in golden_data.holdout33_windows():
+        if w["lo"] in REFERENCE_WINDOWS_33:
+            windows.append((w, REFERENCE_WINDOWS_33[w["lo"]], 33))
+    procs, batch_ids = {}, {32: 0, 33: 0}
+    for bits in (32, 33):
+        procs[bits] = subprocess.Popen([str(worker), "profile", "--domain-bits", str(bits)],
+                                       stdin=subprocess.PIPE, stdout=subprocess.PIPE, text=True,
+                                       bufsize=1)
+        ready = _json.loads(procs[bits].stdout.readline())
+        assert ready["type"] == "ready" and ready["domain_bits"] == bits
 
-    def run_window(rows):
-        nonlocal batch_id
+    def run_window(rows, bits):
+        proc = procs[bits]
         primes = [r["p"] for r in rows]
         got, total = [], 0.0
         for i in range(0, len(primes), 128):
             t0 = time.perf_counter()
-            proc.stdin.write("%d %s\n" % (batch_id, " ".join(map(str, primes[i:i + 128]))))
+            proc.stdin.write("%d %s\n" % (batch_ids[bits], " ".join(map(str, primes[i:i + 128]))))
             proc.stdin.flush()
             while True:
                 item = _json.loads(proc.stdout.readline())
@@ -131,38 +138,90 @@ def bench_holdout(args):
                 elif item["type"] == "batch_done":
                     break
             total += time.perf_counter() - t0
-            batch_id += 1
+            batch_ids[bits] += 1
         if got != rows:
             raise SystemExit("bench: worker results differ from the window's profiles")
         return total
 
     out, ref_total, our_total, profiles = [], 0.0, 0.0, 0
-    for w in windows:
-        run_window(w["rows"])  # warm-up pass
-        reps = [run_window(w["rows"]) for _ in range(max(1, args.steps))]
+    for w, ref, bits in windows:
+        run_window(w["rows"], bits)  # warm-up pass
+        reps = [run_window(w["rows"], bits) for _ in range(max(1, args.steps))]
         mean = sum(reps) / len(reps)
         nprof = sum(len(r["profiles"]) for r in w["rows"])
-        ref = REFERENCE_WINDOWS[w["lo"]]
-        out.append({"window_start": w["lo"], "primes": len(w["rows"]), "profiles": nprof,
-                    "seconds": round(mean, 4), "profiles_per_s": round(nprof / mean, 1),
-                    "reference_seconds": ref, "speedup_vs_reference": round(ref / mean, 2)})
+        out.append({"window_start": w["lo"], "path_bits": bits, "primes": len(w["rows"]),
+                    "profiles": nprof, "seconds": round(mean, 4),
+                    "profiles_per_s": round(nprof / mean, 1), "reference_seconds": ref,
+                    "speedup_vs_reference": round(ref / mean, 2)})
+        print("bench: window %d (%d-bit path) %.4f s, reference %.4f s"
+              % (w["lo"], bits, mean, ref), file=sys.stderr, flush=True)
         ref_total += ref
         our_total += mean
         profiles += nprof
-    proc.stdin.write("QUIT\n")
-    proc.stdin.flush()
-    proc.wait(timeout=60)
+    for proc in procs.values():
+        proc.stdin.write("QUIT\n")
+        proc.stdin.flush()
+        proc.wait(timeout=60)
     line = {"metric": METRIC, "value": round(profiles / our_total, 3), "unit": "profiles/s",
             "n_gpus": 1, "steps": args.steps, "warmup": 1,
             "ms_per_step": round(our_total * 1000.0, 3), "higher_is_better": True,
             "scaling": "weak", "vs_baseline": round(ref_total / our_total, 3), "dtype": "u32",
             "data": "reference throughput windows (committed profiles)",
             "config": {"workload": "reference batch-throughput windows: 1024 consecutive primes "
-                                   "from 0.257e9, 1.337e9, 2.971e9 and 4.237e9, all admissible r, "
+                                   "from 0.257e9, 1.337e9, 2.971e9, 4.237e9 (32-bit path) and "
+                                   "6.023e9, 8.041e9 (33-bit path), all admissible r, "
                                    "8 x 128-prime batches through rbg_worker's line protocol "
                                    "(certified), sum of batch wall times per window",
                        "windows": out,
-                       "baseline": "paper/GPU_METHOD_EN.tex:845-848, RTX 5090, selected engine"}}
+                       "baseline": "paper/GPU_METHOD_EN.tex:839-857, RTX 5090, selected engine"}}
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+def bench_wide(args):
+    """The wider domain (2^32 < p < 2^33): the 256 largest primes below 2^33
+    through rbg_profile_block_wide (host buffers, batched method); records
+    certified by direct powers once, outside the timed steps."""
+    import rabung_amd
+    from rabung_amd import workloads
+    primes = workloads.top_wide(256)
+    profiles = sum(bin(rabung_amd.mask(p)).count("1") for p in primes)
+    eng = rabung_amd.Engine(0)
+    info = eng.device_info()
+    ref = eng.profile_block(primes, certify=True)
+    for _ in range(args.warmup):
+        eng.profile_block(primes)
+    timings = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        if eng.profile_block(primes) != ref:
+            raise SystemExit("bench: wide records differ between steps")
+        timings.append(eng.last_timing())
+    dt = time.perf_counter() - t0
+    peak_tops = info["cus"] * 128 * info["clock_khz"] * 1e3 / 1e12
+    kern_ms = sum(t["sparse_filter_ms"] for t in timings) / len(timings)
+    ops = timings[-1]["filter_windows"] * 8
+    t = timings[-1]
+    line = {"metric": METRIC, "value": round(profiles / (dt / args.steps), 3),
+            "unit": "profiles/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": round(dt * 1000.0 / args.steps, 3), "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "u32/u64",
+            "data": "synthetic (deterministic prime list), certified",
+            "config": {"workload": "top of the wider domain: the 256 largest primes below 2^33 "
+                                   "(%d profiles), all admissible r, full profile + A4, "
+                                   "rbg_profile_block_wide" % profiles,
+                       "primes": len(primes)},
+            "roofline": {"bound": "valu", "kernel": "sp_filter<Dom33>",
+                         "achieved": round(ops / (kern_ms / 1000.0) / 1e12, 3),
+                         "peak": round(peak_tops, 3), "unit": "TOP/s",
+                         "frac": round(ops / (kern_ms / 1000.0) / 1e12 / peak_tops, 4),
+                         "traffic": None, "avg_launch_ms": round(kern_ms, 3),
+                         "algorithmic_ops_per_launch": ops},
+            "stage_ms": {"prep": round(t["sparse_prep_ms"], 3),
+                         "seed": round(t["sparse_seed_ms"], 3),
+                         "filter": round(t["sparse_filter_ms"], 3),
+                         "device_total": round(t["total_ms"], 3)}}
+    eng.close()
     print(json.dumps(line), flush=True)
     return 0
 
@@ -260,8 +319,9 @@ def microbench():
     names = ["add_or_u32", "mul_lo_or_u32", "mul_hi_or_u32", "alignbit_b32", "montgomery_product",
              "add_u32", "mul_lo_u32"]
     rates = {n: round(rabung_amd.microbench(k) / 1e12, 3) for k, n in enumerate(names)}
-    loads = {"%d_dword%s_%s" % (w, "s" if w > 1 else "", "contiguous" if st == 8 else "line_per_lane"):
-             round(rabung_amd.microbench(10 + w, st), 2) for st in (8, 4160) for w in (1, 2, 3, 4)}
+    layout = {4: "lanes_4B_apart", 8: "lanes_8B_apart", 4160: "line_per_lane"}
+    loads = {"%d_dword%s_%s" % (w, "s" if w > 1 else "", layout[st]):
+             round(rabung_amd.microbench(10 + w, st), 2) for st in (4, 8, 4160) for w in (1, 2, 3, 4)}
     print(json.dumps({"valu_tops": rates, "cu_cycles_per_wave_load": loads}), flush=True)
     return 0
 
@@ -285,7 +345,7 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="top256",
-                    choices=sorted(WORKLOADS) + ["batch64", "holdout"])
+                    choices=sorted(WORKLOADS) + ["batch64", "holdout", "wide33"])
     ap.add_argument("--no-plain", action="store_true", help="skip timing the plain method")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--serve-seconds", type=float, default=10.0,
@@ -310,6 +370,8 @@ def main():
         return bench_batch64(args)
     if args.workload == "holdout":
         return bench_holdout(args)
+    if args.workload == "wide33":
+        return bench_wide(args)
     import torch
     import rabung_amd
     from rabung_amd import Record, RECORD_SIZE

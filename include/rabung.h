@@ -201,8 +201,10 @@ int rbg_a4_certificate(uint32_t p, uint32_t *a4, uint64_t *attempts, char *out, 
  * mul_hi(x, y|0x80000001) (each with one v_or_b32), v_alignbit_b32, whole
  * Montgomery products, plain add and plain v_mul_lo_u32 (8 independent
  * chains per thread, 8 waves per SIMD); kind 11-14: CU cycles per wave-wide
- * load of 1-4 dwords per lane with lanes `param` = 8 (contiguous) or 4160
- * (one line each) bytes apart, from an L2-resident buffer. */
+ * load of 1-4 dwords per lane with lanes `param` = 4 (consecutive dwords),
+ * 8 (one contiguous span) or 4160 (one line each) bytes apart, from an
+ * L2-resident buffer, 8 independent loads in flight per wave (issue cost,
+ * not latency). */
 int rbg_microbench(int device, int kind, uint32_t param, double *out);
 
 /* Test hook for the wide residue arithmetic: on `device`, kind 0 writes

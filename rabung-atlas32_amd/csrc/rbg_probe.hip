@@ -5,9 +5,11 @@
 //     products (the dense scan's unit of work).  Each thread runs 8
 //     independent dependency chains so that latency is hidden and the rate is
 //     the issue rate.
-//   * The cost of one wave-wide vector load: 1 to 4 dwords per lane, lanes
-//     8 bytes apart (one contiguous 512-byte span) or 4160 bytes apart (one
-//     line per lane), from a buffer that stays in L2.
+//   * The throughput cost of one wave-wide vector load: 1 to 4 dwords per
+//     lane, lanes 4 bytes apart (consecutive dwords), 8 bytes apart (one
+//     contiguous 512-byte span) or 4160 bytes apart (one line per lane),
+//     from a buffer that stays in L2; each wave keeps 8 independent loads in
+//     flight, so the figure is the issue cost, not the latency.
 //
 // No reference counterpart: the reference quotes its dense rate in source
 // comments only (original_scan_gpu.cu:12-14).
@@ -43,19 +45,30 @@ __global__ __launch_bounds__(256) void valu_rate(u32 seed, u32 iters, u32 p, u32
     if (x == 0x9E3779B9u) out[0] = x;  // keeps the chains alive
 }
 
+constexpr u32 kLoadsInFlight = 8;
+
+// iters must be a multiple of kLoadsInFlight.
 template <int NW>
 __global__ __launch_bounds__(256) void load_rate(const u32 *__restrict__ buf, u32 mask, u32 stride,
                                                  u32 iters, u32 *out) {
     const u32 lane = threadIdx.x & 63u;
     const u32 wave = blockIdx.x * 4u + (threadIdx.x >> 6);
     u32 acc = 0, off = (wave * 1024u) & mask;
-    for (u32 it = 0; it < iters; ++it) {
-        const u32 a = ((off + lane * stride) & mask) >> 2;  // the buffer has 4 spare dwords
-        if (NW == 1) acc ^= buf[a];
-        if (NW == 2) acc ^= buf[a] ^ buf[a + 1];
-        if (NW == 3) acc ^= buf[a] ^ buf[a + 1] ^ buf[a + 2];
-        if (NW == 4) acc ^= buf[a] ^ buf[a + 1] ^ buf[a + 2] ^ buf[a + 3];
-        off = (off + 4160u) & mask;
+    for (u32 it = 0; it < iters; it += kLoadsInFlight) {
+        u32 v[kLoadsInFlight][NW];
+#pragma unroll
+        for (u32 u = 0; u < kLoadsInFlight; ++u) {
+            // the buffer has 4 spare dwords past mask
+            const u32 a = ((off + lane * stride + u * 12480u) & mask) >> 2;
+#pragma unroll
+            for (int w = 0; w < NW; ++w) v[u][w] = buf[a + w];
+        }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (u32 u = 0; u < kLoadsInFlight; ++u)
+#pragma unroll
+            for (int w = 0; w < NW; ++w) acc ^= v[u][w];
+        off = (off + kLoadsInFlight * 12480u + 4160u) & mask;
     }
     if (acc == 0x9E3779B9u) out[0] = acc;
 }
@@ -136,7 +149,7 @@ extern "C" int rbg_microbench(int device, int kind, uint32_t param, double *out)
         }
         if (ms <= 0) rc = RBG_EDEVICE;
         else *out = (double)blocks * 256.0 * iters * rbg::kChains / (ms * 1e-3);
-    } else if (kind >= 11 && kind <= 14 && (param == 8u || param == 4160u)) {
+    } else if (kind >= 11 && kind <= 14 && (param == 4u || param == 8u || param == 4160u)) {
         const uint32_t bytes = 1u << 20, iters = 2000;
         uint32_t *buf = nullptr;
         if (hipMalloc(&buf, bytes + 64) != hipSuccess || hipMemset(buf, 1, bytes + 64) != hipSuccess) {

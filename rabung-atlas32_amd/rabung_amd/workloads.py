@@ -29,6 +29,18 @@ def top(n):
     return sorted(out)
 
 
+def top_wide(n):
+    """The n largest primes below 2^33 (the wider domain's hardest case)."""
+    out = []
+    x = (1 << 33) - 1
+    from . import is_prime
+    while len(out) < n:
+        if is_prime(x):
+            out.append(x)
+        x -= 2 if x % 2 else 1
+    return sorted(out)
+
+
 def block(start, n):
     return primes_from(start, n)
 
