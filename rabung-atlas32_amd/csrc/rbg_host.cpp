@@ -379,18 +379,26 @@ double elapsed(rbg_ctx *c, int a, int b) {
 // Host verification of one profile by direct powers (reference
 // certify_witness, VBB/src/advance.cpp:7-13, and the prefix loop of
 // src/census/worker32.cu:37-38).
+// POW(x) = x^e mod p: 64-bit products below 2^32, 128-bit above.
+template <class POW>
+bool certify_with(u64 p, u32 M, u32 L, u32 W, POW pw) {
+    if (M == 0 || W == 0 || (u64)W + M > p || L == 0) return false;
+    const u64 colour = pw(W);
+    for (u32 j = 1; j < M; ++j)
+        if (pw((u64)W + j) != colour) return false;
+    if (W > 1 && pw(W - 1) == colour) return false;
+    if ((u64)W + M < p && pw((u64)W + M) == colour) return false;
+    for (u32 x = 1; x <= L; ++x)
+        if (pw(x) != 1) return false;
+    if ((u64)L + 1 < p && pw((u64)L + 1) == 1) return false;
+    return true;
+}
+
 bool certify(u64 p, u32 r, u32 M, u32 L, u32 W) {
     const u64 e = (p - 1) / r;
-    if (M == 0 || W == 0 || (u64)W + M > p || L == 0) return false;
-    const u64 colour = powmod_w(W, e, p);
-    for (u32 j = 1; j < M; ++j)
-        if (powmod_w((u64)W + j, e, p) != colour) return false;
-    if (W > 1 && powmod_w(W - 1, e, p) == colour) return false;
-    if ((u64)W + M < p && powmod_w((u64)W + M, e, p) == colour) return false;
-    for (u32 x = 1; x <= L; ++x)
-        if (powmod_w(x, e, p) != 1) return false;
-    if ((u64)L + 1 < p && powmod_w((u64)L + 1, e, p) == 1) return false;
-    return true;
+    if (p < (1ull << 32))
+        return certify_with(p, M, L, W, [&](u64 x) { return (u64)rbg::powmod64((u32)x, e, (u32)p); });
+    return certify_with(p, M, L, W, [&](u64 x) { return powmod_w(x, e, p); });
 }
 
 int certify_all(rbg_ctx *c, const rbg_record *out, const u64 *primes, u32 n) {
@@ -462,7 +470,7 @@ int dense_run(rbg_ctx *c, const u64 *primes, const std::vector<u32> &idx_in, boo
         d.planes = 0;
         for (u32 r = 2; r <= 9; ++r)
             if (d.mask & (1u << (r - 2))) d.planes |= rbg::plane_bits(r);
-        d.ntiles = (d.nh + tile_pos - 1) / tile_pos;
+        d.ntiles = (u32)(((u64)d.nh + tile_pos - 1) / tile_pos);  // nh reaches 2^32 - 5 below 2^33
         d.tile_begin = (u32)tiles;
         d.nr = (u32)__builtin_popcount(d.mask);
         d.sum_off = (u32)runs;

@@ -51,6 +51,28 @@ __device__ __forceinline__ PrimeConsts<Dom33> consts_of(const PrimeDesc &d, Dom3
     return {{w(d.p, 0), d.ninv}, w(d.one, 1), w(d.r2, 2), w(d.step, 3)};
 }
 
+constexpr int kPowChains = 4;  // chunks whose powers a lane computes together
+
+// y[i] = x[i]^e (Montgomery forms) for K independent bases and one exponent:
+// the chains share the square-and-multiply schedule, so each multiply of
+// one chain hides the latency of the others.
+template <class D, int K>
+__device__ __forceinline__ void pow_many(const D &dm, const typename D::V *x, u32 e,
+                                         typename D::V one, typename D::V *y) {
+#pragma unroll
+    for (int i = 0; i < K; ++i) y[i] = e ? x[i] : one;
+    if (e < 2) return;
+    const int top = 31 - __builtin_clz(e);
+    for (int bit = top - 1; bit >= 0; --bit) {
+#pragma unroll
+        for (int i = 0; i < K; ++i) y[i] = dm.mul(y[i], y[i]);
+        if ((e >> bit) & 1u) {
+#pragma unroll
+            for (int i = 0; i < K; ++i) y[i] = dm.mul(y[i], x[i]);
+        }
+    }
+}
+
 // Scan one segment [seg_start, seg_start+seg_len) with the calling wave.
 // Writes per-order summaries (positions relative to seg_start) into out[r-2].
 template <class D>
@@ -70,12 +92,23 @@ __device__ void wave_scan(const PrimeDesc &d, const PrimeConsts<D> &k, u32 seg_s
     V xm = k.dom.mul((V)(seg_start + lane), k.r2);
     const u32 nchunks = (seg_len + 63u) >> 6;
     u32 bad = 0;
-    for (u32 c = 0; c < nchunks; ++c) {
+    for (u32 c0 = 0; c0 < nchunks; c0 += kPowChains) {
+        // colour codes of x: y = x^e for kPowChains chunks at once, then
+        // each y's index in the sorted root table, chunk by chunk
+        V xs[kPowChains], ys[kPowChains];
+        xs[0] = xm;
+#pragma unroll
+        for (int i = 1; i < kPowChains; ++i) xs[i] = k.dom.add(xs[i - 1], k.step);
+        pow_many<D, kPowChains>(k.dom, xs, e, k.one, ys);
+        xm = k.dom.add(xs[kPowChains - 1], k.step);
+#pragma unroll
+    for (int ci = 0; ci < kPowChains; ++ci) {
+        const u32 c = c0 + (u32)ci;
+        if (c >= nchunks) break;
         const u32 base = c << 6;
         const u32 valid = min(64u, seg_len - base);
         const u64 vmask = valid == 64u ? ~0ull : ((1ull << valid) - 1ull);
-        // colour code of x: y = x^e, then its index in the sorted root table
-        const V y = k.dom.pow(xm, e, k.one);
+        const V y = ys[ci];
         const u32 lo = table_index(s_val, m, y);
         bad |= (s_val[lo] != y) & (lane < valid);
         u32 code = s_code[lo];
@@ -99,7 +132,7 @@ __device__ void wave_scan(const PrimeDesc &d, const PrimeConsts<D> &k, u32 seg_s
             }
             order_update(st[k], C, N, base);
         }
-        xm = k.dom.add(xm, k.step);
+    }
     }
     if (__builtin_amdgcn_ballot_w64(bad != 0)) atomicOr(err, 1u);
 #pragma unroll

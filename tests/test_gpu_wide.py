@@ -15,7 +15,7 @@ import pytest
 
 import golden_data
 import rabung_amd
-from rabung_amd import codec
+from rabung_amd import codec, workloads
 from test_worker import WORKER
 
 pytestmark = pytest.mark.gpu
@@ -74,6 +74,15 @@ def test_dense_wide_against_sparse(engine):
     assert dense == sparse
     plain = engine.profile_block(primes[:1], method=rabung_amd.METHOD_PLAIN)
     assert plain == sparse[:1]
+
+
+def test_dense_wide_top_of_domain(engine):
+    """The largest prime below 2^33: its half interval (4294967291 positions)
+    fills the 32-bit tile space to the last tile."""
+    p = workloads.top_wide(1)
+    assert p == [8589934583]
+    dense = engine.profile_block(p, method=rabung_amd.METHOD_DENSE)
+    assert dense == engine.profile_block(p, certify=True)
 
 
 def test_block_straddling_2_32(engine):
