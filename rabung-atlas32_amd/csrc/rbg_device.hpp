@@ -6,20 +6,21 @@
 
 namespace rbg {
 
-// Montgomery product with 32-bit halves (v_mul_lo_u32 / v_mul_hi_u32).  The
-// sum u = hi + mh + (lo != 0) reaches 2p > 2^32 for p > 2^31: it is formed
-// with one add-with-carry (carry c1 is bit 32 of u), and u >= p iff c1 or
-// s - p does not borrow, so the reduction is one subtract-with-borrow and one
-// select (8 VALU instructions, 4 of them multiplies).
+// Montgomery product with 32-bit halves (v_mul_lo_u32 / v_mul_hi_u32).
+// REDC(a b) = hi + mh + carry with carry = [lo != 0] (the low words of a b
+// and m p cancel) and the sum below 2p, which exceeds 2^32 for p > 2^31.
+// With s = mh + carry <= p and q = p - s, the reduced value is hi - q when
+// hi >= q and hi + s otherwise (no overflow either way): one compare and one
+// select, no carry flags, so independent products do not serialise on the
+// condition-code register (4 multiplies + 6 other VALU instructions).
 __device__ __forceinline__ u32 mont_mul(u32 a, u32 b, u32 p, u32 ninv) {
-    const u32 lo = a * b;
-    const u32 hi = __umulhi(a, b);
+    const u64 t = (u64)a * b;  // one v_mad_u64_u32
+    const u32 lo = (u32)t, hi = (u32)(t >> 32);
     const u32 m = lo * ninv;
-    const u32 mh = __umulhi(m, p);
-    unsigned int c1, borrow;
-    const u32 s = __builtin_addc(hi, mh, lo != 0u ? 1u : 0u, &c1);
-    const u32 t = __builtin_subc(s, p, 0u, &borrow);
-    return (c1 | (borrow ^ 1u)) ? t : s;
+    // mh + [lo != 0] = ceil(m p / 2^32): the low word of m p is -lo mod 2^32
+    const u32 s = (u32)(((u64)m * p + 0xFFFFFFFFull) >> 32);
+    const u32 q = p - s;
+    return hi >= q ? hi - q : hi + s;
 }
 
 __device__ __forceinline__ u32 mont_pow(u32 base, u32 e, u32 p, u32 ninv, u32 one) {
