@@ -66,11 +66,22 @@ __device__ __forceinline__ SpConsts<Dom33> sp_consts(const SparsePrime &s, Dom33
     return {{w(s.p, 0), s.ninv}, w(s.one, 1), w(s.r2, 2)};
 }
 
+// Labels are completely multiplicative: (a b)^e = a^e b^e, so
+// j(a b) = j(a) + j(b) mod m.  The first pass powers only y = 2, 3 and the y
+// coprime to 6 (a third of 1..B), listed densely so that every lane works:
+// index i -> 2, 3, then 1, 5, 7, 11, 13, ... (3(i-2) + 1 + ((i-2) & 1)); the
+// second pass (sp_labels_rest) fills y = 2^a 3^b x from j(2), j(3), j(x).
+__device__ __forceinline__ u32 coprime6_at(u32 i) {
+    return i < 2u ? i + 2u : 3u * (i - 2u) + 1u + ((i - 2u) & 1u);
+}
+
 template <class D>
 __device__ __forceinline__ void labels_of(const SparsePrime &d, const typename D::V *tab_val,
                                           const u16 *tab_lab, u16 *labels, u32 *err,
                                           typename D::V *s_val, u16 *s_lab) {
     typedef typename D::V V;
+    const u32 i0 = (blockIdx.x - d.lblk) * (256u * kLabelsPerThread);
+    if (coprime6_at(i0) > d.B && i0 >= 2u) return;  // the block's indices lie past B
     const SpConsts<D> k = sp_consts(d, D{});
     const u32 voff = D::wide ? d.tab64 : d.tab_off;
     for (u32 j = threadIdx.x; j < d.m; j += blockDim.x) {
@@ -78,11 +89,10 @@ __device__ __forceinline__ void labels_of(const SparsePrime &d, const typename D
         s_lab[j] = tab_lab[d.tab_off + j];
     }
     __syncthreads();
-    const u32 y0 = 1u + (blockIdx.x - d.lblk) * (256u * kLabelsPerThread);
     if (blockIdx.x == d.lblk && threadIdx.x == 0) labels[d.lab_off] = 0;
     u32 bad = 0;
     for (u32 i = 0; i < kLabelsPerThread; ++i) {
-        const u32 y = y0 + i * 256u + threadIdx.x;
+        const u32 y = coprime6_at(i0 + i * 256u + threadIdx.x);
         if (y > d.B) break;
         const V v = k.dom.pow(k.dom.mul((V)y, k.r2), d.e, k.one);
         const u32 idx = table_index(s_val, d.m, v);
@@ -108,6 +118,29 @@ __global__ __launch_bounds__(256) void sp_labels(const SparsePrime *__restrict__
         labels_of<Dom33>(d, tab_val64, tab_lab, labels, err, s_val, s_lab);
     else
         labels_of<Dom32>(d, tab_val, tab_lab, labels, err, (u32 *)s_val, s_lab);
+}
+
+// Second pass: y = 2^a 3^b x with x coprime to 6 and y >= 4 not coprime to
+// 6 gets j(y) = (a j(2) + b j(3) + j(x)) mod m (j(1) = 0).
+__global__ __launch_bounds__(256) void sp_labels_rest(const SparsePrime *__restrict__ P,
+                                                      u32 nprimes, u16 *__restrict__ labels) {
+    const u32 pi = __builtin_amdgcn_readfirstlane(find_prime(P, nprimes, blockIdx.x, 0));
+    const SparsePrime d = P[pi];
+    const u16 *lab = labels + d.lab_off;
+    const u32 j2 = lab[2], j3 = lab[3];
+    const u32 y0 = 1u + (blockIdx.x - d.lblk) * (256u * kLabelsPerThread);
+    for (u32 i = 0; i < kLabelsPerThread; ++i) {
+        const u32 y = y0 + i * 256u + threadIdx.x;
+        if (y > d.B) break;
+        if (y < 4u || ((y & 1u) && y % 3u != 0u)) continue;
+        const u32 a = (u32)__builtin_ctz(y);
+        u32 x = y >> a, b = 0;
+        while (x % 3u == 0u) {
+            x /= 3u;
+            ++b;
+        }
+        labels[(u64)d.lab_off + y] = (u16)((a * j2 + b * j3 + lab[x]) % d.m);
+    }
 }
 
 // inv[inv_off + d] = Montgomery form d^{-1} R mod p for d in [1, D] (d^(p-2));
@@ -858,6 +891,8 @@ extern "C" hipError_t rbg_sparse_prepare(const SparsePrime *P, uint32_t nprimes,
                                          uint32_t *planes, uint32_t *err, hipStream_t stream) {
     hipLaunchKernelGGL(rbg::sp_labels, dim3(label_blocks), dim3(256), 0, stream, P, nprimes,
                        tab_val, tab_val64, tab_lab, labels, err);
+    hipLaunchKernelGGL(rbg::sp_labels_rest, dim3(label_blocks), dim3(256), 0, stream, P, nprimes,
+                       labels);
     hipLaunchKernelGGL(rbg::sp_inverses, dim3(inverse_blocks), dim3(256), 0, stream, P, nprimes,
                        inv, inv64);
     hipLaunchKernelGGL(rbg::sp_planes, dim3(plane_blocks), dim3(256), 0, stream, P, nprimes, prof,

@@ -85,6 +85,21 @@ def test_dense_wide_top_of_domain(engine):
     assert dense == engine.profile_block(p, certify=True)
 
 
+def test_random_wide_primes_sparse_vs_dense(engine):
+    """Seeded primes drawn uniformly from (2^32, 2^33): the sparse engine and
+    the dense scan (independent algorithms, both on 33-bit residues) agree bit
+    for bit, A4 included, and the sparse records pass direct certification."""
+    rng = random.Random(20261204)
+    primes = sorted({rabung_amd.primes_from_wide(rng.randrange(R32, 2 * R32 - 2000), 1)[0]
+                     for _ in range(6)})
+    print("random wide", primes, flush=True)
+    sparse = engine.profile_block(primes, certify=True)
+    assert engine.last_timing()["sparse_primes"] == len(primes)
+    dense = engine.profile_block(primes, method=rabung_amd.METHOD_DENSE)
+    for s_, d_ in zip(sparse, dense):
+        assert s_ == d_, (s_["p"], s_, d_)
+
+
 def test_block_straddling_2_32(engine):
     """One call with primes on both sides of 2^32: each domain's kernels,
     records in input order, equal to single-domain calls."""
