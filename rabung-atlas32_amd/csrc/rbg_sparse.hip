@@ -466,47 +466,54 @@ struct FilterCtx {
     u32 d, t, q, B, bmin, wpp;
 };
 
-// Words of one band kind: anchors b = (n0 + band * step) d + c for c in
-// [c_lo, c_hi], split into words of 64 consecutive b; `total` words.
-struct BandKind {
-    u32 c_lo, c_hi, n0, step, W, total;
+// Scheduled anchors of one task as spans.  The schedule keeps the anchors
+// b = n d + c (b in [bmin, B]) with c in [1, (d-1)/2] and n = phi mod s, or
+// c in [d/2 + 1, d-1] and n = t-2-phi mod s (reference schedule
+// GPU_METHOD_EN.tex:380-455).  Since t-2-phi = phi (t even) or phi - 1 (t
+// odd) mod s, the two sides pair up into contiguous spans
+// S(m) = [m d + s_lo, m d + s_lo + width - 1] for m = rm mod s:
+//   t even: S(m) = [m d + 1, m d + d - 1]; c = d/2 (d even) is not scheduled;
+//   t odd:  S(m) = [m d + d/2 + 1, (m+1) d + (d-1)/2]; (m+1) d is not.
+// At stage j every span reads the span j d below it, so consecutive words of
+// a span are consecutive in the colour planes at every stage.  For d <= 2
+// one span [max(bmin, t or t-1), B] holds every anchor.  Words are 64
+// consecutive anchors of a span; `total` words.
+struct Spans {
+    u32 s_lo, width, excl;  // span offset and width; excluded offset (~0u: none)
+    u32 m_first, nsb;       // first span index, spans (step s)
+    u32 W, total;           // words per span, words
     float rW;
 };
 
-// Band layout: kind 0 = c in [1, ca] (2c < d, n = phi mod s), kind 1 =
-// c in [cb, d-1] (2c > d, n = t-2-phi mod s); for d <= 2 one unscheduled band
-// covers [max(bmin, .), B] (reference schedule GPU_METHOD_EN.tex:380-455).
-__device__ __forceinline__ BandKind band_kind(u32 kind, u32 d, u32 t, u32 sp, u32 phi, u32 bmin,
-                                              u32 B) {
-    BandKind k;
-    k.total = 0;
-    k.c_lo = 1;
-    k.c_hi = 0;
-    k.n0 = 0;
-    k.step = 0;
-    k.W = 1;
+__device__ __forceinline__ Spans spans_of(u32 d, u32 t, u32 sp, u32 phi, u32 bmin, u32 B) {
+    Spans S;
+    S.excl = ~0u;
+    S.m_first = 0;
+    S.nsb = 0;
     if (d <= 2) {
         const u32 lo_b = max(bmin, d == 1 ? t : t - 1u);
-        if (kind || lo_b > B) return k;
-        k.c_lo = lo_b;
-        k.c_hi = B;
-        k.W = (B - lo_b + 64u) >> 6;
-        k.total = k.W;
+        S.s_lo = lo_b;
+        S.width = lo_b <= B ? B - lo_b + 1u : 0u;
+        S.nsb = lo_b <= B ? 1u : 0u;
     } else {
-        const u32 ca = (d - 1u) / 2u, cb = d / 2u + 1u;
-        k.c_lo = kind ? cb : 1u;
-        k.c_hi = kind ? d - 1u : ca;
-        if (k.c_lo > k.c_hi) return k;
-        const u32 res = kind ? (t - 2u - phi) % sp : phi;
-        const u32 n_lo = bmin / d, n_hi = B / d;
-        k.n0 = n_lo + (res + sp - n_lo % sp) % sp;
-        const u32 nb = k.n0 > n_hi ? 0u : (n_hi - k.n0) / sp + 1u;
-        k.step = sp;
-        k.W = (k.c_hi - k.c_lo + 64u) >> 6;
-        k.total = nb * k.W;  // < 2^23: at most B/64 + nb words
+        const bool odd_t = (t & 1u) != 0;
+        const u32 rm = odd_t ? (phi + sp - 1u) % sp : phi;
+        S.s_lo = odd_t ? d / 2u + 1u : 1u;
+        S.width = odd_t ? d + (d - 1u) / 2u - d / 2u : d - 1u;
+        if (odd_t) S.excl = d - S.s_lo;
+        else if ((d & 1u) == 0) S.excl = d / 2u - 1u;
+        // spans meeting [bmin, B]: m d + s_lo + width - 1 >= bmin and
+        // m d + s_lo <= B
+        const int x = (int)bmin + 1 - (int)S.s_lo - (int)S.width;
+        const u32 m_lo = x <= 0 ? 0u : ((u32)x + d - 1u) / d;
+        S.m_first = m_lo + (rm + sp - m_lo % sp) % sp;
+        const u32 m_hi = (B - S.s_lo) / d;
+        S.nsb = S.m_first > m_hi ? 0u : (m_hi - S.m_first) / sp + 1u;
     }
-    k.rW = 1.0f / (float)k.W;
-    return k;
+    S.W = (S.width + 63u) >> 6;
+    S.total = S.nsb * S.W;  // < 2^23: at most B/64 + nsb words
+    S.rW = 1.0f / (float)max(S.W, 1u);
+    return S;
 }
 
 // Screen and complete the queued words (one lane per word).
@@ -559,32 +566,13 @@ __device__ __forceinline__ void wave_lds_fence() {
 constexpr u32 kRefill = 48;  // smallest cohort of idle lanes that takes new words
 
 template <class D, int NP>
-__device__ void filter_task(const FilterCtx<D> &c, u32 sp, u32 phi, u32 *qb, u64 *qs, u32 *kinds,
+__device__ void filter_task(const FilterCtx<D> &c, u32 sp, u32 phi, u32 *qb, u64 *qs,
                             u32 &n_windows, u32 &n_words, u32 &n_done) {
     constexpr int G = NP == 1 ? 8 : NP == 2 ? 4 : 2;  // stages per step (register budget)
     const u32 lane = threadIdx.x & 63u;
     const u32 d = c.d, t = c.t, B = c.B, bmin = c.bmin;
-    // the two band kinds go to the wave's LDS slot: a lane reads the record
-    // of its word's kind by address (no per-lane selects, no scalar registers
-    // held over the loop)
-    u32 total0, total;
-    {
-        const BandKind k0 = band_kind(0, d, t, sp, phi, bmin, B);
-        const BandKind k1 = band_kind(1, d, t, sp, phi, bmin, B);
-        total0 = k0.total;
-        total = k0.total + k1.total;
-        if (lane < 2) {
-            const BandKind &k = lane ? k1 : k0;
-            kinds[lane * 8 + 0] = k.c_lo;
-            kinds[lane * 8 + 1] = k.c_hi;
-            kinds[lane * 8 + 2] = k.n0;
-            kinds[lane * 8 + 3] = k.step;
-            kinds[lane * 8 + 4] = k.W;
-            kinds[lane * 8 + 5] = __float_as_uint(k.rW);
-            kinds[lane * 8 + 6] = lane ? k0.total : 0u;  // first word of the kind
-        }
-        wave_lds_fence();
-    }
+    const Spans S = spans_of(d, t, sp, phi, bmin, B);
+    const u32 total = S.total, span_step = sp * d;
     const u32 *plb[NP];
 #pragma unroll
     for (int pb = 0; pb < NP; ++pb) plb[pb] = c.pl + (u32)pb * c.wpp;
@@ -611,28 +599,25 @@ __device__ void filter_task(const FilterCtx<D> &c, u32 sp, u32 phi, u32 *qb, u64
                 if (!have && !pend) {
                     const u32 my = next + lanes_below(idle);
                     if (my < total) {
-                        const u32 *kp = kinds + (my >= total0 ? 8u : 0u);
-                        const u32 wi = my - kp[6];
-                        const u32 W = kp[4];
                         // exact quotient: the float estimate is at most one
                         // low (|error| < 1/W because total < 2^23)
-                        u32 band = (u32)((float)wi * __uint_as_float(kp[5]));
-                        if ((band + 1u) * W <= wi) ++band;
-                        const u32 w = wi - band * W;
-                        const u32 nd = (kp[2] + band * kp[3]) * d;
-                        b0 = nd + kp[0] + 64u * w;
-                        const u32 bend = min(nd + kp[1], B);
+                        u32 span = (u32)((float)my * S.rW);
+                        if ((span + 1u) * S.W <= my) ++span;
+                        const u32 w = my - span * S.W;
+                        const u32 lo = S.m_first * d + S.s_lo + span * span_step;
+                        b0 = lo + 64u * w;
+                        // anchors [first, last] of the word in the span and
+                        // in [bmin, B]
+                        const u32 first = max(max(lo, bmin), b0);
+                        const u32 last = min(min(lo + S.width - 1u, B), b0 + 63u);
                         u64 valid = 0;
-                        if (b0 <= bend) {
-                            const u32 nbits = min(64u, bend - b0 + 1u);
-                            valid = nbits == 64u ? ~0ull : ((1ull << nbits) - 1ull);
-                            if (b0 < bmin) {
-                                const u32 below = bmin - b0;
-                                valid = below >= nbits ? 0ull : valid & (~0ull << below);
-                            }
+                        if (first <= last) {
+                            valid = (~0ull << (first - b0)) & (~0ull >> (63u - (last - b0)));
                             if (even)
                                 valid &= (b0 & 1u) ? 0x5555555555555555ull : 0xAAAAAAAAAAAAAAAAull;
                             if (thirds) valid &= ~(kThirds << ((3u - b0 % 3u) % 3u));
+                            const u32 e = lo + S.excl - b0;  // the span's unscheduled point
+                            if (S.excl != ~0u && e < 64u) valid &= ~(1ull << e);
                         }
                         if (valid) {
                             have = true;
@@ -767,13 +752,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void s
     // per-wave queue of passing words: start b0 and surviving anchors
     __shared__ u32 s_qb[kFilterWaves][64];
     __shared__ u64 s_qs[kFilterWaves][64];
-    __shared__ u32 s_kinds[kFilterWaves][16];
     if (threadIdx.x < 3) s_stats[threadIdx.x] = 0;
     __syncthreads();
     u64 n_windows = 0, n_words = 0, n_done = 0;
     u32 *qb = s_qb[threadIdx.x >> 6];
     u64 *qs = s_qs[threadIdx.x >> 6];
-    u32 *kinds = s_kinds[threadIdx.x >> 6];
     // XCD-aware schedule: workgroups are dispatched round-robin over the 8
     // XCDs, so block b runs on XCD b % 8.  XCD x owns the profiles
     // k = x, x+8, x+16, ... and walks them in groups of kGroup profiles (a
@@ -828,10 +811,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void s
         c.bmin = ((t - 1u) * dd + 1u) / 2u;
         u32 tw = 0, tn = 0, td = 0;  // this task's counts (u32 per lane)
         switch (fg->np) {
-        case 1: filter_task<D, 1>(c, sp, phi, qb, qs, kinds, tw, tn, td); break;
-        case 2: filter_task<D, 2>(c, sp, phi, qb, qs, kinds, tw, tn, td); break;
-        case 3: filter_task<D, 3>(c, sp, phi, qb, qs, kinds, tw, tn, td); break;
-        default: filter_task<D, 4>(c, sp, phi, qb, qs, kinds, tw, tn, td); break;
+        case 1: filter_task<D, 1>(c, sp, phi, qb, qs, tw, tn, td); break;
+        case 2: filter_task<D, 2>(c, sp, phi, qb, qs, tw, tn, td); break;
+        case 3: filter_task<D, 3>(c, sp, phi, qb, qs, tw, tn, td); break;
+        default: filter_task<D, 4>(c, sp, phi, qb, qs, tw, tn, td); break;
         }
         n_windows += tw;
         n_words += tn;
