@@ -41,6 +41,62 @@ def labels_of(p, m):
     return lambda y: root[pow(y, e, p)]
 
 
+def band_ranges(d, t, sp, phi, bmin, B):
+    """Scheduled anchors as the reference states them: for each side of the
+    band (kind 0: c in [1, (d-1)/2], n = phi mod s; kind 1: c in
+    [d/2 + 1, d-1], n = t-2-phi mod s) the ranges [n d + c_lo, n d + c_hi]
+    clipped to [bmin, B] (GPU_METHOD_EN.tex:380-455; d <= 2: one range)."""
+    if d <= 2:
+        return [(max(bmin, t if d == 1 else t - 1), B)]
+    out = []
+    ca, cb = (d - 1) // 2, d // 2 + 1
+    for kind in (0, 1):
+        c_lo, c_hi = (cb, d - 1) if kind else (1, ca)
+        if c_lo > c_hi:
+            continue
+        res = (t - 2 - phi) % sp if kind else phi
+        n_lo, n_hi = bmin // d, B // d
+        n = n_lo + (res + sp - n_lo % sp) % sp
+        while n <= n_hi:
+            out.append((max(n * d + c_lo, bmin), min(n * d + c_hi, B)))
+            n += sp
+    return out
+
+
+def span_ranges(d, t, sp, phi, bmin, B):
+    """The same anchors as the kernel enumerates them (spans_of): the two
+    sides of a band are adjacent, because t-2-phi = phi (t even) or phi - 1
+    (t odd) mod s, so they form spans [m d + s_lo, m d + s_lo + width - 1]
+    for m = rm mod s with one unscheduled point inside (c = d/2 for even t
+    and even d, c = 0 for odd t).  Returns (lo, hi, excluded or None)."""
+    if d <= 2:
+        return [(max(bmin, t if d == 1 else t - 1), B, None)]
+    odd_t = t % 2 == 1
+    rm = (phi + sp - 1) % sp if odd_t else phi
+    s_lo = d // 2 + 1 if odd_t else 1
+    width = d + (d - 1) // 2 - d // 2 if odd_t else d - 1
+    excl = d - s_lo if odd_t else (d // 2 - 1 if d % 2 == 0 else None)
+    x = bmin + 1 - s_lo - width
+    m = 0 if x <= 0 else (x + d - 1) // d
+    m += (rm + sp - m % sp) % sp
+    out = []
+    while m * d + s_lo <= B:
+        lo = m * d + s_lo
+        out.append((max(lo, bmin), min(lo + width - 1, B),
+                    None if excl is None else lo + excl))
+        m += sp
+    return out
+
+
+def anchors(ranges):
+    got = set()
+    for lo, hi, *ex in ranges:
+        got.update(range(lo, hi + 1))
+        if ex and ex[0] is not None:
+            got.discard(ex[0])
+    return got
+
+
 def sparse_profile(p, r, Q=48, screen=True, stats=None):
     """(M, witness, L) of order r by the anchor search; None if refused."""
     if stats is None:
@@ -142,25 +198,10 @@ def sparse_profile(p, r, Q=48, screen=True, stats=None):
         sp = q - t + 1
         phi = ((t - 1) // 2) % sp
         bmin = ((t - 1) * d + 1) // 2
-        ranges = []
-        if d <= 2:
-            lo_b = max(bmin, t if d == 1 else t - 1)
-            ranges.append((lo_b, B))
-        else:
-            ca, cb = (d - 1) // 2, d // 2 + 1
-            for kind in (0, 1):
-                c_lo, c_hi = (cb, d - 1) if kind else (1, ca)
-                if c_lo > c_hi:
-                    continue
-                res = (t - 2 - phi) % sp if kind else phi
-                n_lo, n_hi = bmin // d, B // d
-                n0 = n_lo + (res + sp - n_lo % sp) % sp
-                n = n0
-                while n <= n_hi:
-                    ranges.append((max(n * d + c_lo, bmin), min(n * d + c_hi, B)))
-                    n += sp
-        for lo_b, hi_b in ranges:
+        for lo_b, hi_b, excl in span_ranges(d, t, sp, phi, bmin, B):
             for b in range(lo_b, hi_b + 1):
+                if b == excl:
+                    continue
                 if d % 2 == 0 and b % 2 == 0:
                     continue
                 if d % 3 == 0 and b % 3 == 0:

@@ -46,3 +46,23 @@ def test_screen_prunes_without_changing_results():
             sparse_model.sparse_profile(p, 2, screen=False, stats=b) == \
             (rec["M"][0], rec["W"][0], rec["L"][0])
         assert a["completions"] < b["completions"]
+
+
+def test_spans_hold_exactly_the_scheduled_anchors():
+    """The kernel's fused spans and the reference's two-sided band schedule
+    select the same anchors, for every denominator of several covers and
+    thresholds of both parities (odd and even t, d <= 2, s = 1)."""
+    import random
+    rng = random.Random(11)
+    for p in (65537 + 2, 1000003, 257000003, 4294967291):
+        B, D = sparse_model.cover(p)
+        ds = sorted(set(range(1, min(D, 60) + 1)) | {D, D - 1} | {rng.randint(1, D) for _ in range(20)})
+        for q in (3, 4, 5, 6, 7, 10, 17, 24, 25, 33, 47, 48):
+            t = min(q, max(3, (q + 1) // 2))
+            sp = q - t + 1
+            phi = ((t - 1) // 2) % sp
+            for d in ds:
+                bmin = ((t - 1) * d + 1) // 2
+                bands = sparse_model.anchors(sparse_model.band_ranges(d, t, sp, phi, bmin, B))
+                spans = sparse_model.anchors(sparse_model.span_ranges(d, t, sp, phi, bmin, B))
+                assert bands == spans, (p, q, d)
