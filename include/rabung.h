@@ -164,8 +164,8 @@ typedef struct rbg_timing {
     uint64_t positions;      /* half-interval positions of the dense scan */
     uint64_t products;       /* Montgomery products of the dense scan */
     uint64_t launches;       /* kernel launches of the dominant stages */
-    uint64_t filter_windows; /* 64-bit plane windows compared by the filter */
-    uint64_t filter_words;   /* 64-candidate words the filter examined */
+    uint64_t filter_windows; /* plane windows compared by the filter, in 32-anchor dwords */
+    uint64_t filter_words;   /* 64- or 96-anchor words the filter examined */
     uint64_t filter_completions; /* surviving anchors extended to their runs */
 } rbg_timing;
 

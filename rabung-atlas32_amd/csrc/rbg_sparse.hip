@@ -381,7 +381,7 @@ __device__ __forceinline__ u64 match_left(const u32 *const *plb, const u64 *base
     const u32 idx = (u32)(lo + Bs);
 #pragma unroll
     for (int pb = 0; pb < NP; ++pb) mm |= base[pb] ^ window64(plb[pb], idx);
-    n_windows += NP;
+    n_windows += 2u * NP;  // counted in 32-anchor plane dwords
     const int64_t known = Bs - lo + 1;  // bits 0..known-1 lie in the table
     if (known < 64) mm &= (1ull << known) - 1ull;
     return ~mm;
@@ -403,7 +403,7 @@ __device__ __forceinline__ u64 match_right(const u32 *const *plb, const u64 *bas
         for (int pb = 0; pb < NP; ++pb) mm |= base[pb] ^ (window64(plb[pb], 0) << shift);
         mm &= ~((1ull << shift) - 1ull);
     }
-    n_windows += NP;
+    n_windows += 2u * NP;
     return ~mm;
 }
 
@@ -476,13 +476,10 @@ struct FilterCtx {
 //   t odd:  S(m) = [m d + d/2 + 1, (m+1) d + (d-1)/2]; (m+1) d is not.
 // At stage j every span reads the span j d below it, so consecutive words of
 // a span are consecutive in the colour planes at every stage.  For d <= 2
-// one span [max(bmin, t or t-1), B] holds every anchor.  Words are 64
-// consecutive anchors of a span; `total` words.
+// one span [max(bmin, t or t-1), B] holds every anchor.
 struct Spans {
     u32 s_lo, width, excl;  // span offset and width; excluded offset (~0u: none)
     u32 m_first, nsb;       // first span index, spans (step s)
-    u32 W, total;           // words per span, words
-    float rW;
 };
 
 __device__ __forceinline__ Spans spans_of(u32 d, u32 t, u32 sp, u32 phi, u32 bmin, u32 B) {
@@ -510,31 +507,35 @@ __device__ __forceinline__ Spans spans_of(u32 d, u32 t, u32 sp, u32 phi, u32 bmi
         const u32 m_hi = (B - S.s_lo) / d;
         S.nsb = S.m_first > m_hi ? 0u : (m_hi - S.m_first) / sp + 1u;
     }
-    S.W = (S.width + 63u) >> 6;
-    S.total = S.nsb * S.W;  // < 2^23: at most B/64 + nsb words
-    S.rW = 1.0f / (float)max(S.W, 1u);
     return S;
 }
+
+// Words are 96 consecutive anchors of a span: a stage reads one 96-bit
+// window per plane, at any bit offset, with one four-dword load.
+constexpr u32 kWordBits = 96;
+constexpr u32 kQueue = 128;  // entries of a wave's queue of passing words (two per word at most)
 
 // Screen and complete the queued words (one lane per word).
 template <class D, int NP>
 __device__ __forceinline__ void drain_queue(const FilterCtx<D> &c, const u32 *const *plb, u32 qn, const u32 *qb,
                             const u64 *qs, u32 lane, u32 &n_windows, u32 &n_done) {
-    if (lane >= qn) return;
-    const u32 b0 = qb[lane];
-    u64 surv = qs[lane];
-    if (c.q > c.t) {
-        u64 base[NP];
+    for (u32 e = lane; e < qn; e += 64u) {
+        const u32 b0 = qb[e];
+        u64 surv = qs[e];
+        if (!surv) continue;
+        if (c.q > c.t) {
+            u64 base[NP];
 #pragma unroll
-        for (int pb = 0; pb < NP; ++pb) base[pb] = window64(plb[pb], b0 + c.B);
-        surv &= screen<NP>(plb, base, surv, b0, c.B, c.d, c.t, c.q - c.t, n_windows);
-    }
-    while (surv) {
-        const u32 bit = (u32)__builtin_ctzll(surv);
-        surv &= surv - 1ull;
-        complete_anchor<D>(c.s, c.f, c.pl, b0 + bit, c.d, c.dinv, c.t, c.tab_val, c.tab_lab,
-                        c.best, c.failed);
-        ++n_done;
+            for (int pb = 0; pb < NP; ++pb) base[pb] = window64(plb[pb], b0 + c.B);
+            surv &= screen<NP>(plb, base, surv, b0, c.B, c.d, c.t, c.q - c.t, n_windows);
+        }
+        while (surv) {
+            const u32 bit = (u32)__builtin_ctzll(surv);
+            surv &= surv - 1ull;
+            complete_anchor<D>(c.s, c.f, c.pl, b0 + bit, c.d, c.dinv, c.t, c.tab_val,
+                               c.tab_lab, c.best, c.failed);
+            ++n_done;
+        }
     }
 }
 // 64-bit window from the three words at its first bit (shift sh = bit & 31).
@@ -550,16 +551,17 @@ __device__ __forceinline__ void wave_lds_fence() {
 
 // Every scheduled word of one (profile, denominator) task for NP planes.
 //
-// Each lane holds one word (64 consecutive anchors b) and tests it G stages
-// per step: for stages j the 64-bit windows at numerators b - j d must equal
+// Each lane holds one word (96 consecutive anchors b) and tests it G stages
+// per step: for stages j the 96-bit windows at numerators b - j d must equal
 // the base window at b.  The base words of a new word are loaded when the
 // lane takes it and aligned only after the step's stage loads are issued,
 // so taking a word costs no extra memory round trip.  A lane whose word
 // dies or passes takes the next word at once (words are handed out in order
 // with a ballot), so the lanes stay busy however differently the words end.
 // Passing words (anchors that survived every stage) go to a per-wave LDS
-// queue of 64 entries; the queue is screened and completed with all lanes at
-// once outside the word loop, at the end of the task or when it is full (the
+// queue, one entry per half (anchors b0..b0+63, b0+64..b0+95) that has
+// survivors; the queue is screened and completed with all lanes at once
+// outside the word loop, at the end of the task or when it is full (the
 // lanes whose words did not fit keep them and the word loop stops handing
 // out words until the queue has been drained).  Stages past t-1 re-read
 // stage t-1, which keeps every address inside the table.
@@ -568,11 +570,16 @@ constexpr u32 kRefill = 48;  // smallest cohort of idle lanes that takes new wor
 template <class D, int NP>
 __device__ void filter_task(const FilterCtx<D> &c, u32 sp, u32 phi, u32 *qb, u64 *qs,
                             u32 &n_windows, u32 &n_words, u32 &n_done) {
-    constexpr int G = NP == 1 ? 8 : NP == 2 ? 4 : 2;  // stages per step (register budget)
+    constexpr u32 WB = kWordBits;
+    constexpr u32 NW = 4;  // dwords per window load
+    constexpr int G = NP == 1 ? 6 : NP == 2 ? 3 : NP == 3 ? 2 : 1;  // stages per step (register budget)
     const u32 lane = threadIdx.x & 63u;
     const u32 d = c.d, t = c.t, B = c.B, bmin = c.bmin;
     const Spans S = spans_of(d, t, sp, phi, bmin, B);
-    const u32 total = S.total, span_step = sp * d;
+    const u32 W = (S.width + WB - 1u) / WB;  // words per span
+    const u32 total = S.nsb * W;  // < 2^23: at most B/96 + nsb words
+    const float rW = 1.0f / (float)max(W, 1u);
+    const u32 span_step = sp * d;
     const u32 *plb[NP];
 #pragma unroll
     for (int pb = 0; pb < NP; ++pb) plb[pb] = c.pl + (u32)pb * c.wpp;
@@ -580,15 +587,18 @@ __device__ void filter_task(const FilterCtx<D> &c, u32 sp, u32 phi, u32 *qb, u64
     u32 next = 0, qn = 0;
     bool pend = false;  // a passing word that did not fit the queue
     u32 b0 = 0;
-    u64 live = 0;
+    u64 live = 0;    // surviving anchors b0 .. b0+63
+    u32 live_hi = 0; // surviving anchors b0+64 .. b0+95
     for (;;) {
         bool stall = false;
         bool have = false;
         u32 j = 1, sh = 0;
         int io = 0, io_last = 0;  // table index of stage j and of stage t-1
-        u32 braw[NP][3];
+        u32 braw[NP][NW];
 #pragma unroll
-        for (int pb = 0; pb < NP; ++pb) braw[pb][0] = braw[pb][1] = braw[pb][2] = 0;
+        for (int pb = 0; pb < NP; ++pb)
+#pragma unroll
+            for (u32 k = 0; k < NW; ++k) braw[pb][k] = 0;
         for (;;) {
             // idle lanes take words in cohorts of at least kRefill (or all
             // 64 at once): a cohort starts together on consecutive words, so
@@ -601,27 +611,42 @@ __device__ void filter_task(const FilterCtx<D> &c, u32 sp, u32 phi, u32 *qb, u64
                     if (my < total) {
                         // exact quotient: the float estimate is at most one
                         // low (|error| < 1/W because total < 2^23)
-                        u32 span = (u32)((float)my * S.rW);
-                        if ((span + 1u) * S.W <= my) ++span;
-                        const u32 w = my - span * S.W;
+                        u32 span = (u32)((float)my * rW);
+                        if ((span + 1u) * W <= my) ++span;
+                        const u32 w = my - span * W;
                         const u32 lo = S.m_first * d + S.s_lo + span * span_step;
-                        b0 = lo + 64u * w;
+                        b0 = lo + WB * w;
                         // anchors [first, last] of the word in the span and
-                        // in [bmin, B]
-                        const u32 first = max(max(lo, bmin), b0);
-                        const u32 last = min(min(lo + S.width - 1u, B), b0 + 63u);
+                        // in [bmin, B], as offsets from b0
+                        const u32 first_b = max(max(lo, bmin), b0);
+                        const u32 last_b = min(min(lo + S.width - 1u, B), b0 + WB - 1u);
+                        const u32 first = first_b - b0, last = last_b - b0;
                         u64 valid = 0;
-                        if (first <= last) {
-                            valid = (~0ull << (first - b0)) & (~0ull >> (63u - (last - b0)));
-                            if (even)
-                                valid &= (b0 & 1u) ? 0x5555555555555555ull : 0xAAAAAAAAAAAAAAAAull;
-                            if (thirds) valid &= ~(kThirds << ((3u - b0 % 3u) % 3u));
+                        u32 valid_hi = 0;
+                        if (first_b <= last_b) {
+                            if (first < 64u)
+                                valid = (~0ull << first) & (~0ull >> (63u - min(last, 63u)));
+                            if (last >= 64u)
+                                valid_hi = (~0u << (max(first, 64u) - 64u)) & (~0u >> (95u - last));
+                            if (even) {
+                                const u64 par = (b0 & 1u) ? 0x5555555555555555ull : 0xAAAAAAAAAAAAAAAAull;
+                                valid &= par;
+                                valid_hi &= (u32)par;
+                            }
+                            if (thirds) {
+                                valid &= ~(kThirds << ((3u - b0 % 3u) % 3u));
+                                valid_hi &= ~(u32)(kThirds << ((3u - (b0 + 64u) % 3u) % 3u));
+                            }
                             const u32 e = lo + S.excl - b0;  // the span's unscheduled point
-                            if (S.excl != ~0u && e < 64u) valid &= ~(1ull << e);
+                            if (S.excl != ~0u) {
+                                if (e < 64u) valid &= ~(1ull << e);
+                                else if (e < 96u) valid_hi &= ~(1u << (e - 64u));
+                            }
                         }
-                        if (valid) {
+                        if (valid | valid_hi) {
                             have = true;
                             live = valid;
+                            live_hi = valid_hi;
                             j = 1;
                             const u32 ib = b0 + B;
                             io = (int)ib - (int)d;
@@ -629,11 +654,9 @@ __device__ void filter_task(const FilterCtx<D> &c, u32 sp, u32 phi, u32 *qb, u64
                             sh = ib & 31u;
                             const u32 wo = ib >> 5;
 #pragma unroll
-                            for (int pb = 0; pb < NP; ++pb) {
-                                braw[pb][0] = plb[pb][wo];
-                                braw[pb][1] = plb[pb][wo + 1];
-                                braw[pb][2] = plb[pb][wo + 2];
-                            }
+                            for (int pb = 0; pb < NP; ++pb)
+#pragma unroll
+                                for (u32 k = 0; k < NW; ++k) braw[pb][k] = plb[pb][wo + k];
                             ++n_words;
                         }
                     }
@@ -648,7 +671,7 @@ __device__ void filter_task(const FilterCtx<D> &c, u32 sp, u32 phi, u32 *qb, u64
             bool fin = false;
             if (have) {
                 // every load of the step is issued before the first use
-                u32 raw[G][NP][3], shg[G];
+                u32 raw[G][NP][NW], shg[G];
 #pragma unroll
                 for (int g = 0; g < G; ++g) {
                     const u32 ig = (u32)max(io - g * (int)d, io_last);
@@ -656,45 +679,62 @@ __device__ void filter_task(const FilterCtx<D> &c, u32 sp, u32 phi, u32 *qb, u64
 #pragma unroll
                     for (int pb = 0; pb < NP; ++pb) {
                         const u32 *q = plb[pb] + (ig >> 5);
-                        raw[g][pb][0] = q[0];
-                        raw[g][pb][1] = q[1];
-                        raw[g][pb][2] = q[2];
+#pragma unroll
+                        for (u32 k = 0; k < NW; ++k) raw[g][pb][k] = q[k];
                     }
                 }
                 __builtin_amdgcn_sched_barrier(0);
                 u64 mism = 0;
+                u32 mism_hi = 0;
 #pragma unroll
                 for (int pb = 0; pb < NP; ++pb) {
                     const u64 base = window_of(braw[pb][0], braw[pb][1], braw[pb][2], sh);
 #pragma unroll
                     for (int g = 0; g < G; ++g)
                         mism |= base ^ window_of(raw[g][pb][0], raw[g][pb][1], raw[g][pb][2], shg[g]);
+                    const u32 base_hi = __builtin_amdgcn_alignbit(braw[pb][3], braw[pb][2], sh);
+#pragma unroll
+                    for (int g = 0; g < G; ++g)
+                        mism_hi |= base_hi ^ __builtin_amdgcn_alignbit(raw[g][pb][3], raw[g][pb][2], shg[g]);
                 }
-                n_windows += (u32)NP * min((u32)G, t - j);
+                n_windows += (u32)NP * min((u32)G, t - j) * (WB / 32u);
                 live &= ~mism;
+                live_hi &= ~mism_hi;
                 j += G;
                 io -= G * (int)d;
-                if (!live) {
+                if (!(live | live_hi)) {
                     have = false;
                 } else if (j >= t) {
                     have = false;
                     fin = true;
                 }
             }
-            const u64 finm = ballot64(fin);
-            if (finm) {
+            // a passing word takes one queue entry per half with survivors:
+            // anchors b0 .. b0+63 and b0+64 .. b0+95
+            const u64 fin_lo = ballot64(fin && live != 0);
+            const u64 fin_hi = ballot64(fin && live_hi != 0);
+            if (fin_lo | fin_hi) {
                 if (fin) {
-                    const u32 slot = qn + lanes_below(finm);
-                    if (slot < 64u) {
-                        qb[slot] = b0;
-                        qs[slot] = live;
+                    u32 slot = qn + lanes_below(fin_lo) + lanes_below(fin_hi);
+                    const u32 need = (live != 0 ? 1u : 0u) + (live_hi != 0 ? 1u : 0u);
+                    if (slot + need <= kQueue) {
+                        if (live) {
+                            qb[slot] = b0;
+                            qs[slot++] = live;
+                        }
+                        if (live_hi) {
+                            qb[slot] = b0 + 64u;
+                            qs[slot] = live_hi;
+                        }
                     } else {
+                        // a slot this word cannot fill is left empty
+                        if (slot < kQueue) qs[slot] = 0;
                         pend = true;
                     }
                 }
-                const u32 n = (u32)__builtin_popcountll(finm);
-                if (qn + n > 64u) stall = true;
-                qn = min(qn + n, 64u);
+                const u32 n = (u32)__builtin_popcountll(fin_lo) + (u32)__builtin_popcountll(fin_hi);
+                if (qn + n > kQueue) stall = true;
+                qn = min(qn + n, kQueue);
             }
         }
         // no word in flight: screen and complete the queue, then requeue the
@@ -702,16 +742,23 @@ __device__ void filter_task(const FilterCtx<D> &c, u32 sp, u32 phi, u32 *qb, u64
         wave_lds_fence();
         if (qn) drain_queue<D, NP>(c, plb, qn, qb, qs, lane, n_windows, n_done);
         wave_lds_fence();
-        const u64 pm = ballot64(pend);
+        const u64 pm_lo = ballot64(pend && live != 0);
+        const u64 pm_hi = ballot64(pend && live_hi != 0);
         qn = 0;
-        if (pm) {
+        if (pm_lo | pm_hi) {
             if (pend) {
-                const u32 slot = lanes_below(pm);
-                qb[slot] = b0;
-                qs[slot] = live;
+                u32 slot = lanes_below(pm_lo) + lanes_below(pm_hi);
+                if (live) {
+                    qb[slot] = b0;
+                    qs[slot++] = live;
+                }
+                if (live_hi) {
+                    qb[slot] = b0 + 64u;
+                    qs[slot] = live_hi;
+                }
                 pend = false;
             }
-            qn = (u32)__builtin_popcountll(pm);
+            qn = (u32)__builtin_popcountll(pm_lo) + (u32)__builtin_popcountll(pm_hi);
             wave_lds_fence();
             continue;
         }
@@ -746,12 +793,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void s
                                                  u32 *__restrict__ failed,
                                                  unsigned long long *__restrict__ stats,
                                                  u32 *__restrict__ queue, u32 kGroup) {
-    // per-lane work counts: 64-bit window comparisons, candidate words,
-    // completed anchors (summed per block, three atomics per block)
+    // per-lane work counts: window comparisons in 32-anchor plane dwords,
+    // candidate words, completed anchors (summed per block, three atomics
+    // per block)
     __shared__ unsigned long long s_stats[3];
     // per-wave queue of passing words: start b0 and surviving anchors
-    __shared__ u32 s_qb[kFilterWaves][64];
-    __shared__ u64 s_qs[kFilterWaves][64];
+    __shared__ u32 s_qb[kFilterWaves][kQueue];
+    __shared__ u64 s_qs[kFilterWaves][kQueue];
     if (threadIdx.x < 3) s_stats[threadIdx.x] = 0;
     __syncthreads();
     u64 n_windows = 0, n_words = 0, n_done = 0;
