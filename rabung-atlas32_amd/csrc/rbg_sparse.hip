@@ -802,7 +802,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void s
     __shared__ u64 s_qs[kFilterWaves][kQueue];
     if (threadIdx.x < 3) s_stats[threadIdx.x] = 0;
     __syncthreads();
-    u64 n_windows = 0, n_words = 0, n_done = 0;
+    // u32 per lane, added to the block's totals before they can overflow: a
+    // task adds less than 2^28 to n_windows and n_done, and n_words stays
+    // below n_windows (every word takes at least one step)
+    u32 n_windows = 0, n_words = 0, n_done = 0;
     u32 *qb = s_qb[threadIdx.x >> 6];
     u64 *qs = s_qs[threadIdx.x >> 6];
     // XCD-aware schedule: workgroups are dispatched round-robin over the 8
@@ -857,16 +860,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void s
         c.q = q;
         c.B = sg->B;
         c.bmin = ((t - 1u) * dd + 1u) / 2u;
-        u32 tw = 0, tn = 0, td = 0;  // this task's counts (u32 per lane)
         switch (fg->np) {
-        case 1: filter_task<D, 1>(c, sp, phi, qb, qs, tw, tn, td); break;
-        case 2: filter_task<D, 2>(c, sp, phi, qb, qs, tw, tn, td); break;
-        case 3: filter_task<D, 3>(c, sp, phi, qb, qs, tw, tn, td); break;
-        default: filter_task<D, 4>(c, sp, phi, qb, qs, tw, tn, td); break;
+        case 1: filter_task<D, 1>(c, sp, phi, qb, qs, n_windows, n_words, n_done); break;
+        case 2: filter_task<D, 2>(c, sp, phi, qb, qs, n_windows, n_words, n_done); break;
+        case 3: filter_task<D, 3>(c, sp, phi, qb, qs, n_windows, n_words, n_done); break;
+        default: filter_task<D, 4>(c, sp, phi, qb, qs, n_windows, n_words, n_done); break;
         }
-        n_windows += tw;
-        n_words += tn;
-        n_done += td;
+        if ((n_windows | n_done) >= 0x80000000u) {
+            atomicAdd(&s_stats[0], (unsigned long long)n_windows);
+            atomicAdd(&s_stats[1], (unsigned long long)n_words);
+            atomicAdd(&s_stats[2], (unsigned long long)n_done);
+            n_windows = n_words = n_done = 0;
+        }
     }
     atomicAdd(&s_stats[0], (unsigned long long)n_windows);
     atomicAdd(&s_stats[1], (unsigned long long)n_words);
