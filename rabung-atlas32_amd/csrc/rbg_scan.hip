@@ -170,7 +170,10 @@ __global__ __launch_bounds__(kWavesPerBlock * 64) void scan_tiles(
     __shared__ u32 s_first[kWavesPerBlock];
     __shared__ u32 s_len[kWavesPerBlock];
     __shared__ u32 s_tile, s_pi;
-    const u32 wave = threadIdx.x >> 6;
+    // wave-uniform: everything derived from it (segment bounds, chunk masks,
+    // the run state of every order) then lives in scalar registers and runs
+    // on the scalar unit beside the vector powers
+    const u32 wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const u32 seg_positions = chunks * 64u;
     u32 cur_pi = 0xFFFFFFFFu;
     for (u32 iter = 0;; ++iter) {

@@ -175,7 +175,8 @@ __global__ __launch_bounds__(256) void sp_planes(const SparsePrime *__restrict__
     u32 k = s.prof_begin;
     while (k + 1 < s.prof_begin + s.nprof && prof[k + 1].pblk <= blockIdx.x) ++k;
     const SparseProf f = prof[k];
-    const u32 lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    const u32 lane = threadIdx.x & 63u;
+    const u32 wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const u32 pairs = (f.wpp + 1) / 2;
     const u32 first = ((blockIdx.x - f.pblk) * 4u + wave) * kPlanePairsPerWave;
     for (u32 q = 0; q < kPlanePairsPerWave; ++q) {
@@ -806,8 +807,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void s
     // task adds less than 2^28 to n_windows and n_done, and n_words stays
     // below n_windows (every word takes at least one step)
     u32 n_windows = 0, n_words = 0, n_done = 0;
-    u32 *qb = s_qb[threadIdx.x >> 6];
-    u64 *qs = s_qs[threadIdx.x >> 6];
+    const u32 wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    u32 *qb = s_qb[wave];
+    u64 *qs = s_qs[wave];
     // XCD-aware schedule: workgroups are dispatched round-robin over the 8
     // XCDs, so block b runs on XCD b % 8.  XCD x owns the profiles
     // k = x, x+8, x+16, ... and walks them in groups of kGroup profiles (a
