@@ -25,6 +25,9 @@ HBM_PEAK_GBS = 8000.0
 # with the base window (one funnel shift, one xor, one or, and a share of the
 # address arithmetic)
 OPS_PER_WINDOW_DWORD = 4
+# scan_tiles' unit of work: one Montgomery product (a 32x32->64 multiply, a
+# low and a high 32-bit multiply, a subtract with borrow, a select, an add)
+OPS_PER_PRODUCT = 6
 
 
 def top_primes(count):
@@ -246,15 +249,15 @@ def lcm_mask(mask):
 
 
 def int32_ops(primes):
-    """Algorithmic INT32 work of the dense scan: nh * mm(e) * 8 per prime
-    (one Montgomery product = 4 multiplies + 4 add/compare/select)."""
+    """Algorithmic INT32 work of the dense scan: nh * mm(e) * OPS_PER_PRODUCT
+    per prime (mm(e) Montgomery products per position)."""
     import rabung_amd
     total = 0
     for p in primes:
         if p < 3:
             continue
         m = lcm_mask(rabung_amd.mask(p))
-        total += ((p - 1) // 2) * mm_products((p - 1) // m) * 8
+        total += ((p - 1) // 2) * mm_products((p - 1) // m) * OPS_PER_PRODUCT
     return total
 
 
@@ -448,9 +451,9 @@ def main():
                               "scan_ms_per_step": round(scan_s * 1000.0, 3),
                               "peak_basis": peak_basis,
                               # products/s against the measured product rate
-                              "montgomery_products_per_s": round(ops / 8 / scan_s, 0),
+                              "montgomery_products_per_s": round(ops / OPS_PER_PRODUCT / scan_s, 0),
                               "frac_of_measured_product_rate": round(
-                                  ops / 8 / scan_s / probe["montgomery_products_per_s"], 4),
+                                  ops / OPS_PER_PRODUCT / scan_s / probe["montgomery_products_per_s"], 4),
                               "probe": probe}}
 
     # dominant kernel of the batched path: the sparse anchor filter (or the

@@ -6,21 +6,22 @@
 
 namespace rbg {
 
-// Montgomery product with 32-bit halves (v_mul_lo_u32 / v_mul_hi_u32).
-// REDC(a b) = hi + mh + carry with carry = [lo != 0] (the low words of a b
-// and m p cancel) and the sum below 2p, which exceeds 2^32 for p > 2^31.
-// With s = mh + carry <= p and q = p - s, the reduced value is hi - q when
-// hi >= q and hi + s otherwise (no overflow either way): one compare and one
-// select, no carry flags, so independent products do not serialise on the
-// condition-code register (4 multiplies + 6 other VALU instructions).
+// Montgomery product with 32-bit halves, subtractive form: with
+// m = lo(a b) p^{-1} mod 2^32 the low words of a b and m p are equal, so
+// REDC(a b) = (a b - m p) / 2^32 = hi(a b) - hi(m p) exactly, a value in
+// (-p, p) for a b < p 2^32; a borrow adds p back.  One v_mad_u64_u32, one
+// v_mul_lo_u32, one v_mul_hi_u32, a subtract whose borrow lands in a lane
+// mask, a select and an add: 6 VALU instructions.  `ninv` is -p^{-1} mod
+// 2^32 (the kernels' descriptor constant); its negation is wave-uniform and
+// hoisted out of every loop.
 __device__ __forceinline__ u32 mont_mul(u32 a, u32 b, u32 p, u32 ninv) {
     const u64 t = (u64)a * b;  // one v_mad_u64_u32
     const u32 lo = (u32)t, hi = (u32)(t >> 32);
-    const u32 m = lo * ninv;
-    // mh + [lo != 0] = ceil(m p / 2^32): the low word of m p is -lo mod 2^32
-    const u32 s = (u32)(((u64)m * p + 0xFFFFFFFFull) >> 32);
-    const u32 q = p - s;
-    return hi >= q ? hi - q : hi + s;
+    const u32 m = lo * (0u - ninv);
+    const u32 mh = __umulhi(m, p);
+    u32 r;
+    const bool borrow = __builtin_sub_overflow(hi, mh, &r);
+    return r + (borrow ? p : 0u);
 }
 
 __device__ __forceinline__ u32 mont_pow(u32 base, u32 e, u32 p, u32 ninv, u32 one) {
