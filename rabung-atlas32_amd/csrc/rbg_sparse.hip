@@ -584,7 +584,23 @@ __device__ void filter_task(const FilterCtx<D> &c, u32 sp, u32 phi, u32 *qb, u64
     const u32 *plb[NP];
 #pragma unroll
     for (int pb = 0; pb < NP; ++pb) plb[pb] = c.pl + (u32)pb * c.wpp;
-    const bool even = (d & 1u) == 0, thirds = d % 3u == 0;
+    // Wheel: anchors b with 2 | b (d even) or 3 | b (3 | d) share a factor
+    // with d and are tested with a smaller denominator.  Every word start b0
+    // = m_first d + s_lo + span sp d + 96 w is = s_lo mod 2 when d is even
+    // and mod 3 when 3 | d, so the wheel is one mask per task (bit i of
+    // dword k stands for b0 + 32 k + i; 32 = 2 mod 3).
+    u32 wheel[3] = {~0u, ~0u, ~0u};
+    if ((d & 1u) == 0) {
+        const u32 par = (S.s_lo & 1u) ? 0x55555555u : 0xAAAAAAAAu;
+        wheel[0] = wheel[1] = wheel[2] = par;
+    }
+    if (d % 3u == 0) {
+        const u32 r = S.s_lo % 3u;
+#pragma unroll
+        for (u32 k = 0; k < 3; ++k) wheel[k] &= ~(0x49249249u << ((6u - r - 2u * k) % 3u));
+    }
+    // the span's unscheduled point lies in word excl / 96 of every span
+    const u32 ex_w = S.excl == ~0u ? ~0u : S.excl / WB, ex_b = S.excl % WB;
     u32 next = 0, qn = 0;
     bool pend = false;  // a passing word that did not fit the queue
     u32 b0 = 0;
@@ -629,19 +645,11 @@ __device__ void filter_task(const FilterCtx<D> &c, u32 sp, u32 phi, u32 *qb, u64
                                 valid = (~0ull << first) & (~0ull >> (63u - min(last, 63u)));
                             if (last >= 64u)
                                 valid_hi = (~0u << (max(first, 64u) - 64u)) & (~0u >> (95u - last));
-                            if (even) {
-                                const u64 par = (b0 & 1u) ? 0x5555555555555555ull : 0xAAAAAAAAAAAAAAAAull;
-                                valid &= par;
-                                valid_hi &= (u32)par;
-                            }
-                            if (thirds) {
-                                valid &= ~(kThirds << ((3u - b0 % 3u) % 3u));
-                                valid_hi &= ~(u32)(kThirds << ((3u - (b0 + 64u) % 3u) % 3u));
-                            }
-                            const u32 e = lo + S.excl - b0;  // the span's unscheduled point
-                            if (S.excl != ~0u) {
-                                if (e < 64u) valid &= ~(1ull << e);
-                                else if (e < 96u) valid_hi &= ~(1u << (e - 64u));
+                            valid &= ((u64)wheel[1] << 32) | wheel[0];
+                            valid_hi &= wheel[2];
+                            if (w == ex_w) {
+                                if (ex_b < 64u) valid &= ~(1ull << ex_b);
+                                else valid_hi &= ~(1u << (ex_b - 64u));
                             }
                         }
                         if (valid | valid_hi) {

@@ -88,6 +88,52 @@ def span_ranges(d, t, sp, phi, bmin, B):
     return out
 
 
+def span_words(d, t, sp, phi, bmin, B, word=96):
+    """The anchors the kernel marks valid when it cuts the spans into words
+    of `word` consecutive anchors from each span's unclipped start (filter
+    word hand-out in rbg_sparse.hip): the word's range clipped to the span
+    and to [bmin, B], the task's wheel mask built from s_lo alone (every word
+    start is = s_lo mod 2 when d is even and mod 3 when 3 | d), and the
+    excluded point cleared in word excl // word of every span."""
+    if d <= 2:
+        lo_b = max(bmin, t if d == 1 else t - 1)
+        s_lo, width, excl, starts = lo_b, (B - lo_b + 1 if lo_b <= B else 0), None, [lo_b]
+        if lo_b > B:
+            starts = []
+    else:
+        odd_t = t % 2 == 1
+        rm = (phi + sp - 1) % sp if odd_t else phi
+        s_lo = d // 2 + 1 if odd_t else 1
+        width = d + (d - 1) // 2 - d // 2 if odd_t else d - 1
+        excl = d - s_lo if odd_t else (d // 2 - 1 if d % 2 == 0 else None)
+        x = bmin + 1 - s_lo - width
+        m = 0 if x <= 0 else (x + d - 1) // d
+        m += (rm + sp - m % sp) % sp
+        starts = []
+        while m * d + s_lo <= B:
+            starts.append(m * d + s_lo)
+            m += sp
+    # task wheel: bit i of a word stands for b0 + i
+    wheel = (1 << word) - 1
+    if d % 2 == 0:
+        wheel &= int(("01" if s_lo % 2 else "10") * (word // 2), 2)  # keep odd b
+    if d % 3 == 0:
+        wheel &= sum(1 << i for i in range(word) if (s_lo + i) % 3 != 0)
+    got = set()
+    nwords = (width + word - 1) // word
+    for lo in starts:
+        for w in range(nwords):
+            b0 = lo + word * w
+            first, last = max(bmin, b0), min(lo + width - 1, B, b0 + word - 1)
+            if first > last:
+                continue
+            valid = sum(1 << (b - b0) for b in range(first, last + 1)) & wheel
+            if excl is not None and w == excl // word:
+                valid &= ~(1 << (excl % word))
+            got.update(b0 + i for i in range(word) if valid >> i & 1)
+    return got
+
+
 def anchors(ranges):
     got = set()
     for lo, hi, *ex in ranges:

@@ -66,3 +66,26 @@ def test_spans_hold_exactly_the_scheduled_anchors():
                 bands = sparse_model.anchors(sparse_model.band_ranges(d, t, sp, phi, bmin, B))
                 spans = sparse_model.anchors(sparse_model.span_ranges(d, t, sp, phi, bmin, B))
                 assert bands == spans, (p, q, d)
+
+
+def test_word_masks_keep_the_scheduled_anchors_coprime_to_the_wheel():
+    """The filter's words (96 anchors from each span's start) with the task's
+    wheel mask built from s_lo alone keep exactly the scheduled anchors that
+    are not excluded by gcd(b, d) in {2, 3}: the word starts are congruent to
+    s_lo mod 2 and mod 3 whenever the wheel applies."""
+    import random
+    rng = random.Random(5)
+    for p in (65537 + 2, 1000003, 4294967291):
+        B, D = sparse_model.cover(p)
+        ds = sorted(set(range(1, 25)) | {D, D - 1, D - 2} | {rng.randint(1, D) for _ in range(6)})
+        for q in (3, 4, 17, 24, 25, 48):
+            t = min(q, max(3, (q + 1) // 2))
+            sp = q - t + 1
+            phi = ((t - 1) // 2) % sp
+            for d in ds:
+                bmin = ((t - 1) * d + 1) // 2
+                want = {b for b in sparse_model.anchors(
+                            sparse_model.span_ranges(d, t, sp, phi, bmin, B))
+                        if not (d % 2 == 0 and b % 2 == 0) and not (d % 3 == 0 and b % 3 == 0)}
+                got = sparse_model.span_words(d, t, sp, phi, bmin, B)
+                assert got == want, (p, q, d)
