@@ -362,7 +362,6 @@ __device__ __forceinline__ u64 window64(const u32 *pl, u32 i) {
     return ((u64)hi << 32) | lo;
 }
 
-constexpr u64 kThirds = 0x9249249249249249ull;  // bits i = 0 mod 3
 constexpr u32 kXcds = 8;  // XCDs of an MI355X
 
 // Bilateral screen (every placement of a q-window containing the anchor;
