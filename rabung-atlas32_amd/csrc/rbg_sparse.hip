@@ -776,7 +776,7 @@ __device__ void filter_task(const FilterCtx<D> &c, u32 sp, u32 phi, u32 *qb, u64
 
 constexpr u32 kFilterWaves = 4;  // waves per filter block (blockDim 256)
 
-constexpr u32 kChunk = 16;  // filter tasks claimed per queue access
+constexpr u32 kChunk = 32;  // filter tasks claimed per queue access
 
 // Next task of the wave: the rest of its chunk, then the chunk claimed
 // during the chunk's last task (`claim` is valid in lane 0).
@@ -827,9 +827,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void s
     // queue, zeroed before the launch) in chunks of kChunk consecutive
     // tasks, so the tasks in flight on an XCD stay within a short window
     // however many waves are resident or queued; the next chunk is claimed
-    // while the last task of the current one runs.
-    const u32 xcd = blockIdx.x % kXcds;
+    // while the last task of the current one runs.  A wave whose XCD has no
+    // unclaimed task left takes chunks from the other XCDs' queues in turn
+    // (their planes then come through its own L2), so no XCD idles while
+    // another still has work.
     const u32 lane = threadIdx.x & 63u;
+    for (u32 v = 0; v < kXcds; ++v) {
+    const u32 xcd = (blockIdx.x + v) % kXcds;
     const u32 nprof_x = nprof > xcd ? (nprof - xcd + kXcds - 1) / kXcds : 0u;
     const u32 ngroups = (nprof_x + kGroup - 1) / kGroup;
     const u32 items = ngroups * kGroup * dmax;  // checked below 2^32 by the host
@@ -881,6 +885,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void s
             atomicAdd(&s_stats[2], (unsigned long long)n_done);
             n_windows = n_words = n_done = 0;
         }
+    }
     }
     atomicAdd(&s_stats[0], (unsigned long long)n_windows);
     atomicAdd(&s_stats[1], (unsigned long long)n_words);
