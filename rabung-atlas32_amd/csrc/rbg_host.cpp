@@ -32,6 +32,7 @@ using rbg::RunSum;
 using rbg::SparsePrime;
 using rbg::SparseProf;
 using rbg::TileHdr;
+using rbg::kPlanePairsPerBlock;
 using rbg::u16;
 using rbg::u32;
 using rbg::u64;
@@ -620,7 +621,7 @@ int sparse_plan(rbg_ctx *c, const u64 *primes, const std::vector<u32> &idx_in,
                 f.sigma0 = (((p - 1) / 2) % r) == 0;
                 f.pblk = (u32)pblk;
                 words += (u64)f.np * f.wpp;
-                pblk += ((f.wpp + 1ull) / 2ull + 63ull) / 64ull;
+                pblk += ((f.wpp + 1ull) / 2ull + kPlanePairsPerBlock - 1ull) / kPlanePairsPerBlock;
                 c->h_prof.push_back(f);
                 ++s.nprof;
                 if (!wide) ++c->nprof32;
@@ -651,7 +652,7 @@ int sparse_run(rbg_ctx *c, rbg_record *recs, u32 *err, std::vector<u32> &failed_
     const u32 segs_per_prof = rbg_seed_segments((u32)bmax);
     for (const SparseProf &f : c->h_prof) {
         words += (u64)f.np * f.wpp;
-        pblk += ((f.wpp + 1ull) / 2ull + 63ull) / 64ull;
+        pblk += ((f.wpp + 1ull) / 2ull + kPlanePairsPerBlock - 1ull) / kPlanePairsPerBlock;
     }
     const u64 bytes = np * sizeof(SparsePrime) + nf * sizeof(SparseProf) + c->h_sval.size() * 4 +
                       c->h_sval64.size() * 8 + c->h_slab.size() * 2 + lab * 2 + inv * 4 +

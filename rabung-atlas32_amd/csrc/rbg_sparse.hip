@@ -160,10 +160,12 @@ __global__ __launch_bounds__(256) void sp_inverses(const SparsePrime *__restrict
     }
 }
 
-constexpr u32 kPlanePairsPerWave = 16;
+constexpr u32 kPlanePairsPerWave = kPlanePairsPerBlock / 4;  // 64: one per lane
 
 // Colour bit planes of one order over the signed table: bit i of plane k is
-// bit k of cl(i - B).  One wave builds 64 bits (two words) per step.
+// bit k of cl(i - B).  One wave builds 64 bits (two words) of every plane per
+// step; lane q keeps the words of step q, and after 64 steps the lanes store
+// them together (one 512-byte store per plane word).
 __global__ __launch_bounds__(256) void sp_planes(const SparsePrime *__restrict__ P, u32 nprimes,
                                                  const SparseProf *__restrict__ prof, u32 nprof,
                                                  const u16 *__restrict__ labels,
@@ -179,23 +181,31 @@ __global__ __launch_bounds__(256) void sp_planes(const SparsePrime *__restrict__
     const u32 wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const u32 pairs = (f.wpp + 1) / 2;
     const u32 first = ((blockIdx.x - f.pblk) * 4u + wave) * kPlanePairsPerWave;
-    for (u32 q = 0; q < kPlanePairsPerWave; ++q) {
-        const u32 pair = first + q;
-        if (pair >= pairs) break;
-        const int64_t y = (int64_t)pair * 64 + lane - (int64_t)s.B;
-        u32 c = 0;
-        if (y > 0 && y <= (int64_t)s.B) {
-            c = labels[(u64)s.lab_off + (u64)y] % f.r;
-        } else if (y < 0 && -y <= (int64_t)s.B) {
-            c = ((u32)labels[(u64)s.lab_off + (u64)(-y)] + s.jneg) % s.m % f.r;
-        }
-        for (u32 b = 0; b < f.np; ++b) {
+    u64 mine[4] = {0, 0, 0, 0};  // this lane's step: words of planes 0..np-1
+    const u32 nq = first < pairs ? min(kPlanePairsPerWave, pairs - first) : 0u;
+    const u16 *lab = labels + s.lab_off;
+    // branch-free body (every lane loads an in-range label, position 1 for
+    // the positions outside the table), so the unrolled steps keep several
+    // label loads in flight
+#pragma unroll 8
+    for (u32 q = 0; q < nq; ++q) {
+        const int64_t y = (int64_t)(first + q) * 64 + lane - (int64_t)s.B;
+        const u32 a = (u32)(y < 0 ? -y : y);
+        const bool in = y != 0 && a <= s.B;
+        const u32 j = lab[in ? a : 1u];
+        const u32 c = !in ? 0u : y > 0 ? j % f.r : (j + s.jneg) % s.m % f.r;
+#pragma unroll
+        for (u32 b = 0; b < 4; ++b) {
             const u64 bits = ballot64((c >> b) & 1u);
-            if (lane == 0) {
-                u32 *pl = planes + f.plane_off + (u64)b * f.wpp;
-                pl[2 * pair] = (u32)bits;
-                if (2 * pair + 1 < f.wpp) pl[2 * pair + 1] = (u32)(bits >> 32);
-            }
+            if (lane == q) mine[b] = bits;
+        }
+    }
+    const u32 pair = first + lane;
+    if (pair < pairs) {
+        for (u32 b = 0; b < f.np; ++b) {
+            u32 *pl = planes + f.plane_off + (u64)b * f.wpp;
+            pl[2 * pair] = (u32)mine[b];
+            if (2 * pair + 1 < f.wpp) pl[2 * pair + 1] = (u32)(mine[b] >> 32);
         }
     }
 }

@@ -18,6 +18,10 @@ struct SparsePrime {
     u32 hi, tab64;
 };
 
+// Colour-plane construction: a block of 4 waves builds 64-bit steps
+// (two plane words) for this many consecutive steps of one profile.
+constexpr u32 kPlanePairsPerBlock = 256;
+
 // Per (prime, admissible order).
 struct SparseProf {
     u32 prime;      // index into the SparsePrime array
