@@ -257,6 +257,10 @@ class Engine:
             pass
 
     def _profile(self, name, primes, method, certify, graph):
+        out = self._profile_raw(name, primes, method, certify, graph)
+        return [record_to_dict(out[i]) for i in range(len(primes))]
+
+    def _profile_raw(self, name, primes, method, certify, graph):
         primes = [int(p) for p in primes]
         n = len(primes)
         flags = method | (CERTIFY if certify else 0) | (GRAPH if graph else 0)
@@ -272,7 +276,7 @@ class Engine:
             rc = getattr(self._lib, name)(self._ctx, arr, n, flags, out)
         if rc:
             raise RabungError(rc, self._lib.rbg_last_error(self._ctx).decode())
-        return [record_to_dict(out[i]) for i in range(n)]
+        return out
 
     def profile_block(self, primes, method=METHOD_BATCHED, certify=False, graph=False):
         """Profiles of up to 4096 primes below 2^33; returns a list of dicts."""
@@ -281,6 +285,12 @@ class Engine:
     def profile_batch(self, primes, method=METHOD_BATCHED, certify=False, graph=False):
         """Profiles of up to 2^20 primes (many blocks), split by HBM budget."""
         return self._profile("rbg_profile_batch", primes, method, certify, graph)
+
+    def profile_batch_raw(self, primes, method=METHOD_BATCHED, certify=False, graph=False):
+        """profile_batch returning the C record array itself (ctypes Record,
+        or RecordWide when a prime exceeds 2^32), without building one dict
+        per prime: the form for callers that hand the records on to C."""
+        return self._profile_raw("rbg_profile_batch", primes, method, certify, graph)
 
     def autotune(self, primes, method=METHOD_BATCHED):
         """Pick the fastest launch shape on these sample primes; returns it."""
