@@ -204,6 +204,8 @@ def test_batch_across_blocks(engine):
     t = engine.last_timing()
     assert t["sparse_primes"] == len(flat)
     assert got == [r for b in blocks for r in engine.profile_block(b)]
+    raw = engine.profile_batch_raw(flat)
+    assert [rabung_amd.record_to_dict(raw[i]) for i in range(len(flat))] == got
 
 
 @pytest.mark.parametrize("method", [rabung_amd.METHOD_BATCHED, rabung_amd.METHOD_PLAIN,
