@@ -15,10 +15,10 @@
 //     there are real runs and give the first threshold;
 //   * filter: one wave per (order, denominator d) task.  The scheduled anchors
 //     b (b = n d + c, n = phi or t-2-phi mod s by the side of c, folded to
-//     b >= (t-1)d/2) form bands of consecutive b; each lane tests 64
-//     consecutive b of a band at once: for every plane, the 64-bit windows at
+//     b >= (t-1)d/2) form spans of consecutive b; each lane tests 96
+//     consecutive b of a span at once: for every plane, the 96-bit windows at
 //     numerators b - j d (j = 1..t-1) are compared with the window at b, and a
-//     lane stops as soon as no candidate is left;
+//     lane takes a new word as soon as no candidate is left;
 //   * completion: a surviving anchor is mapped to x = -b / d mod p and
 //     extended left and right through the table (numerators stay in [-B, B]
 //     for the short runs that occur) and, past the table, by direct powers;
@@ -548,7 +548,8 @@ __device__ __forceinline__ void drain_queue(const FilterCtx<D> &c, const u32 *co
         }
     }
 }
-// 64-bit window from the three words at its first bit (shift sh = bit & 31).
+// 64-bit window from the three words at its first bit (shift sh = bit & 31);
+// the filter's 96-bit windows are this plus one more funnel shift.
 __device__ __forceinline__ u64 window_of(u32 a, u32 b, u32 c, u32 sh) {
     return ((u64)__builtin_amdgcn_alignbit(c, b, sh) << 32) | __builtin_amdgcn_alignbit(b, a, sh);
 }
