@@ -185,7 +185,7 @@ u64 root_of_order(u64 p, u32 m) {
 // Sorted Montgomery images z^j R mod p (R = 2^32) with their label j (and
 // colour code); V = u32 below 2^32, u64 for the wider primes.
 template <class V>
-bool root_table(u64 p, u32 m, std::vector<V> &vals, std::vector<u16> &codes,
+bool root_table(u64 p, u32 m, std::vector<V> &vals, std::vector<u16> *codes,
                 std::vector<u16> *labs) {
     const u64 z = root_of_order(p, m);
     if (!z) return false;
@@ -198,7 +198,7 @@ bool root_table(u64 p, u32 m, std::vector<V> &vals, std::vector<u16> &codes,
     std::sort(tab.begin(), tab.end());
     for (auto &t : tab) {
         vals.push_back((V)t.first);
-        codes.push_back((u16)rbg::colour_code(t.second, m));
+        if (codes) codes->push_back((u16)rbg::colour_code(t.second, m));
         if (labs) labs->push_back((u16)t.second);
     }
     return true;
@@ -303,7 +303,7 @@ struct rbg_ctx {
     std::vector<SparseProf> h_prof;
     std::vector<u32> h_sval;
     std::vector<u64> h_sval64;
-    std::vector<u16> h_scode, h_slab;
+    std::vector<u16> h_slab;
     u32 nprof32 = 0;  // profiles of primes below 2^32 (they come first)
     // timing of the last call
     rbg_timing timing{};
@@ -483,8 +483,8 @@ int dense_run(rbg_ctx *c, const u64 *primes, const std::vector<u32> &idx_in, boo
         runs += (u64)d.ntiles * d.nr;
         c->timing.positions += d.nh;
         c->timing.products += (u64)d.nh * rbg::montpow_products(d.e);
-        const bool ok = wide ? root_table(p, d.m, c->h_val64, c->h_code, nullptr)
-                             : root_table(p, d.m, c->h_val, c->h_code, nullptr);
+        const bool ok = wide ? root_table(p, d.m, c->h_val64, &c->h_code, nullptr)
+                             : root_table(p, d.m, c->h_val, &c->h_code, nullptr);
         if (!ok) return c->fail(RBG_EINVAL, "no root of the required order");
         c->h_descs.push_back(d);
     }
@@ -560,7 +560,10 @@ int dense_run(rbg_ctx *c, const u64 *primes, const std::vector<u32> &idx_in, boo
 // ---- sparse engine -------------------------------------------------------
 
 // Plans the sparse primes; indices that the sparse engine cannot take are
-// appended to `rest`.
+// appended to `rest`.  Per prime the certified cover, the Montgomery
+// constants and the sorted root table are built on up to 16 host threads
+// (contiguous ranges of primes), then one ordered pass assigns every offset
+// and lays out the profiles.
 int sparse_plan(rbg_ctx *c, const u64 *primes, const std::vector<u32> &idx_in,
                 std::vector<u32> &rest) {
     const std::vector<u32> idx = by_domain(primes, idx_in);
@@ -568,33 +571,74 @@ int sparse_plan(rbg_ctx *c, const u64 *primes, const std::vector<u32> &idx_in,
     c->h_prof.clear();
     c->h_sval.clear();
     c->h_sval64.clear();
-    c->h_scode.clear();
     c->h_slab.clear();
     c->nprof32 = 0;
+    const u32 n = (u32)idx.size();
+    struct Item {
+        SparsePrime s;
+        u32 part, val_at, lab_at;  // root table: thread part and its offsets
+        bool sparse, ok;
+    };
+    struct Part {
+        std::vector<u32> val;
+        std::vector<u64> val64;
+        std::vector<u16> lab;
+    };
+    std::vector<Item> items(n);
+    const u32 nt = n < 128 ? 1u : std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    std::vector<Part> parts(nt);
+    auto build = [&](u32 t) {
+        Part &pt = parts[t];
+        for (u32 k = (u32)((u64)n * t / nt); k < (u32)((u64)n * (t + 1) / nt); ++k) {
+            Item &it = items[k];
+            const u64 p = primes[idx[k]];
+            const bool wide = p >= (1ull << 32);
+            u32 B = 0, D = 0;
+            it.sparse = p >= kSparseMin && choose_cover(p, kCoverQ, B, D);
+            it.ok = true;
+            if (!it.sparse) continue;
+            const MontConsts mc = mont_consts(p);
+            SparsePrime &sp = it.s;
+            sp = SparsePrime{};
+            sp.p = (u32)p;
+            sp.mask = mask_of(p);
+            sp.m = lcm_of(sp.mask);
+            sp.e = (u32)((p - 1) / sp.m);
+            sp.ninv = mc.ninv;
+            sp.one = mc.one;
+            sp.r2 = mc.r2;
+            sp.hi = mc.hi & 7u;
+            sp.index = idx[k];
+            sp.B = B;
+            sp.D = D;
+            sp.Q = kCoverQ;
+            sp.jneg = (sp.e & 1u) ? sp.m / 2 : 0;  // (-1)^e = z^(m/2) when e is odd
+            it.part = t;
+            it.val_at = (u32)(wide ? pt.val64.size() : pt.val.size());
+            it.lab_at = (u32)pt.lab.size();
+            it.ok = wide ? root_table(p, sp.m, pt.val64, nullptr, &pt.lab)
+                         : root_table(p, sp.m, pt.val, nullptr, &pt.lab);
+        }
+    };
+    if (nt == 1) {
+        build(0);
+    } else {
+        std::vector<std::thread> th;
+        for (u32 t = 0; t < nt; ++t) th.emplace_back(build, t);
+        for (auto &t : th) t.join();
+    }
     u64 lab = 0, inv = 0, inv64 = 0, words = 0, lblk = 0, iblk = 0, pblk = 0;
-    for (u32 i : idx) {
-        const u64 p = primes[i];
-        const bool wide = p >= (1ull << 32);
-        u32 B = 0, D = 0;
-        if (p < kSparseMin || !choose_cover(p, kCoverQ, B, D)) {
-            rest.push_back(i);
+    for (u32 k = 0; k < n; ++k) {
+        const Item &it = items[k];
+        if (!it.sparse) {
+            rest.push_back(idx[k]);
             continue;
         }
-        const MontConsts k = mont_consts(p);
-        SparsePrime s{};
-        s.p = (u32)p;
-        s.mask = mask_of(p);
-        s.m = lcm_of(s.mask);
-        s.e = (u32)((p - 1) / s.m);
-        s.ninv = k.ninv;
-        s.one = k.one;
-        s.r2 = k.r2;
-        s.hi = k.hi & 7u;
-        s.index = i;
-        s.B = B;
-        s.D = D;
-        s.Q = kCoverQ;
-        s.jneg = (s.e & 1u) ? s.m / 2 : 0;  // (-1)^e = z^(m/2) when e is odd
+        if (!it.ok) return c->fail(RBG_EINVAL, "no root of the required order");
+        const u64 p = primes[idx[k]];
+        const bool wide = p >= (1ull << 32);
+        SparsePrime s = it.s;
+        const u32 B = s.B, D = s.D;
         s.tab_off = (u32)c->h_slab.size();
         s.tab64 = (u32)c->h_sval64.size();
         s.lab_off = (u32)lab;
@@ -607,9 +651,15 @@ int sparse_plan(rbg_ctx *c, const u64 *primes, const std::vector<u32> &idx_in,
         (wide ? inv64 : inv) += D + 1ull;
         lblk += (B + 2047ull) / 2048ull;
         iblk += (D + 1ull + 255ull) / 256ull;
-        const bool ok = wide ? root_table(p, s.m, c->h_sval64, c->h_scode, &c->h_slab)
-                             : root_table(p, s.m, c->h_sval, c->h_scode, &c->h_slab);
-        if (!ok) return c->fail(RBG_EINVAL, "no root of the required order");
+        const Part &pt = parts[it.part];
+        c->h_slab.insert(c->h_slab.end(), pt.lab.begin() + it.lab_at,
+                         pt.lab.begin() + it.lab_at + s.m);
+        if (wide)
+            c->h_sval64.insert(c->h_sval64.end(), pt.val64.begin() + it.val_at,
+                               pt.val64.begin() + it.val_at + s.m);
+        else
+            c->h_sval.insert(c->h_sval.end(), pt.val.begin() + it.val_at,
+                             pt.val.begin() + it.val_at + s.m);
         for (u32 r = 2; r <= 9; ++r)
             if (s.mask & (1u << (r - 2))) {
                 SparseProf f{};
