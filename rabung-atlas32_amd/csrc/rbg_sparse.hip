@@ -381,7 +381,8 @@ constexpr u32 kXcds = 8;  // XCDs of an MI355X
 // numerator space (negated, so equality is unchanged) position x-k reads
 // b + k d and position x+t-1+k reads b - (t-1+k) d; numerators outside
 // [-B, B] are unknown and never reject.  R is kept as a bit-sliced count
-// (5 planes, h <= 23) so that every candidate of the word is screened at once.
+// (5 planes, h <= 31; h <= 24 for q <= 48, t = ceil(q/2)) so that every
+// candidate of the word is screened at once.
 template <int NP>
 __device__ __forceinline__ u64 match_left(const u32 *const *plb, const u64 *base, u32 b0, u32 B,
                                           u32 d, u32 k, u32 &n_windows) {
