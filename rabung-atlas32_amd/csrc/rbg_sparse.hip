@@ -162,6 +162,16 @@ __global__ __launch_bounds__(256) void sp_inverses(const SparsePrime *__restrict
 
 constexpr u32 kPlanePairsPerWave = kPlanePairsPerBlock / 4;  // 64: one per lane
 
+// x mod r for x < 2^16 and 2 <= r < 2^12 (labels, label + label of -1, and
+// their colours) by one high multiply: with g = floor((2^32-1)/r) + 1 the
+// quotient floor(x g / 2^32) is exact because x g / 2^32 exceeds x / r by
+// less than x / 2^32 < 1 / r.
+struct SmallMod {
+    u32 r, g;
+    __device__ __forceinline__ explicit SmallMod(u32 r_) : r(r_), g(0xFFFFFFFFu / r_ + 1u) {}
+    __device__ __forceinline__ u32 operator()(u32 x) const { return x - r * __umulhi(x, g); }
+};
+
 // Colour bit planes of one order over the signed table: bit i of plane k is
 // bit k of cl(i - B).  One wave builds 64 bits (two words) of every plane per
 // step; lane q keeps the words of step q, and after 64 steps the lanes store
@@ -182,22 +192,33 @@ __global__ __launch_bounds__(256) void sp_planes(const SparsePrime *__restrict__
     const u32 pairs = (f.wpp + 1) / 2;
     const u32 first = ((blockIdx.x - f.pblk) * 4u + wave) * kPlanePairsPerWave;
     u64 mine[4] = {0, 0, 0, 0};  // this lane's step: words of planes 0..np-1
+    const SmallMod mod_r(f.r), mod_m(s.m);
     const u32 nq = first < pairs ? min(kPlanePairsPerWave, pairs - first) : 0u;
     const u16 *lab = labels + s.lab_off;
-    // branch-free body (every lane loads an in-range label, position 1 for
-    // the positions outside the table), so the unrolled steps keep several
-    // label loads in flight
-#pragma unroll 8
-    for (u32 q = 0; q < nq; ++q) {
-        const int64_t y = (int64_t)(first + q) * 64 + lane - (int64_t)s.B;
-        const u32 a = (u32)(y < 0 ? -y : y);
-        const bool in = y != 0 && a <= s.B;
-        const u32 j = lab[in ? a : 1u];
-        const u32 c = !in ? 0u : y > 0 ? j % f.r : (j + s.jneg) % s.m % f.r;
+    // eight steps at a time: the eight label loads first (every lane loads an
+    // in-range label, position 1 for the positions outside the table), then
+    // the colours and ballots of the steps that exist
+    for (u32 q0 = 0; q0 < nq; q0 += 8u) {
+        u32 jv[8];
+        int64_t yv[8];
 #pragma unroll
-        for (u32 b = 0; b < 4; ++b) {
-            const u64 bits = ballot64((c >> b) & 1u);
-            if (lane == q) mine[b] = bits;
+        for (u32 u = 0; u < 8u; ++u) {
+            yv[u] = (int64_t)(first + q0 + u) * 64 + lane - (int64_t)s.B;
+            const u32 a = (u32)(yv[u] < 0 ? -yv[u] : yv[u]);
+            jv[u] = lab[(yv[u] != 0 && a <= s.B) ? a : 1u];
+        }
+#pragma unroll
+        for (u32 u = 0; u < 8u; ++u) {
+            if (q0 + u >= nq) break;
+            const int64_t y = yv[u];
+            const u32 a = (u32)(y < 0 ? -y : y);
+            const bool in = y != 0 && a <= s.B;
+            const u32 c = !in ? 0u : y > 0 ? mod_r(jv[u]) : mod_r(mod_m(jv[u] + s.jneg));
+#pragma unroll
+            for (u32 b = 0; b < 4; ++b) {
+                const u64 bits = ballot64((c >> b) & 1u);
+                if (lane == q0 + u) mine[b] = bits;
+            }
         }
     }
     const u32 pair = first + lane;
@@ -224,6 +245,7 @@ __global__ __launch_bounds__(256) void sp_seed_scan(const SparsePrime *__restric
     if (k >= nprof) return;
     const SparseProf f = prof[k];
     const SparsePrime s = P[f.prime];
+    const SmallMod mod_r(f.r);
     const u32 lane = threadIdx.x & 63u;
     const u32 first = 1u + sg * kSeedSeg;
     const u32 len = first > s.B ? 0u : min(kSeedSeg, s.B - first + 1u);
@@ -236,7 +258,7 @@ __global__ __launch_bounds__(256) void sp_seed_scan(const SparsePrime *__restric
         const u32 valid = min(64u, len - base);
         const u64 vmask = valid == 64u ? ~0ull : ((1ull << valid) - 1ull);
         const u32 y = first + base + lane;
-        const u32 col = y <= s.B ? labels[(u64)s.lab_off + y] % f.r : 0u;
+        const u32 col = y <= s.B ? mod_r(labels[(u64)s.lab_off + y]) : 0u;
         const u32 left = shift_up_one(col, carry);
         carry = __builtin_amdgcn_readlane(col, 63);
         if (c == 0) fcol = __builtin_amdgcn_readfirstlane(col);
