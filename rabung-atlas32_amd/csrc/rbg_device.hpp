@@ -11,26 +11,33 @@ namespace rbg {
 // REDC(a b) = (a b - m p) / 2^32 = hi(a b) - hi(m p) exactly, a value in
 // (-p, p) for a b < p 2^32; a borrow adds p back.  One v_mad_u64_u32, one
 // v_mul_lo_u32, one v_mul_hi_u32, a subtract whose borrow lands in a lane
-// mask, a select and an add: 6 VALU instructions.  `ninv` is -p^{-1} mod
-// 2^32 (the kernels' descriptor constant); its negation is wave-uniform and
-// hoisted out of every loop.
-__device__ __forceinline__ u32 mont_mul(u32 a, u32 b, u32 p, u32 ninv) {
+// mask, a select and an add: 6 VALU instructions.  `pinv` is p^{-1} mod
+// 2^32 (pos_inverse of the descriptors' -p^{-1}).
+__device__ __forceinline__ u32 mont_mul(u32 a, u32 b, u32 p, u32 pinv) {
     const u64 t = (u64)a * b;  // one v_mad_u64_u32
     const u32 lo = (u32)t, hi = (u32)(t >> 32);
-    const u32 m = lo * (0u - ninv);
+    const u32 m = lo * pinv;
     const u32 mh = __umulhi(m, p);
     u32 r;
     const bool borrow = __builtin_sub_overflow(hi, mh, &r);
     return r + (borrow ? p : 0u);
 }
 
-__device__ __forceinline__ u32 mont_pow(u32 base, u32 e, u32 p, u32 ninv, u32 one) {
+// p^{-1} mod 2^32 from the descriptors' -p^{-1}, for a wave-uniform p.  The
+// negation goes through readfirstlane so that the compiler cannot see it:
+// otherwise it rewrites lo * (0 - ninv) as 0 - lo * ninv, one more
+// instruction in every product.
+__device__ __forceinline__ u32 pos_inverse(u32 ninv) {
+    return __builtin_amdgcn_readfirstlane(0u - ninv);
+}
+
+__device__ __forceinline__ u32 mont_pow(u32 base, u32 e, u32 p, u32 pinv, u32 one) {
     if (e == 0) return one;
     u32 y = base;
     const int top = 31 - __builtin_clz(e);
     for (int bit = top - 1; bit >= 0; --bit) {
-        y = mont_mul(y, y, p, ninv);
-        if ((e >> bit) & 1u) y = mont_mul(y, base, p, ninv);
+        y = mont_mul(y, y, p, pinv);
+        if ((e >> bit) & 1u) y = mont_mul(y, base, p, pinv);
     }
     return y;
 }
@@ -90,11 +97,11 @@ __device__ __forceinline__ u64 mont_pow33(u64 base, u64 e, u64 p, u32 ninv, u64 
 struct Dom32 {
     typedef u32 V;
     static constexpr bool wide = false;
-    u32 p, ninv;
-    __device__ __forceinline__ V mul(V a, V b) const { return mont_mul(a, b, p, ninv); }
+    u32 p, pinv;  // pinv = p^{-1} mod 2^32 (pos_inverse)
+    __device__ __forceinline__ V mul(V a, V b) const { return mont_mul(a, b, p, pinv); }
     __device__ __forceinline__ V add(V a, V b) const { return add_mod(a, b, p); }
     __device__ __forceinline__ V pow(V base, u64 e, V one) const {
-        return mont_pow(base, (u32)e, p, ninv, one);
+        return mont_pow(base, (u32)e, p, pinv, one);
     }
 };
 

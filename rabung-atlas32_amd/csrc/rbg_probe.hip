@@ -34,7 +34,7 @@ __global__ __launch_bounds__(256) void valu_rate(u32 seed, u32 iters, u32 p, u32
             if (OP == 1) v[c] = v[c] * (v[(c + 1) % kChains] | 1u);
             if (OP == 2) v[c] = __umulhi(v[c], v[(c + 1) % kChains] | 0x80000001u);
             if (OP == 3) v[c] = __builtin_amdgcn_alignbit(v[c], v[(c + 1) % kChains], v[c]);
-            if (OP == 4) v[c] = mont_mul(v[c], v[c], p, ninv);
+            if (OP == 4) v[c] = mont_mul(v[c], v[c], p, pos_inverse(ninv));
             if (OP == 5) v[c] = v[c] + v[(c + 1) % kChains];
             if (OP == 6) v[c] = v[c] * v[(c + 1) % kChains];
         }

@@ -43,7 +43,7 @@ struct PrimeConsts {
 };
 
 __device__ __forceinline__ PrimeConsts<Dom32> consts_of(const PrimeDesc &d, Dom32) {
-    return {{d.p, d.ninv}, d.one, d.r2, d.step};
+    return {{d.p, pos_inverse(d.ninv)}, d.one, d.r2, d.step};
 }
 
 __device__ __forceinline__ PrimeConsts<Dom33> consts_of(const PrimeDesc &d, Dom33) {

@@ -58,7 +58,7 @@ struct SpConsts {
 };
 
 __device__ __forceinline__ SpConsts<Dom32> sp_consts(const SparsePrime &s, Dom32) {
-    return {{s.p, s.ninv}, s.one, s.r2};
+    return {{s.p, pos_inverse(s.ninv)}, s.one, s.r2};
 }
 
 __device__ __forceinline__ SpConsts<Dom33> sp_consts(const SparsePrime &s, Dom33) {
@@ -155,8 +155,9 @@ __global__ __launch_bounds__(256) void sp_inverses(const SparsePrime *__restrict
         const SpConsts<Dom33> k = sp_consts(s, Dom33{});
         inv64[(u64)s.inv_off + d] = k.dom.pow(k.dom.mul(d, k.r2), k.dom.p - 2u, k.one);
     } else {
-        const u32 dm = mont_mul(d, s.r2, s.p, s.ninv);
-        inv[(u64)s.inv_off + d] = mont_pow(dm, s.p - 2, s.p, s.ninv, s.one);
+        const u32 pinv = pos_inverse(s.ninv);
+        const u32 dm = mont_mul(d, s.r2, s.p, pinv);
+        inv[(u64)s.inv_off + d] = mont_pow(dm, s.p - 2, s.p, pinv, s.one);
     }
 }
 
