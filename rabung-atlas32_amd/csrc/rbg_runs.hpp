@@ -69,9 +69,11 @@ __device__ __forceinline__ void interior_best(u64 N, u32 f, u32 h, u32 base, u32
     take(l + 1, base + c - 1, best, bstart);
 }
 
+// prefix == 0 while the run that began at relative position 0 is still
+// open: a closed head run has length >= 1, so no separate flag is kept (one
+// scalar register less per order).
 struct OrderState {
     u32 prefix, cur, best, bstart;
-    u32 popen;  // the run that began at relative position 0 is still open
 };
 
 __device__ __forceinline__ void order_init(OrderState &s) {
@@ -79,7 +81,6 @@ __device__ __forceinline__ void order_init(OrderState &s) {
     s.cur = 0;
     s.best = 0;
     s.bstart = 0;
-    s.popen = 1;
 }
 
 // Advance the state over one chunk at relative offset `base` with change mask
@@ -89,20 +90,15 @@ __device__ __forceinline__ void order_update(OrderState &s, u64 C, u64 N, u32 ba
     const u32 f = (u32)__builtin_ctzll(C);
     const u32 closed = base + f - s.cur;
     if (closed) {
-        if (s.popen) {
-            s.prefix = closed;
-            s.popen = 0;
-        }
+        if (s.prefix == 0) s.prefix = closed;
         take(closed, s.cur, s.best, s.bstart);
     }
     s.cur = base + f;
     const u64 rest = C & (C - 1ull);
     if (rest) {
         const u32 h = 63u - (u32)__builtin_clzll(rest);
-        if (s.popen) {  // the head run closes at the second change (first chunk)
+        if (s.prefix == 0)  // the head run closes at the second change (first chunk)
             s.prefix = base + (u32)__builtin_ctzll(rest) - s.cur;
-            s.popen = 0;
-        }
         interior_best(N, f, h, base, s.best, s.bstart);
         s.cur = base + h;
     }
@@ -112,7 +108,7 @@ __device__ __forceinline__ void order_update(OrderState &s, u64 C, u64 N, u32 ba
 // `best`; a head run never closed makes the whole stretch the prefix.
 __device__ __forceinline__ u32 order_finish(OrderState &s, u32 len) {
     const u32 suffix = len - s.cur;
-    if (s.popen) s.prefix = len;
+    if (s.prefix == 0) s.prefix = len;
     if (len) take(suffix, s.cur, s.best, s.bstart);
     return len ? suffix : 0u;
 }
