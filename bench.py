@@ -373,8 +373,9 @@ def main():
     if args.microbench:
         return microbench()
     if args.gpus != 1:
-        print("bench.py: this build measures one GPU only (the path shards across primes with "
-              "no exchange; see DESIGN.md); run with --gpus 1", file=sys.stderr)
+        print("bench.py: this build measures one GPU only (the path would shard across primes, "
+              "each GPU profiling its own blocks with no exchange); run with --gpus 1",
+              file=sys.stderr)
         return 2
     if args.workload == "batch64":
         return bench_batch64(args)
