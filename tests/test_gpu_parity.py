@@ -173,6 +173,27 @@ def test_sparse_block49900_large_subset(engine):
         assert recs[j]["A4"] == a4[i]
 
 
+def test_top256_headline_block(engine):
+    """The benchmark's workload, the 256 largest primes below 2^32 (871
+    profiles), in one batched call: every profile and A4 equal the committed
+    sample block's, and the autotuned launch shapes give the same records."""
+    from rabung_amd import workloads
+    rows, a4, _ = golden_data.sample_block(49900)
+    index = {r["p"]: (r, v) for r, v in zip(rows, a4)}
+    primes = workloads.top(256)
+    assert all(p in index for p in primes)
+    recs = engine.profile_block(primes)
+    t = engine.last_timing()
+    assert t["sparse_primes"] == 256 and t["sparse_profiles"] == 871 and t["fallbacks"] == 0
+    for rec, row in zip(recs, as_rows(recs)):
+        want, want_a4 = index[row["p"]]
+        assert row == want, row["p"]
+        assert rec["A4"] == want_a4
+    with rabung_amd.Engine() as eng:
+        eng.autotune(primes[:32])
+        assert eng.profile_block(primes) == recs
+
+
 def test_sparse_all_orders_and_small_domain(engine):
     """Primes = 1 mod 2520 (every order admissible) across the sparse domain."""
     primes = [p for p in range(2520 * 27 + 1, 2520 * 3000, 2520 * 7) if oracle.is_prime(p)][:24]
