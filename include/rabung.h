@@ -88,6 +88,15 @@ typedef struct rbg_config {
     uint32_t blocks_per_cu;  /* persistent blocks (4 waves) per CU; 0 = 8 */
     uint64_t hbm_budget;     /* bytes of device workspace allowed; 0 = 16 GiB */
     uint32_t xcd_group;      /* sparse filter: profiles one XCD works on at a time; 0 = 1 */
+    uint32_t cover_q;        /* sparse engine: threshold cap Q of the certified cover,
+                                3..63; 0 = 48 (the reference's cover, frozen_entry.cu:
+                                132-148, takes 3 <= Q <= 256; the bit-sliced screen here
+                                counts up to q - t = 31) */
+    uint32_t extend_limit;   /* sparse engine: most steps one passing anchor's run may be
+                                extended by before the prime is refused and recomputed by
+                                the dense scan; 0 = 2^22.  A small value forces the
+                                refusal path (the reference's queue-overflow refusal,
+                                gpu_pipeline.cuh:335,342,363) */
     uint32_t reserved;
 } rbg_config;
 
@@ -188,12 +197,16 @@ int rbg_a4_host(const uint32_t *primes, uint32_t n, uint32_t *out, uint32_t thre
  * 1000 <= p < 2^33, src/census/theorie/elliptic4_mestre.cpp:75). */
 int rbg_a4_host_wide(const uint64_t *primes, uint32_t n, uint32_t *out, uint32_t threads);
 
-/* A4 of one prime with the evidence of its point count, as JSON in the
- * caller's buffer (same length convention as the frame writers): curve
- * order, Hasse window, lcm of the point orders found on the curve and its
- * twist, twist parameter, points drawn (*attempts).  Serves the audit
- * certificate of the moment worker's protocol, moment_worker.cpp:43-45
- * (fields after elliptic4_mestre.cpp:68-73; the point list is not kept). */
+/* A4 of one prime and the number of points its count drew (*attempts).  With
+ * out_len non-NULL also its audit certificate as JSON in the caller's buffer
+ * (same length convention as the frame writers): the field names and meaning
+ * of the reference's certificate (src/census/theorie/elliptic4_mestre.cpp:
+ * 68-73, read by src/census/theorie/certify_elliptic.py:13-47) -- point count
+ * N, Hasse window, the curve y^2 = x (x - s)(x - 9 s) (twist_parameter s,
+ * 1 for the curve itself) and its certificate_points (x, y, exact order and
+ * the order's prime factors) whose orders' lcm has exactly one multiple in
+ * the Hasse window.  p < 1000 gives the small_dense certificate.  Serves the
+ * audit entry of the moment worker's protocol, moment_worker.cpp:43-45. */
 int rbg_a4_certificate(uint32_t p, uint32_t *a4, uint64_t *attempts, char *out, size_t *out_len);
 
 /* Measurement probes for the roofline (no reference counterpart).  kind 0-6:
@@ -215,6 +228,19 @@ int rbg_microbench(int device, int kind, uint32_t param, double *out);
  * lemma, paper/GPU_METHOD_EN.tex:1001-1025). */
 int rbg_check_wide(int device, const uint64_t *a, const uint64_t *b, uint32_t n, uint64_t p,
                    int kind, uint64_t *out);
+
+/* Direct verification of n records on the host (no device needed): the
+ * admissibility mask, K from M and L, and for every admissible order the
+ * witness run (monochromatic over W..W+M-1 with differing neighbours) and
+ * the prefix (x = 1..L in the identity class, L+1 not) by direct powers.
+ * Replaces the reference's certify_witness and prefix check
+ * (src/census/engine32/vendor/research/baseline/baseline/src/advance.cpp:
+ * 7-13, src/census/worker32.cu:37-38), the check that RBG_CERTIFY runs on
+ * every result.  Returns RBG_OK, RBG_ECERTIFY with the index of the first
+ * failing record in *first_bad (n when all pass), or RBG_EINVAL for a
+ * record whose p is not prime.  threads = 0: one per host core. */
+int rbg_certify_records(const rbg_record *recs, uint32_t n, uint32_t threads,
+                        uint32_t *first_bad);
 
 /* Admissibility mask of p (bit r-2 iff r | p-1); 0 for p < 3. */
 uint32_t rbg_mask(uint32_t p);
@@ -262,14 +288,20 @@ int rbg_prime_hash(const rbg_record *recs, uint32_t n, uint8_t out[32]);
 
 /* ---- reference bridge (baseline_bridge.cu:7,21-27) ---------------------- */
 
-/* Prepares a process-wide context sized for primes up to `cap`. */
+/* Prepares the process-wide context of the bridge.  `cap` (the largest
+ * prime the reference sized its buffers for) is advisory here: workspaces
+ * grow on demand. */
 void baseline_init(unsigned cap);
 
 /* M by order into out[r] and prefixes into prefix[r] (arrays of 10, indexed
- * by r, entries r in mask written).  `full` nonzero requests every
- * admissible order of p; zero requests exactly `mask`, which must be a subset
- * of p's admissible orders.  Returns 0, or 2 for an unsupported mask, or
- * another nonzero status on error. */
+ * by r); only the orders r whose bit r-2 is set in `mask` are written, as in
+ * the reference.  `full` nonzero computes every admissible order of p (an
+ * order of `mask` that does not divide p-1 then gets 0); zero requires
+ * `mask` to be a nonempty subset of p's admissible orders (the reference
+ * serves five fixed masks there, baseline_bridge.cu:24).  Returns 0; 2 for a
+ * mask that cannot be served; -1 for any other failure (not a prime below
+ * 2^32, no device, a device error), where the reference ends the process.
+ * Serialised by a lock (the reference is not thread-safe). */
 int baseline_profile(unsigned p, unsigned mask, unsigned *out, unsigned *prefix, int full);
 
 #ifdef __cplusplus

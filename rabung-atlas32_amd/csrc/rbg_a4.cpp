@@ -18,6 +18,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <string>
 #include <thread>
 #include <unordered_map>
 #include <vector>
@@ -165,15 +166,36 @@ u64 gcd64(u64 a, u64 b) {
     return a;
 }
 
-// Evidence of one point count: the Hasse window [lower, upper], the lcm of the
-// orders of the random points found on the curve and on its quadratic twist
-// (parameter `twist`, a non-residue), and the number of points drawn.
-struct CountInfo {
-    u64 lower = 0, upper = 0, lcm_curve = 1, lcm_twist = 1, attempts = 0, twist = 0;
+// A point drawn on one curve and its exact order.
+struct Witness {
+    u64 x, y, order;
 };
 
-// #E(F_p) for E: y^2 = x(x-1)(x-9); returns 0 on failure.
-u64 curve_order(u64 p, CountInfo *info = nullptr) {
+// Evidence of one point count: the Hasse window [lower, upper], the lcm of the
+// orders of the random points found on the curve and on its quadratic twist
+// (parameter `twist`, a non-residue), and the number of points drawn.  In
+// certificate mode also the curve whose own lcm fixes the count (`cert_s`:
+// 1 for E, the twist parameter for E') and that curve's points.
+struct CountInfo {
+    u64 lower = 0, upper = 0, lcm_curve = 1, lcm_twist = 1, attempts = 0, twist = 0;
+    u64 cert_s = 0, cert_lcm = 0;
+    std::vector<Witness> cert_points;
+};
+
+// The one multiple of l in [L, H], or 0 when there are none or several.
+u64 unique_multiple(u64 l, u64 L, u64 H) {
+    const u64 first = (L + l - 1) / l * l;
+    return first <= H && first + l > H ? first : 0;
+}
+
+// #E(F_p) for E: y^2 = x(x-1)(x-9); returns 0 on failure.  Points are drawn
+// alternately on E and E'.  The count is settled as soon as the two lcms
+// leave one N in the Hasse window with lcm(E) | N and lcm(E') | 2p + 2 - N.
+// With `certify` the draw goes on until one curve's lcm alone has a single
+// multiple in the window (Mestre: for p > 229 one of the two curves has such
+// points), so that the points of that curve certify N by themselves; that
+// N must agree with the other curve's lcm too.
+u64 curve_order(u64 p, CountInfo *info = nullptr, bool certify = false) {
     const u64 R = isqrt64(4 * p), L = p + 1 - R, H = p + 1 + R, W = H - L;
     u64 m = isqrt64(W + 1);
     if (m * m < W + 1) ++m;
@@ -183,6 +205,7 @@ u64 curve_order(u64 p, CountInfo *info = nullptr) {
     while (F.chi(s) != -1) ++s;
     Curve curves[2] = {Curve(p, 1), Curve(p, s)};
     u64 order_lcm[2] = {1, 1};
+    std::vector<Witness> points[2];
     for (int attempt = 0; attempt < 400; ++attempt) {
         const int which = attempt & 1;
         const Curve &E = curves[which];
@@ -219,9 +242,28 @@ u64 curve_order(u64 p, CountInfo *info = nullptr) {
         for (u64 q : prime_factors(n))
             while (ord % q == 0 && E.mul(P, ord / q).inf) ord /= q;
         order_lcm[which] = order_lcm[which] / gcd64(order_lcm[which], ord) * ord;
+        if (certify) points[which].push_back({P.x, P.y, ord});
+        const u64 l0 = order_lcm[0], l1 = order_lcm[1];
+        if (certify) {
+            const u64 own = unique_multiple(order_lcm[which], L, H);
+            if (!own) continue;
+            const u64 N = which ? 2 * p + 2 - own : own;
+            if (N % l0 || (2 * p + 2 - N) % l1) return 0;  // the curves disagree
+            if (info) {
+                info->lower = L;
+                info->upper = H;
+                info->lcm_curve = l0;
+                info->lcm_twist = l1;
+                info->attempts = (u64)attempt + 1;
+                info->twist = s;
+                info->cert_s = which ? s : 1;
+                info->cert_lcm = order_lcm[which];
+                info->cert_points = points[which];
+            }
+            return N;
+        }
         // candidates N in [L, H]: lcm0 | N and lcm1 | (2p + 2 - N)
         u64 found = 0, count = 0;
-        const u64 l0 = order_lcm[0], l1 = order_lcm[1];
         for (u64 N = (L + l0 - 1) / l0 * l0; N <= H && count < 2; N += l0)
             if ((2 * p + 2 - N) % l1 == 0) {
                 found = N;
@@ -253,10 +295,11 @@ u32 dense_a4(u32 p) {
 }
 
 // 0 on success; a4 = UINT32_MAX on failure.
-int a4_one(u64 p, u32 *a4, u64 *points = nullptr, CountInfo *info = nullptr) {
+int a4_one(u64 p, u32 *a4, u64 *points = nullptr, CountInfo *info = nullptr,
+           bool certify = false) {
     if (p == 2) { *a4 = 0; return 0; }
     if (p < 1000) { *a4 = dense_a4((u32)p); return 0; }
-    const u64 N = curve_order(p, info);
+    const u64 N = curve_order(p, info, certify);
     if (points) *points = N;
     if (!N) return 1;
     Field F{p};
@@ -307,33 +350,51 @@ extern "C" int rbg_a4_host_wide(const uint64_t *primes, uint32_t n, uint32_t *ou
 
 extern "C" int rbg_a4_certificate(uint32_t p, uint32_t *a4, uint64_t *attempts, char *out,
                                   size_t *out_len) {
-    if (!a4 || !out_len || !rbg_is_prime(p)) return RBG_EINVAL;
+    if (!a4 || !rbg_is_prime(p)) return RBG_EINVAL;
     u64 N = 0;
     CountInfo info;
-    if (a4_one(p, a4, &N, &info)) return RBG_ECERTIFY;
+    const bool certify = out_len != nullptr;
+    if (a4_one(p, a4, &N, &info, certify)) return RBG_ECERTIFY;
     if (attempts) *attempts = info.attempts;
+    if (!certify) return RBG_OK;
+    std::string text;
     char buf[512];
-    int len;
-    if (p < 1000)
-        len = std::snprintf(buf, sizeof buf,
-                            "{\"status\":\"PASS\",\"p\":%u,\"r\":2,\"quadruples\":%u,"
-                            "\"certificate\":\"small_dense\"}",
-                            p, *a4);
-    else
-        len = std::snprintf(
-            buf, sizeof buf,
-            "{\"status\":\"PASS\",\"p\":%u,\"r\":2,\"quadruples\":%u,\"elliptic_points\":%llu,"
-            "\"curve\":\"y^2=x(x-1)(x-9)\",\"hasse_lower\":%llu,\"hasse_upper\":%llu,"
-            "\"order_lcm\":%llu,\"twist_order_lcm\":%llu,\"twist_parameter\":%llu,"
-            "\"attempts\":%llu,\"certificate\":\"unique_lcm_multiple_in_Hasse\"}",
-            p, *a4, (unsigned long long)N, (unsigned long long)info.lower,
-            (unsigned long long)info.upper, (unsigned long long)info.lcm_curve,
-            (unsigned long long)info.lcm_twist, (unsigned long long)info.twist,
-            (unsigned long long)info.attempts);
-    if (len < 0 || (size_t)len >= sizeof buf) return RBG_EINVAL;
+    if (p < 1000) {
+        std::snprintf(buf, sizeof buf,
+                      "{\"status\":\"PASS\",\"p\":%u,\"r\":2,\"quadruples\":%u,"
+                      "\"certificate\":\"small_dense\"}",
+                      p, *a4);
+        text = buf;
+    } else {
+        // field names and meaning of the reference's certificate
+        // (elliptic4_mestre.cpp:68-73): the points of the curve y^2 =
+        // x (x - s)(x - 9 s) whose orders' lcm has one multiple in the
+        // Hasse window, each with its order and the order's prime factors
+        std::snprintf(buf, sizeof buf,
+                      "{\"status\":\"PASS\",\"p\":%u,\"r\":2,\"quadruples\":%u,"
+                      "\"elliptic_points\":%llu,\"order_lcm\":%llu,\"hasse_lower\":%llu,"
+                      "\"hasse_upper\":%llu,\"attempts\":%llu,\"twist_parameter\":%llu,"
+                      "\"certificate_points\":[",
+                      p, *a4, (unsigned long long)N, (unsigned long long)info.cert_lcm,
+                      (unsigned long long)info.lower, (unsigned long long)info.upper,
+                      (unsigned long long)info.attempts, (unsigned long long)info.cert_s);
+        text = buf;
+        for (size_t i = 0; i < info.cert_points.size(); ++i) {
+            const Witness &w = info.cert_points[i];
+            std::snprintf(buf, sizeof buf, "%s{\"x\":%llu,\"y\":%llu,\"order\":%llu,\"prime_factors\":[",
+                          i ? "," : "", (unsigned long long)w.x, (unsigned long long)w.y,
+                          (unsigned long long)w.order);
+            text += buf;
+            const std::vector<u64> f = prime_factors(w.order);
+            for (size_t k = 0; k < f.size(); ++k)
+                text += (k ? "," : "") + std::to_string(f[k]);
+            text += "]}";
+        }
+        text += "],\"certificate\":\"unique_lcm_multiple_in_Hasse\"}";
+    }
     const size_t cap = *out_len;
-    *out_len = (size_t)len;
-    if (!out || cap < (size_t)len) return RBG_ECAPACITY;
-    std::memcpy(out, buf, (size_t)len);
+    *out_len = text.size();
+    if (!out || cap < text.size()) return RBG_ECAPACITY;
+    std::memcpy(out, text.data(), text.size());
     return RBG_OK;
 }

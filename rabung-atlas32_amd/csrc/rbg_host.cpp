@@ -66,7 +66,7 @@ extern "C" hipError_t rbg_sparse_filter(const SparsePrime *P, const SparseProf *
                                         const uint16_t *tab_lab, unsigned long long *best,
                                         uint32_t *failed, unsigned long long *stats,
                                         uint32_t *queue, int wide, uint32_t grid, uint32_t group,
-                                        hipStream_t stream);
+                                        uint32_t extend_limit, hipStream_t stream);
 extern "C" hipError_t rbg_sparse_finalize(const SparsePrime *P, uint32_t nprimes,
                                           const SparseProf *prof, const unsigned long long *best,
                                           const uint32_t *a4, rbg_record *out, hipStream_t stream);
@@ -267,7 +267,9 @@ struct DevBuf {
 };
 
 constexpr u32 kSparseMin = 1u << 16;  // smallest prime sent to the sparse engine
-constexpr u32 kCoverQ = 48;           // threshold cap of the cover
+constexpr u32 kCoverQ = 48;           // default threshold cap of the cover
+constexpr u32 kMaxCoverQ = 63;        // the screen counts up to q - t = 31 (t = ceil(q/2))
+constexpr u32 kExtendLimit = 1u << 22;  // default extension steps of one passing anchor
 constexpr u32 kBatchChunk = 16384;    // primes per launch chain of a batch
 // filter statistics (3 x u64) and the filter's per-XCD task counters (8 x u32
 // per launch: primes below 2^32, wider primes)
@@ -371,6 +373,12 @@ u32 xcd_group(const rbg_ctx *c) { return c->cfg.xcd_group ? c->cfg.xcd_group : 1
 
 u64 budget(const rbg_ctx *c) { return c->cfg.hbm_budget ? c->cfg.hbm_budget : (16ull << 30); }
 
+u32 cover_q(const rbg_ctx *c) { return c->cfg.cover_q ? c->cfg.cover_q : kCoverQ; }
+
+u32 extend_limit(const rbg_ctx *c) {
+    return c->cfg.extend_limit ? c->cfg.extend_limit : kExtendLimit;
+}
+
 double elapsed(rbg_ctx *c, int a, int b) {
     float ms = 0;
     if (hipEventElapsedTime(&ms, c->ev[a], c->ev[b]) != hipSuccess) return 0;
@@ -402,30 +410,54 @@ bool certify(u64 p, u32 r, u32 M, u32 L, u32 W) {
     return certify_with(p, M, L, W, [&](u64 x) { return powmod_w(x, e, p); });
 }
 
-int certify_all(rbg_ctx *c, const rbg_record *out, const u64 *primes, u32 n) {
-    unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
-    std::vector<int> bad(nt, -1);
+// Index of the first record that fails its direct verification (the
+// admissibility mask, K from M and L, and for every admissible order the
+// witness run, its neighbours and the prefix by direct powers), or -1.
+// primes[i] is the full prime of record i (records keep its low word).
+long long first_uncertified(const rbg_record *recs, const u64 *primes, u32 n, unsigned threads) {
+    const unsigned nt = std::max(1u, std::min(threads, n));
+    std::vector<long long> bad(nt, -1);
+    auto check = [&](u32 i) {
+        const rbg_record &rec = recs[i];
+        const u64 p = primes[i];
+        if (rec.mask != mask_of(p)) return false;
+        for (u32 r = 2; r <= 9; ++r) {
+            const u32 k = r - 2;
+            if (!(rec.mask & (1u << k))) continue;
+            const u64 bound = ((p - 1) / 2) % r == 0 ? 2ull * rec.L[k] + 2 : (u64)rec.L[k] + 2;
+            if (rec.K[k] != std::max<u64>((u64)rec.M[k] + 1, bound)) return false;
+            if (rec.L[k] > rec.M[k] || rec.M[k] > (p - 1) / r || rec.W[k] > (p - 1) / 2)
+                return false;
+            if (!certify(p, r, rec.M[k], rec.L[k], rec.W[k])) return false;
+        }
+        return true;
+    };
     std::vector<std::thread> th;
     for (unsigned t = 0; t < nt; ++t)
         th.emplace_back([&, t] {
-            for (u32 i = t; i < n; i += nt) {
-                const rbg_record &rec = out[i];
-                for (u32 r = 2; r <= 9; ++r)
-                    if (rec.mask & (1u << (r - 2)))
-                        if (!certify(primes[i], r, rec.M[r - 2], rec.L[r - 2], rec.W[r - 2])) {
-                            bad[t] = (int)i;
-                            return;
-                        }
-            }
+            for (u32 i = t; i < n; i += nt)
+                if (!check(i)) {
+                    bad[t] = i;
+                    return;
+                }
         });
     for (auto &t : th) t.join();
-    for (int b : bad)
-        if (b >= 0) {
-            char msg[128];
-            snprintf(msg, sizeof msg, "direct verification failed for p=%llu",
-                     (unsigned long long)primes[b]);
-            return c->fail(RBG_ECERTIFY, msg);
-        }
+    long long first = -1;
+    for (long long b : bad)
+        if (b >= 0 && (first < 0 || b < first)) first = b;
+    return first;
+}
+
+unsigned host_threads() { return std::max(1u, std::thread::hardware_concurrency()); }
+
+int certify_all(rbg_ctx *c, const rbg_record *out, const u64 *primes, u32 n) {
+    const long long b = first_uncertified(out, primes, n, std::min(16u, host_threads()));
+    if (b >= 0) {
+        char msg[128];
+        snprintf(msg, sizeof msg, "direct verification failed for p=%llu",
+                 (unsigned long long)primes[b]);
+        return c->fail(RBG_ECERTIFY, msg);
+    }
     return RBG_OK;
 }
 
@@ -594,7 +626,7 @@ int sparse_plan(rbg_ctx *c, const u64 *primes, const std::vector<u32> &idx_in,
             const u64 p = primes[idx[k]];
             const bool wide = p >= (1ull << 32);
             u32 B = 0, D = 0;
-            it.sparse = p >= kSparseMin && choose_cover(p, kCoverQ, B, D);
+            it.sparse = p >= kSparseMin && choose_cover(p, cover_q(c), B, D);
             it.ok = true;
             if (!it.sparse) continue;
             const MontConsts mc = mont_consts(p);
@@ -611,7 +643,7 @@ int sparse_plan(rbg_ctx *c, const u64 *primes, const std::vector<u32> &idx_in,
             sp.index = idx[k];
             sp.B = B;
             sp.D = D;
-            sp.Q = kCoverQ;
+            sp.Q = cover_q(c);
             sp.jneg = (sp.e & 1u) ? sp.m / 2 : 0;  // (-1)^e = z^(m/2) when e is odd
             it.part = t;
             it.val_at = (u32)(wide ? pt.val64.size() : pt.val.size());
@@ -788,7 +820,7 @@ int sparse_run(rbg_ctx *c, rbg_record *recs, u32 *err, std::vector<u32> &failed_
                                        (const u16 *)c->sp_lab.ptr, best + f0, failed,
                                        (unsigned long long *)c->sp_stats.ptr,
                                        (u32 *)((char *)c->sp_stats.ptr + kQueueOffset) + 8 * wide,
-                                       wide, grid, xcd_group(c), s));
+                                       wide, grid, xcd_group(c), extend_limit(c), s));
         }
         return RBG_OK;
     });
@@ -819,14 +851,14 @@ int sparse_run(rbg_ctx *c, rbg_record *recs, u32 *err, std::vector<u32> &failed_
 
 // Device workspace one prime needs under `flags` (the planning rules of
 // sparse_plan and dense_run).
-u64 prime_bytes(u64 p, u32 flags, u32 tile_pos) {
+u64 prime_bytes(u64 p, u32 flags, u32 tile_pos, u32 q) {
     if (p < 3) return sizeof(rbg_record);
     const u32 mask = mask_of(p), m = lcm_of(mask), nr = (u32)__builtin_popcount(mask);
     const u64 vbytes = p >= (1ull << 32) ? 8 : 4;  // root-table and inverse values
     u64 bytes = sizeof(rbg_record) + (u64)m * (vbytes + 4) + 64;
     u32 B = 0, D = 0;
     if ((flags & RBG_METHOD_MASK) == RBG_METHOD_BATCHED && p >= kSparseMin &&
-        choose_cover(p, kCoverQ, B, D)) {
+        choose_cover(p, q, B, D)) {
         bytes += 2ull * (B + 1) + vbytes * (D + 1) + 88;
         for (u32 r = 2; r <= 9; ++r)
             if (mask & (1u << (r - 2)))
@@ -848,6 +880,8 @@ int run_block(rbg_ctx *c, const u64 *h_primes, u32 n, u32 flags, rbg_record *d_o
     c->graph = (flags & RBG_GRAPH) != 0;
     const u32 method = flags & RBG_METHOD_MASK;
     if (method > RBG_METHOD_DENSE) return c->fail(RBG_EINVAL, "unknown method");
+    if (cover_q(c) < 3 || cover_q(c) > kMaxCoverQ)
+        return c->fail(RBG_EINVAL, "cover_q outside 3..63");
     std::vector<u32> odd, dense, sparse, failed;
     for (u32 i = 0; i < n; ++i) {
         const u64 p = h_primes[i];
@@ -917,7 +951,7 @@ int run_batch(rbg_ctx *c, const u64 *primes, u32 n, u32 flags, rbg_record *out, 
         u32 last = first;
         u64 bytes = 0;
         while (last < n && last - first < kBatchChunk) {
-            const u64 b = prime_bytes(primes[last], flags, tile_pos);
+            const u64 b = prime_bytes(primes[last], flags, tile_pos, cover_q(c));
             if (last > first && bytes + b > cap) break;
             bytes += b;
             ++last;
@@ -1035,7 +1069,7 @@ int rbg_estimate_bytes_wide(const rbg_ctx *c, const uint64_t *primes, uint32_t n
     if (!c || (!primes && n) || !bytes) return RBG_EINVAL;
     const u32 tile_pos = (u32)rbg_waves_per_block() * tile_chunks(c) * 64u;
     u64 total = 0;
-    for (u32 i = 0; i < n; ++i) total += prime_bytes(primes[i], flags, tile_pos);
+    for (u32 i = 0; i < n; ++i) total += prime_bytes(primes[i], flags, tile_pos, cover_q(c));
     *bytes = total;
     return RBG_OK;
 }
@@ -1045,7 +1079,7 @@ int rbg_estimate_bytes(const rbg_ctx *c, const uint32_t *primes, uint32_t n, uin
     if (!c || (!primes && n) || !bytes) return RBG_EINVAL;
     const u32 tile_pos = (u32)rbg_waves_per_block() * tile_chunks(c) * 64u;
     u64 total = 0;
-    for (u32 i = 0; i < n; ++i) total += prime_bytes(primes[i], flags, tile_pos);
+    for (u32 i = 0; i < n; ++i) total += prime_bytes(primes[i], flags, tile_pos, cover_q(c));
     *bytes = total;
     return RBG_OK;
 }
@@ -1162,6 +1196,20 @@ void rbg_close(rbg_ctx *c) {
     delete c;
 }
 
+int rbg_certify_records(const rbg_record *recs, uint32_t n, uint32_t threads,
+                        uint32_t *first_bad) {
+    if (!recs && n) return RBG_EINVAL;
+    std::vector<u64> primes(n);
+    for (u32 i = 0; i < n; ++i) {
+        primes[i] = recs[i].p;
+        if (!prime_u32(recs[i].p)) return RBG_EINVAL;
+    }
+    const long long b = first_uncertified(recs, primes.data(), n,
+                                          threads ? threads : host_threads());
+    if (first_bad) *first_bad = b < 0 ? n : (uint32_t)b;
+    return b < 0 ? RBG_OK : RBG_ECERTIFY;
+}
+
 uint32_t rbg_mask(uint32_t p) { return mask_of(p); }
 
 uint32_t rbg_primes_from(uint64_t start, uint32_t count, uint32_t *out) {
@@ -1194,25 +1242,31 @@ static std::mutex g_bridge_mutex;
 static rbg_ctx *g_bridge = nullptr;
 
 void baseline_init(unsigned cap) {
-    (void)cap;  // workspaces grow on demand
+    (void)cap;  // advisory: the context's workspaces grow on demand
     std::lock_guard<std::mutex> lock(g_bridge_mutex);
     if (!g_bridge) g_bridge = rbg_open(nullptr);
 }
 
+// Return codes: 0; 2 for a mask the call cannot serve (without `full`: empty,
+// or naming an order that does not divide p - 1); -1 for any other failure
+// (not a prime below 2^32, no device, a device error), which the reference
+// would have ended the process for.
 int baseline_profile(unsigned p, unsigned mask, unsigned *out, unsigned *prefix, int full) {
     std::lock_guard<std::mutex> lock(g_bridge_mutex);
-    if (!g_bridge) g_bridge = rbg_open(nullptr);
-    if (!g_bridge) return RBG_EDEVICE;
     const u32 adm = mask_of(p);
-    if (full) mask = adm;
-    if (mask & ~adm) return 2;
+    if (!full && (mask == 0 || (mask & ~adm))) return 2;
+    if (!out || !prefix) return -1;
+    if (!g_bridge) g_bridge = rbg_open(nullptr);
+    if (!g_bridge) return -1;
     rbg_record rec;
-    int rc = rbg_profile_block(g_bridge, &p, 1, RBG_METHOD_PLAIN, &rec);
-    if (rc) return rc;
+    if (rbg_profile_block(g_bridge, &p, 1, RBG_METHOD_PLAIN, &rec) != RBG_OK) return -1;
+    // only the caller's orders are written; with `full` an order of the mask
+    // that does not divide p - 1 has no profile and gets 0
     for (u32 r = 2; r <= 9; ++r)
         if (mask & (1u << (r - 2))) {
-            out[r] = rec.M[r - 2];
-            prefix[r] = rec.L[r - 2];
+            const bool have = (adm >> (r - 2)) & 1u;
+            out[r] = have ? rec.M[r - 2] : 0u;
+            prefix[r] = have ? rec.L[r - 2] : 0u;
         }
     return 0;
 }

@@ -320,16 +320,16 @@ __device__ u32 colour_direct(const SparsePrime &s, const SpConsts<D> &k, u32 r,
     return tab_lab[s.tab_off + lo] % r;
 }
 
-constexpr u32 kExtendLimit = 1u << 22;
-
 // Extend the monochromatic window [x, x+t-1] (numerators -b + i d) to its run.
 // Rare (a few per task), so kept out of line: the filter loop then holds only
-// the two record pointers, not the records.
+// the two record pointers, not the records.  A run that needs more than
+// `limit` extension steps refuses the prime (failed bit 2): the host then
+// recomputes it with the dense scan.
 template <class D>
 __device__ void complete_anchor(
     const SparsePrime *sg, const SparseProf *fg, const u32 *pl, u32 b, u32 d, typename D::V dinv,
     u32 t, const typename D::V *tab_val, const u16 *tab_lab, unsigned long long *best_slot,
-    u32 *failed) {
+    u32 *failed, u32 limit) {
     typedef typename D::V V;
     const SparsePrime s = *sg;
     const SparseProf f = *fg;
@@ -355,7 +355,7 @@ __device__ void complete_anchor(
             if (abs0 == 0xFFFFFFFFu) abs0 = colour_direct<D>(s, k, f.r, tab_val, tab_lab, x);
             same = colour_direct<D>(s, k, f.r, tab_val, tab_lab, lo - 1u) == abs0;
         }
-        if (!same || ++steps > kExtendLimit) break;
+        if (!same || ++steps > limit) break;
         --lo;
         a -= d;
     }
@@ -372,11 +372,11 @@ __device__ void complete_anchor(
             if (abs0 == 0xFFFFFFFFu) abs0 = colour_direct<D>(s, k, f.r, tab_val, tab_lab, x);
             same = colour_direct<D>(s, k, f.r, tab_val, tab_lab, hi + 1u) == abs0;
         }
-        if (!same || ++steps > kExtendLimit) break;
+        if (!same || ++steps > limit) break;
         ++hi;
         a += d;
     }
-    if (steps > kExtendLimit) {
+    if (steps > limit) {
         atomicOr(&failed[f.prime], 2u);
         return;
     }
@@ -497,7 +497,7 @@ struct FilterCtx {
     unsigned long long *best;
     u32 *failed;
     typename D::V dinv;
-    u32 d, t, q, B, bmin, wpp;
+    u32 d, t, q, B, bmin, wpp, ext_limit;
 };
 
 // Scheduled anchors of one task as spans.  The schedule keeps the anchors
@@ -567,7 +567,7 @@ __device__ __forceinline__ void drain_queue(const FilterCtx<D> &c, const u32 *co
             const u32 bit = (u32)__builtin_ctzll(surv);
             surv &= surv - 1ull;
             complete_anchor<D>(c.s, c.f, c.pl, b0 + bit, c.d, c.dinv, c.t, c.tab_val,
-                               c.tab_lab, c.best, c.failed);
+                               c.tab_lab, c.best, c.failed, c.ext_limit);
             ++n_done;
         }
     }
@@ -835,7 +835,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void s
                                                  unsigned long long *__restrict__ best,
                                                  u32 *__restrict__ failed,
                                                  unsigned long long *__restrict__ stats,
-                                                 u32 *__restrict__ queue, u32 kGroup) {
+                                                 u32 *__restrict__ queue, u32 kGroup,
+                                                 u32 ext_limit) {
     // per-lane work counts: window comparisons in 32-anchor plane dwords,
     // candidate words, completed anchors (summed per block, three atomics
     // per block)
@@ -908,6 +909,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void s
         c.q = q;
         c.B = sg->B;
         c.bmin = ((t - 1u) * dd + 1u) / 2u;
+        c.ext_limit = ext_limit;
         switch (fg->np) {
         case 1: filter_task<D, 1>(c, sp, phi, qb, qs, n_windows, n_words, n_done); break;
         case 2: filter_task<D, 2>(c, sp, phi, qb, qs, n_windows, n_words, n_done); break;
@@ -1012,15 +1014,15 @@ extern "C" hipError_t rbg_sparse_filter(const SparsePrime *P, const SparseProf *
                                         const uint16_t *tab_lab, unsigned long long *best,
                                         uint32_t *failed, unsigned long long *stats,
                                         uint32_t *queue, int wide, uint32_t grid, uint32_t group,
-                                        hipStream_t stream) {
+                                        uint32_t extend_limit, hipStream_t stream) {
     if (wide)
         hipLaunchKernelGGL(rbg::sp_filter<rbg::Dom33>, dim3(grid), dim3(256), 0, stream, P, prof,
                            nprof, dmax, planes, (const uint64_t *)inv, (const uint64_t *)tab_val,
-                           tab_lab, best, failed, stats, queue, group);
+                           tab_lab, best, failed, stats, queue, group, extend_limit);
     else
         hipLaunchKernelGGL(rbg::sp_filter<rbg::Dom32>, dim3(grid), dim3(256), 0, stream, P, prof,
                            nprof, dmax, planes, (const uint32_t *)inv, (const uint32_t *)tab_val,
-                           tab_lab, best, failed, stats, queue, group);
+                           tab_lab, best, failed, stats, queue, group, extend_limit);
     return hipGetLastError();
 }
 

@@ -3,20 +3,27 @@
 //
 //   rbg_worker profile [--method batched|plain|dense] [--max-batch N]
 //                      [--no-certify] [--device D] [--domain-bits 32|33]
+//                      [--floor F] [--with-a4] [--hbm-budget BYTES]
 //     stdin : "<id> p1 ... pk" lines (ids consecutive from 0, 1 <= k <= N,
-//             default N = 128, primes < 2^32; with --domain-bits 33 primes
-//             1000 <= p < 2^33, the reference's wider worker,
+//             default N = 128, primes F <= p < 2^32 with F = 1000 as in
+//             src/census/worker32.cu:24 -- `--floor 2` accepts the small
+//             primes that the CPU worker src/census/small_cpu.cpp takes;
+//             with --domain-bits 33 primes 1000 <= p < 2^33, the
+//             reference's wider worker,
 //             reproducibility/gpu_method/source/worker33.cu:24-41), then
 //             "QUIT";
 //     stdout: JSON lines {"type":"ready"}, one {"type":"measurement"} per
-//             prime in input order, {"type":"batch_done"} per batch,
-//             {"type":"bye"}.  Reference: src/census/worker32.cu:10-45 and
-//             src/census/small_cpu.cpp:34-40 (docs/REPRODUCTION.md:29-39).
+//             prime in input order (with --with-a4 an "A4" field is added),
+//             {"type":"batch_done"} per batch, {"type":"bye"}.  Reference:
+//             src/census/worker32.cu:10-45 and src/census/small_cpu.cpp:
+//             34-40 (docs/REPRODUCTION.md:29-39).
 //     A batch is profiled by one library call (all-or-nothing).  A capacity
 //     refusal replays the same whole batch in halves, recursively, within a
 //     30-second deadline (the reference's replay rule, GPU_CONFIG.json:
-//     164-170, with the batch split in place of the epoch); any other
-//     failure, or a replay past the deadline, ends the process with
+//     164-170, with the batch split in place of the epoch) and notes each
+//     split on stderr; --hbm-budget lowers the library's workspace budget,
+//     which makes large batches take that path.  Any other failure, or a
+//     replay past the deadline, ends the process with
 //     "VDW_CAMPAIGN_FAILURE: <reason>" on stderr and exit status 2, with no
 //     output for the batch.
 //
@@ -69,6 +76,9 @@ struct ProfileOptions {
     bool certify = true;
     int device = 0;
     int domain_bits = 32;
+    uint64_t floor = 1000;
+    bool with_a4 = false;
+    uint64_t hbm_budget = 0;
 };
 
 // Profiles `primes` into `out`, replaying halves on a capacity refusal.
@@ -81,6 +91,8 @@ int profile_replay(rbg_ctx *ctx, const uint64_t *primes, uint32_t n, uint32_t fl
     why = rbg_last_error(ctx);
     if (rc != RBG_ECAPACITY || n == 1 || Clock::now() > deadline) return rc;
     const uint32_t half = n / 2;
+    std::cerr << "rbg_worker: capacity refusal of " << n << " primes, replayed as " << half
+              << " + " << n - half << '\n';
     const int a = profile_replay(ctx, primes, half, flags, out, deadline, why);
     if (a) return a;
     return profile_replay(ctx, primes + half, n - half, flags, out + half, deadline, why);
@@ -90,6 +102,7 @@ int run_profile(const ProfileOptions &o) {
     const char *tag = "VDW_CAMPAIGN_FAILURE";
     rbg_config cfg{};
     cfg.device = o.device;
+    cfg.hbm_budget = o.hbm_budget;
     rbg_ctx *ctx = rbg_open(&cfg);
     if (!ctx) fail(tag, "device context");
     std::cout << "{\"type\":\"ready\",\"domain_bits\":" << o.domain_bits << ",\"engine\":\""
@@ -112,7 +125,7 @@ int run_profile(const ProfileOptions &o) {
         std::vector<uint64_t> primes;
         while (in >> tok) {
             if (!parse_u64(tok, v)) fail(tag, "prime token");
-            if (wide ? (v < 1000 || v >= (1ull << 33)) : v >= (1ull << 32))
+            if (wide ? (v < 1000 || v >= (1ull << 33)) : (v < o.floor || v >= (1ull << 32)))
                 fail(tag, "worker prime range");
             if (!rbg_is_prime_wide(v)) fail(tag, "worker primality");
             primes.push_back(v);
@@ -130,8 +143,9 @@ int run_profile(const ProfileOptions &o) {
         outs.precision(12);
         for (const rbg_record_wide &r : recs) {
             outs << "{\"type\":\"measurement\",\"status\":\"PASS\",\"p\":" << r.p
-                 << ",\"mask\":" << r.mask << ",\"total_ms\":" << wall / recs.size()
-                 << ",\"A4\":" << r.A4 << ",\"profiles\":[";
+                 << ",\"mask\":" << r.mask << ",\"total_ms\":" << wall / recs.size();
+            if (o.with_a4) outs << ",\"A4\":" << r.A4;
+            outs << ",\"profiles\":[";
             bool comma = false;
             for (uint32_t k = 0; k < 8; ++k) {
                 if (!(r.mask & (1u << k))) continue;
@@ -192,11 +206,19 @@ int run_a4() {
         std::vector<std::thread> pool;
         for (unsigned t = 0; t < threads; ++t)
             pool.emplace_back([&, t] {
-                char buf[512];
                 for (uint64_t i = t; i < n; i += threads) {
-                    size_t len = sizeof buf;
-                    bad[i] = rbg_a4_certificate(primes[i], &values[i], &attempts[i], buf, &len);
-                    if (!bad[i] && (long long)i == audit) certificate.assign(buf, len);
+                    if ((long long)i != audit) {
+                        bad[i] = rbg_a4_certificate(primes[i], &values[i], &attempts[i],
+                                                    nullptr, nullptr);
+                        continue;
+                    }
+                    // the audited prime: ask for the length, then the text
+                    size_t len = 0;
+                    bad[i] = rbg_a4_certificate(primes[i], &values[i], &attempts[i], nullptr, &len);
+                    if (bad[i] != RBG_ECAPACITY) continue;
+                    std::string text(len, '\0');
+                    bad[i] = rbg_a4_certificate(primes[i], &values[i], &attempts[i], &text[0], &len);
+                    if (!bad[i]) certificate = text;
                 }
             });
         for (auto &th : pool) th.join();
@@ -233,7 +255,8 @@ int run_a4() {
 int main(int argc, char **argv) {
     if (argc < 2) {
         std::cerr << "usage: rbg_worker profile [--method batched|plain|dense] [--max-batch N] "
-                     "[--no-certify] [--device D] [--domain-bits 32|33] | rbg_worker a4\n";
+                     "[--no-certify] [--device D] [--domain-bits 32|33] [--floor F] "
+                     "[--with-a4] [--hbm-budget BYTES] | rbg_worker a4\n";
         return 2;
     }
     const std::string mode = argv[1];
@@ -255,6 +278,14 @@ int main(int argc, char **argv) {
             o.max_batch = (uint32_t)std::strtoul(argv[++i], nullptr, 10);
             if (o.max_batch < 1 || o.max_batch > RBG_MAX_BLOCK)
                 fail("VDW_CAMPAIGN_FAILURE", "max batch outside 1..4096");
+        } else if (a == "--floor" && i + 1 < argc) {
+            if (!parse_u64(argv[++i], o.floor) || o.floor < 2)
+                fail("VDW_CAMPAIGN_FAILURE", "floor must be an integer >= 2");
+        } else if (a == "--with-a4") {
+            o.with_a4 = true;
+        } else if (a == "--hbm-budget" && i + 1 < argc) {
+            if (!parse_u64(argv[++i], o.hbm_budget) || o.hbm_budget == 0)
+                fail("VDW_CAMPAIGN_FAILURE", "hbm budget must be a positive integer");
         } else if (a == "--no-certify") {
             o.certify = false;
         } else if (a == "--device" && i + 1 < argc) {

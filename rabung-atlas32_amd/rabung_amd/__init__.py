@@ -63,7 +63,8 @@ class Timing(ctypes.Structure):
 class Config(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int32), ("tile_chunks", ctypes.c_uint32),
                 ("blocks_per_cu", ctypes.c_uint32), ("hbm_budget", ctypes.c_uint64),
-                ("xcd_group", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
+                ("xcd_group", ctypes.c_uint32), ("cover_q", ctypes.c_uint32),
+                ("extend_limit", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
 
 
 EXPORTS = ["rbg_open", "rbg_profile_block", "rbg_profile_block_device", "rbg_profile_batch",
@@ -73,7 +74,7 @@ EXPORTS = ["rbg_open", "rbg_profile_block", "rbg_profile_block_device", "rbg_pro
            "rbg_encode_a4_frame", "rbg_science_hash", "rbg_prime_hash", "rbg_a4_certificate", "rbg_microbench",
            "rbg_profile_block_wide", "rbg_profile_batch_wide", "rbg_estimate_bytes_wide",
            "rbg_a4_host_wide", "rbg_is_prime_wide", "rbg_mask_wide", "rbg_primes_from_wide",
-           "rbg_check_wide",
+           "rbg_check_wide", "rbg_certify_records",
            "baseline_init",
            "baseline_profile"]
 
@@ -171,6 +172,12 @@ def load():
                                    ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int,
                                    P(ctypes.c_uint64)]
     lib.rbg_check_wide.restype = ctypes.c_int
+    lib.rbg_a4_certificate.argtypes = [ctypes.c_uint32, P(ctypes.c_uint32), P(ctypes.c_uint64),
+                                       ctypes.c_void_p, P(ctypes.c_size_t)]
+    lib.rbg_a4_certificate.restype = ctypes.c_int
+    lib.rbg_certify_records.argtypes = [P(Record), ctypes.c_uint32, ctypes.c_uint32,
+                                        P(ctypes.c_uint32)]
+    lib.rbg_certify_records.restype = ctypes.c_int
     _lib = lib
     return lib
 
@@ -231,9 +238,11 @@ def science_hash(records):
 class Engine:
     """One library context (device, stream, workspaces)."""
 
-    def __init__(self, device=0, tile_chunks=0, blocks_per_cu=0, hbm_budget=0, xcd_group=0):
+    def __init__(self, device=0, tile_chunks=0, blocks_per_cu=0, hbm_budget=0, xcd_group=0,
+                 cover_q=0, extend_limit=0):
         lib = load()
-        cfg = Config(device, tile_chunks, blocks_per_cu, hbm_budget, xcd_group, 0)
+        cfg = Config(device, tile_chunks, blocks_per_cu, hbm_budget, xcd_group, cover_q,
+                     extend_limit, 0)
         self._ctx = lib.rbg_open(ctypes.byref(cfg))
         if not self._ctx:
             raise RabungError(2, "rbg_open failed (no HIP device %d?)" % device)
@@ -398,11 +407,45 @@ def a4_host(primes, threads=0):
     return list(out[:n])
 
 
-def baseline_profile(p, mask_bits=0, full=True):
-    """Reference bridge: returns (status, M by r[10], prefix by r[10])."""
+def a4_certificate(p):
+    """(A4, audit certificate dict) of one prime below 2^32 by the host point
+    count (rbg_a4_certificate)."""
+    import json
     lib = load()
-    out = (ctypes.c_uint * 10)()
-    pre = (ctypes.c_uint * 10)()
+    a4 = ctypes.c_uint32()
+    attempts = ctypes.c_uint64()
+    size = ctypes.c_size_t(0)
+    rc = lib.rbg_a4_certificate(p, ctypes.byref(a4), ctypes.byref(attempts), None,
+                                ctypes.byref(size))
+    if rc not in (0, 4):
+        raise RabungError(rc, "A4 certificate")
+    buf = ctypes.create_string_buffer(size.value)
+    rc = lib.rbg_a4_certificate(p, ctypes.byref(a4), ctypes.byref(attempts), buf,
+                                ctypes.byref(size))
+    if rc:
+        raise RabungError(rc, "A4 certificate")
+    return a4.value, json.loads(buf.raw[:size.value].decode())
+
+
+def certify_records(records, threads=0):
+    """Host-only direct verification of library records (rbg_certify_records):
+    returns None when all pass, else the index of the first failing record."""
+    arr = _records_array(records)
+    bad = ctypes.c_uint32(0)
+    rc = load().rbg_certify_records(arr, len(records), threads, ctypes.byref(bad))
+    if rc == 0:
+        return None
+    if rc == 3:
+        return bad.value
+    raise RabungError(rc, "certify_records")
+
+
+def baseline_profile(p, mask_bits=0, full=True, fill=0):
+    """Reference bridge: returns (status, M by r[10], prefix by r[10]); the
+    arrays start as `fill` (entries the call does not write keep it)."""
+    lib = load()
+    out = (ctypes.c_uint * 10)(*([fill] * 10))
+    pre = (ctypes.c_uint * 10)(*([fill] * 10))
     rc = lib.baseline_profile(p, mask_bits, out, pre, 1 if full else 0)
     return rc, list(out), list(pre)
 

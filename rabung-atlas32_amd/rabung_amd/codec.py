@@ -1,10 +1,9 @@
 """Pack-record codec for census profiles (scan frames and A4 frames).
 
 Written from the layout specification in the reference's docs/DATA_FORMAT.md
-(lines 9-47 for frames, 63-77 for chains); the reference's own decoder is
-scripts/atlas_codec.py:20-60.  The byte layout must be identical to the
-reference's so that frames written here decode there and vice versa, which is
-pinned by re-encoding the reference sample blocks byte for byte.
+(lines 9-47 for frames, 63-77 for chains); frames written here must be the
+reference's bytes (scripts/atlas_codec.py:20-60 reads the same layout), which
+is pinned by re-encoding the reference sample blocks byte for byte.
 
 Scan frame: header struct '<8sQQQQII32s' =
     (b'VDWSM1\\0\\0', compressed_len, raw_len, lo, hi, count, witness_flag,
@@ -56,21 +55,6 @@ def put_varint(out, value):
             return
 
 
-def get_varint(raw, index):
-    value = 0
-    for shift_index in range(10):
-        if index >= len(raw):
-            raise FrameError("truncated integer")
-        byte = raw[index]
-        index += 1
-        value |= (byte & 0x7F) << (7 * shift_index)
-        if byte < 0x80:
-            if value >= 1 << 64 or (shift_index and byte == 0):
-                raise FrameError("noncanonical integer")
-            return value, index
-    raise FrameError("oversized integer")
-
-
 def encode_scan_raw(records, lo, witnesses=True):
     """records: iterable of dicts {'p', 'profiles': [{'r','max','prefix','witness'}...]}"""
     out = bytearray()
@@ -105,62 +89,6 @@ def encode_scan_frame(records, lo, hi, witnesses=True, level=9):
     return header + comp
 
 
-def _inflate(payload, size):
-    d = zlib.decompressobj()
-    raw = d.decompress(payload, size + 1)
-    if len(raw) != size or not d.eof or d.unused_data:
-        raise FrameError("frame hash/length")
-    return raw
-
-
-def decode_scan_frame(frame):
-    """Returns (rows, meta); rows are {'p', 'profiles': [{'r','max','prefix','K'[, 'witness']}]}."""
-    if len(frame) < SCAN_HEADER.size:
-        raise FrameError("truncated header")
-    magic, n, size, lo, hi, count, flags, digest = SCAN_HEADER.unpack(frame[:SCAN_HEADER.size])
-    if magic != SCAN_MAGIC or flags not in (0, 1) or n != len(frame) - SCAN_HEADER.size:
-        raise FrameError("frame header")
-    if size > MAX_FRAME or n > MAX_FRAME:
-        raise FrameError("oversized frame")
-    try:
-        raw = _inflate(frame[SCAN_HEADER.size:], size)
-    except zlib.error as exc:
-        raise FrameError("frame payload: %s" % exc) from None
-    if hashlib.sha256(raw).digest() != digest:
-        raise FrameError("frame hash/length")
-    rows = []
-    i = 0
-    p = lo - 1
-    for _ in range(count):
-        delta, i = get_varint(raw, i)
-        p += delta
-        if delta == 0 or not lo <= p < hi or i >= len(raw):
-            raise FrameError("record ordering")
-        mask = raw[i]
-        i += 1
-        if mask != admissible_mask(p):
-            raise FrameError("decoded mask")
-        profiles = []
-        for r in range(2, 10):
-            if mask & (1 << (r - 2)):
-                m, i = get_varint(raw, i)
-                l, i = get_varint(raw, i)
-                v = {"r": r, "max": m, "prefix": l, "K": threshold_k(p, r, m, l)}
-                if not 1 <= l <= m <= (p - 1) // r:
-                    raise FrameError("decoded profile bound")
-                if flags:
-                    v["witness"], i = get_varint(raw, i)
-                    if not 1 <= v["witness"] <= (p - 1) // 2:
-                        raise FrameError("decoded witness")
-                profiles.append(v)
-        rows.append({"p": p, "profiles": profiles})
-    if i != len(raw):
-        raise FrameError("trailing record bytes")
-    meta = {"lo": lo, "hi": hi, "count": count, "witnesses": bool(flags),
-            "raw_sha256": digest.hex(), "raw_bytes": size, "compressed_bytes": n}
-    return rows, meta
-
-
 def prime_hash(primes):
     h = hashlib.sha256(b"vdw-primes-u64le")
     for p in primes:
@@ -191,22 +119,170 @@ def encode_a4_frame(primes, values, lo, hi, profile_hash, level=9):
     return header + comp
 
 
-def decode_a4_frame(frame):
-    if len(frame) < A4_HEADER.size:
-        raise FrameError("A4 short frame")
-    magic, lo, hi, n, csize, rsize, flags, ph, sh, digest = A4_HEADER.unpack(frame[:A4_HEADER.size])
-    if (magic != A4_MAGIC or n > MAX_A4_PRIMES or rsize != 4 * n or flags
-            or csize != len(frame) - A4_HEADER.size):
-        raise FrameError("A4 header")
+# ---- decoding -------------------------------------------------------------
+#
+# A decoder trusts nothing in a frame: the header fields are checked against
+# the bytes that follow, the payload is inflated to exactly its declared
+# length and must hash to the header's digest, and every record must satisfy
+# the output contract (ascending primes inside [lo, hi), the admissibility
+# mask of its prime, 1 <= L <= M <= (p-1)/r, 1 <= W <= (p-1)/2).  Every
+# failure raises FrameError naming the frame kind and what was wrong.
+
+
+class PayloadReader:
+    """Forward-only cursor over an inflated scan payload."""
+
+    __slots__ = ("data", "offset")
+
+    def __init__(self, data):
+        self.data = data
+        self.offset = 0
+
+    def at_end(self):
+        return self.offset == len(self.data)
+
+    def byte(self, context):
+        if self.offset >= len(self.data):
+            raise FrameError("scan payload ends inside %s" % context)
+        value = self.data[self.offset]
+        self.offset += 1
+        return value
+
+    def uint(self, context):
+        """One unsigned base-128 integer, least significant group first, in
+        its shortest form and below 2^64."""
+        first = self.offset
+        value = 0
+        for group in range(10):
+            byte = self.byte(context)
+            value |= (byte & 0x7F) << (7 * group)
+            if not byte & 0x80:
+                if value >> 64:
+                    raise FrameError("%s at payload byte %d exceeds 64 bits" % (context, first))
+                if group and not byte:
+                    raise FrameError("%s at payload byte %d has a redundant zero group"
+                                     % (context, first))
+                return value
+        raise FrameError("%s at payload byte %d runs past ten bytes" % (context, first))
+
+
+def read_varint(data, offset=0):
+    """(value, next offset) of the varint at `offset` of `data`."""
+    reader = PayloadReader(data)
+    reader.offset = offset
+    value = reader.uint("integer")
+    return value, reader.offset
+
+
+def _split(frame, layout, magic, kind):
+    """Header fields and compressed body of a frame whose first field is its
+    magic."""
+    if len(frame) < layout.size:
+        raise FrameError("%s frame: %d bytes, shorter than its %d-byte header"
+                         % (kind, len(frame), layout.size))
+    fields = layout.unpack_from(frame)
+    if fields[0] != magic:
+        raise FrameError("%s frame: magic %r instead of %r" % (kind, fields[0], magic))
+    return fields, bytes(frame[layout.size:])
+
+
+def _inflate_exact(body, raw_len, digest, kind):
+    """The payload of `body`, which must inflate to exactly raw_len bytes
+    with the SHA-256 `digest`, using the whole compressed stream."""
+    if raw_len > MAX_FRAME:
+        raise FrameError("%s frame: declared payload of %d bytes exceeds %d"
+                         % (kind, raw_len, MAX_FRAME))
+    inflater = zlib.decompressobj()
     try:
-        raw = _inflate(frame[A4_HEADER.size:], rsize)
+        # output is capped at the declared length; input left over at the cap
+        # is fed once more with room for one byte, which either ends the
+        # stream or proves it longer than declared
+        data = inflater.decompress(body, max(raw_len, 1))
+        if not inflater.eof and inflater.unconsumed_tail:
+            data += inflater.decompress(inflater.unconsumed_tail, 1)
     except zlib.error as exc:
-        raise FrameError("A4 payload: %s" % exc) from None
-    if hashlib.sha256(raw).digest() != digest:
-        raise FrameError("A4 frame hash")
-    values = list(struct.unpack("<%dI" % n, raw))
-    meta = {"lo": lo, "hi": hi, "count": n, "prime_hash": ph.hex(),
-            "profile_science_hash": sh.hex(), "values_sha256": digest.hex()}
+        raise FrameError("%s frame: compressed payload is corrupt (%s)" % (kind, exc)) from None
+    if len(data) != raw_len or inflater.unconsumed_tail:
+        raise FrameError("%s frame: payload inflates to a length other than the declared %d"
+                         % (kind, raw_len))
+    if not inflater.eof:
+        raise FrameError("%s frame: compressed stream is cut short" % kind)
+    if inflater.unused_data:
+        raise FrameError("%s frame: %d bytes follow the compressed stream"
+                         % (kind, len(inflater.unused_data)))
+    if hashlib.sha256(data).digest() != digest:
+        raise FrameError("%s frame: payload SHA-256 differs from the header" % kind)
+    return data
+
+
+def decode_scan_frame(frame):
+    """Rows {'p', 'profiles': [{'r', 'max', 'prefix', 'K'[, 'witness']}]}
+    and the frame's metadata, from one scan frame (DATA_FORMAT.md:9-31)."""
+    (_magic, comp_len, raw_len, lo, hi, count, witness_flag,
+     digest), body = _split(frame, SCAN_HEADER, SCAN_MAGIC, "scan")
+    if witness_flag > 1:
+        raise FrameError("scan frame: witness flag %d is neither 0 nor 1" % witness_flag)
+    if comp_len != len(body):
+        raise FrameError("scan frame: header declares %d compressed bytes, %d follow"
+                         % (comp_len, len(body)))
+    reader = PayloadReader(_inflate_exact(body, raw_len, digest, "scan"))
+    rows = []
+    prime = lo - 1
+    for index in range(count):
+        where = "record %d" % index
+        step = reader.uint(where + " (prime delta)")
+        if not step:
+            raise FrameError("scan frame: %s repeats the previous prime" % where)
+        prime += step
+        if prime >= hi:
+            raise FrameError("scan frame: %s has p=%d outside [%d, %d)" % (where, prime, lo, hi))
+        mask = reader.byte(where + " (mask)")
+        if mask != admissible_mask(prime):
+            raise FrameError("scan frame: %s stores mask %#04x, p=%d has %#04x"
+                             % (where, mask, prime, admissible_mask(prime)))
+        profiles = []
+        for r in (r for r in range(2, 10) if mask >> (r - 2) & 1):
+            longest = reader.uint("%s r=%d (M)" % (where, r))
+            prefix = reader.uint("%s r=%d (L)" % (where, r))
+            if not 1 <= prefix <= longest <= (prime - 1) // r:
+                raise FrameError("scan frame: %s p=%d r=%d has M=%d, L=%d outside "
+                                 "1 <= L <= M <= (p-1)/r" % (where, prime, r, longest, prefix))
+            entry = {"r": r, "max": longest, "prefix": prefix,
+                     "K": threshold_k(prime, r, longest, prefix)}
+            if witness_flag:
+                start = reader.uint("%s r=%d (witness)" % (where, r))
+                if not 1 <= start <= (prime - 1) // 2:
+                    raise FrameError("scan frame: %s p=%d r=%d has witness %d outside "
+                                     "1..(p-1)/2" % (where, prime, r, start))
+                entry["witness"] = start
+            profiles.append(entry)
+        rows.append({"p": prime, "profiles": profiles})
+    if not reader.at_end():
+        raise FrameError("scan frame: %d payload bytes after record %d"
+                         % (len(reader.data) - reader.offset, count - 1))
+    meta = {"lo": lo, "hi": hi, "count": count, "witnesses": witness_flag == 1,
+            "raw_sha256": digest.hex(), "raw_bytes": raw_len, "compressed_bytes": comp_len}
+    return rows, meta
+
+
+def decode_a4_frame(frame):
+    """A4 values (uint32 per prime) and the frame's metadata, from one A4
+    frame (DATA_FORMAT.md:35-47)."""
+    (_magic, lo, hi, count, comp_len, raw_len, flags, primes_digest, profile_digest,
+     digest), body = _split(frame, A4_HEADER, A4_MAGIC, "A4")
+    if flags:
+        raise FrameError("A4 frame: flags %#x, the format defines only 0" % flags)
+    if count > MAX_A4_PRIMES:
+        raise FrameError("A4 frame: %d primes, a block holds at most %d" % (count, MAX_A4_PRIMES))
+    if raw_len != 4 * count:
+        raise FrameError("A4 frame: %d payload bytes declared for %d primes" % (raw_len, count))
+    if comp_len != len(body):
+        raise FrameError("A4 frame: header declares %d compressed bytes, %d follow"
+                         % (comp_len, len(body)))
+    data = _inflate_exact(body, raw_len, digest, "A4")
+    values = [v for (v,) in struct.iter_unpack("<I", data)]
+    meta = {"lo": lo, "hi": hi, "count": count, "prime_hash": primes_digest.hex(),
+            "profile_science_hash": profile_digest.hex(), "values_sha256": digest.hex()}
     return values, meta
 
 

@@ -49,3 +49,18 @@ def night33_rows():
     with gzip.open(GOLDEN / "night33.json.gz", "rt") as f:
         data = json.load(f)
     return _expand(data["rows"])
+
+
+def block2_24():
+    """(rows, a4, document) of the 4096 consecutive primes from the first
+    prime >= 2^24, written by tests/golden/make_block2_24.py (oracle values,
+    cross-checked there against the reference's moment_worker for every A4
+    and independent_profile on a seeded subset)."""
+    with gzip.open(GOLDEN / "block2_24.json.gz", "rt") as f:
+        doc = json.load(f)
+    rows, a4 = [], []
+    for p, _mask, prof, value in doc["rows"]:
+        rows.append({"p": p, "profiles": [{"r": r, "max": m, "prefix": l, "K": k, "witness": w}
+                                          for r, m, l, k, w in prof]})
+        a4.append(value)
+    return rows, a4, doc

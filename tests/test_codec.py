@@ -5,6 +5,7 @@ truncated, extended or bit-flipped frames raise) and the sample checks of
 scripts/verify_atlas.py:18-45 (prime/science/values hashes, A4 range).
 """
 import hashlib
+import re
 
 import pytest
 
@@ -50,11 +51,11 @@ def test_varint_canonical():
     for v in [0, 1, 127, 128, 300, 2 ** 32 - 1, 2 ** 63, 2 ** 64 - 1]:
         out = bytearray()
         codec.put_varint(out, v)
-        assert codec.get_varint(bytes(out), 0) == (v, len(out))
+        assert codec.read_varint(bytes(out), 0) == (v, len(out))
     with pytest.raises(ValueError):
-        codec.get_varint(b"\x80\x00", 0)  # non-canonical trailing zero
+        codec.read_varint(b"\x80\x00", 0)  # non-canonical trailing zero
     with pytest.raises(ValueError):
-        codec.get_varint(b"\x80", 0)  # truncated
+        codec.read_varint(b"\x80", 0)  # truncated
 
 
 def test_k_formula_worked_example():
@@ -122,3 +123,74 @@ def test_library_encoder_refuses_bad_records():
     worse[7] = dict(worse[7], A4=worse[7]["p"])  # beyond (p-3)/2
     with pytest.raises(rabung_amd.RabungError):
         rabung_amd.encode_a4_frame(worse, lo, hi)
+
+
+def _scan_frame_of(raw, lo, hi, count, body=None, raw_len=None):
+    """A scan frame around an arbitrary payload (hash and lengths made to
+    match unless overridden), for the decoder's record checks."""
+    import zlib
+    body = zlib.compress(raw, 6) if body is None else body
+    header = codec.SCAN_HEADER.pack(codec.SCAN_MAGIC, len(body),
+                                    len(raw) if raw_len is None else raw_len, lo, hi, count, 1,
+                                    hashlib.sha256(raw).digest())
+    return header + body
+
+
+def _payload(*records):
+    out = bytearray()
+    for delta, mask, triples in records:
+        codec.put_varint(out, delta)
+        out.append(mask)
+        for value in triples:
+            codec.put_varint(out, value)
+    return bytes(out)
+
+
+def test_decoder_record_checks():
+    """p = 101 (mask 0b1101: r = 2, 4, 5) then p = 103 (r = 2, 3, 6): each
+    broken field is reported, the intact payload decodes."""
+    good = [(101 - 99, 0b1101, [7, 1, 19, 3, 1, 10, 3, 1, 7]), (2, 0x13, [5, 1, 9, 4, 2, 3, 2, 1, 7])]
+    rows, meta = codec.decode_scan_frame(_scan_frame_of(_payload(*good), 100, 110, 2))
+    assert [r["p"] for r in rows] == [101, 103] and meta["count"] == 2
+    assert rows[0]["profiles"][0] == {"r": 2, "max": 7, "prefix": 1, "K": 8, "witness": 19}
+    cases = {
+        "repeats the previous prime": [good[0], (0, 0x13, [5, 1, 9, 4, 2, 3, 2, 1, 7])],
+        "outside [100, 110)": [good[0], (10, 0x13, [5, 1, 9, 4, 2, 3, 2, 1, 7])],
+        "stores mask": [(2, 0b1100, [7, 1, 19, 3, 1, 10, 3, 1, 7])],
+        "outside 1 <= L <= M": [(2, 0b1101, [7, 8, 19, 3, 1, 10, 3, 1, 7])],
+        "witness 0 outside": [(2, 0b1101, [7, 1, 0, 3, 1, 10, 3, 1, 7])],
+        "witness 51 outside": [(2, 0b1101, [7, 1, 51, 3, 1, 10, 3, 1, 7])],
+    }
+    for text, recs in cases.items():
+        with pytest.raises(codec.FrameError, match=re.escape(text)):
+            codec.decode_scan_frame(_scan_frame_of(_payload(*recs), 100, 110, len(recs)))
+    raw = _payload(*good)
+    with pytest.raises(codec.FrameError, match="payload bytes after record"):
+        codec.decode_scan_frame(_scan_frame_of(raw + b"\x01", 100, 110, 2))
+    with pytest.raises(codec.FrameError, match="ends inside record 1"):
+        codec.decode_scan_frame(_scan_frame_of(raw[:-2], 100, 110, 2))
+
+
+def test_decoder_stream_checks():
+    import zlib
+    raw = _payload((2, 0b1101, [7, 1, 19, 3, 1, 10, 3, 1, 7]))
+    body = zlib.compress(raw, 6)
+    with pytest.raises(codec.FrameError, match="follow the compressed stream"):
+        codec.decode_scan_frame(_scan_frame_of(raw, 100, 110, 1, body=body + b"\x00\x00"))
+    with pytest.raises(codec.FrameError, match="cut short|corrupt"):
+        codec.decode_scan_frame(_scan_frame_of(raw, 100, 110, 1, body=body[:-5]))
+    with pytest.raises(codec.FrameError, match="length other than the declared"):
+        codec.decode_scan_frame(_scan_frame_of(raw, 100, 110, 1, raw_len=len(raw) - 1))
+    with pytest.raises(codec.FrameError, match="length other than the declared"):
+        codec.decode_scan_frame(_scan_frame_of(raw, 100, 110, 1, raw_len=len(raw) + 1))
+    with pytest.raises(codec.FrameError, match="exceeds"):
+        codec.decode_scan_frame(_scan_frame_of(raw, 100, 110, 1, raw_len=codec.MAX_FRAME + 1))
+    frame = bytearray(_scan_frame_of(raw, 100, 110, 1))
+    frame[-1] ^= 0x01  # adler-32 trailer of the stream
+    with pytest.raises(codec.FrameError):
+        codec.decode_scan_frame(bytes(frame))
+    values = [0, 3, 5]
+    a4 = codec.encode_a4_frame([2, 3, 5], values, 2, 7, bytes(32), 6)
+    assert codec.decode_a4_frame(a4)[0] == values
+    with pytest.raises(codec.FrameError, match="flags"):
+        codec.decode_a4_frame(a4[:36] + b"\x01" + a4[37:])

@@ -84,6 +84,27 @@ def test_holdout_profiles(engine):
             assert got[j] == w["rows"][i], (primes[j], got[j], w["rows"][i])
 
 
+@pytest.mark.parametrize("method", [rabung_amd.METHOD_BATCHED, rabung_amd.METHOD_PLAIN,
+                                    rabung_amd.METHOD_DENSE])
+def test_block2_24_complete(engine, method):
+    """The plain-method workload, all 4096 consecutive primes from the first
+    prime >= 2^24 (13,741 profiles), in one call per method: every (M, L, W,
+    K) and A4 equals the committed fixture (oracle values, cross-checked
+    against the reference's moment_worker and independent_profile)."""
+    rows, a4, doc = golden_data.block2_24()
+    primes = [r["p"] for r in rows]
+    recs = engine.profile_block(primes, method=method)
+    got = as_rows(recs)
+    bad = [i for i in range(len(rows)) if got[i] != rows[i] or recs[i]["A4"] != a4[i]]
+    assert not bad, ("mismatch", primes[bad[0]], got[bad[0]], rows[bad[0]], recs[bad[0]]["A4"],
+                     a4[bad[0]])
+    assert codec.science_hash(got) == codec.science_hash(rows)
+    t = engine.last_timing()
+    if method == rabung_amd.METHOD_BATCHED:
+        assert t["sparse_primes"] == 4096 and t["sparse_profiles"] == doc["profiles"]
+        assert t["fallbacks"] == 0
+
+
 def test_oracle_near_2_24(engine):
     """Seeded primes of the 4096-prime block starting at 2^24 against the oracle."""
     primes = oracle.next_primes(1 << 24, 4096)
@@ -94,6 +115,109 @@ def test_oracle_near_2_24(engine):
     for g, e in zip(got, exp):
         assert g == e, (g["p"], g, e)
 
+
+BATCH64_SHA256 = "f4ebfa462d8f7a50593e5a31f4265609788fdb4492f929ad2323afb7c5b1b8e1"
+
+
+def test_batch64_workload(engine):
+    """The batching workload: 64 seeded blocks of U{1..4096} consecutive
+    primes from log-uniform starts in [2^20, 2^32) (152,696 primes, 508,348
+    profiles) in ONE rbg_profile_batch call.  Checked four ways: equal to
+    one profile_block call per block; equal to every committed golden value
+    the blocks happen to contain; equal to the dense scan on a seeded subset
+    of 48 primes; equal to the CPU oracle on a seeded subset of its primes
+    below 2^25."""
+    import hashlib
+    import struct
+    from rabung_amd import workloads
+    blocks = workloads.batch_blocks(seed=2026, nblocks=64)
+    flat = [p for b in blocks for p in b]
+    assert hashlib.sha256(b"".join(struct.pack("<I", p) for p in flat)).hexdigest() == \
+        BATCH64_SHA256
+    got = engine.profile_batch(flat)
+    t = engine.last_timing()
+    assert t["sparse_profiles"] == 508348 and t["fallbacks"] == 0
+    # per block
+    k = 0
+    for b in blocks:
+        assert engine.profile_block(b) == got[k:k + len(b)], b[0]
+        k += len(b)
+    # golden values inside the blocks
+    golden = {}
+    for block in (0, 49900):
+        rows, a4, _ = golden_data.sample_block(block)
+        golden.update({r["p"]: (r, v) for r, v in zip(rows, a4)})
+    rows, a4, _ = golden_data.block2_24()
+    golden.update({r["p"]: (r, v) for r, v in zip(rows, a4)})
+    for w in golden_data.holdout_windows():
+        golden.update({r["p"]: (r, None) for r in w["rows"]})
+    golden.update({r["p"]: (r, None) for r in golden_data.night_rows()})
+    index = {p: i for i, p in enumerate(flat)}
+    for p in set(golden) & set(index):
+        row, value = golden[p]
+        rec = got[index[p]]
+        assert as_rows([rec])[0] == row, p
+        assert value is None or rec["A4"] == value, p
+    # dense scan on a seeded subset
+    rng = random.Random(64)
+    pick = sorted(rng.sample(range(len(flat)), 48))
+    dense = engine.profile_block([flat[i] for i in pick], method=rabung_amd.METHOD_DENSE)
+    for i, d in zip(pick, dense):
+        assert got[i] == d, flat[i]
+    # the oracle on small primes of the workload
+    small = [i for i in range(len(flat)) if flat[i] < 1 << 25]
+    pick = sorted(rng.sample(small, min(16, len(small))))
+    exp = oracle.profile_many([flat[i] for i in pick])
+    for i, e in zip(pick, exp):
+        assert got[i] == e, flat[i]
+
+
+def test_forced_refusal_falls_back_to_dense():
+    """With an extension limit of one step every passing anchor whose run
+    needs more steps refuses its prime (the refusal the reference takes on
+    queue overflow, gpu_pipeline.cuh:335,342,363); the prime is recomputed by
+    the dense scan and the records equal the default engine's and the dense
+    method's, golden values included."""
+    rows, a4, _ = golden_data.sample_block(49900)
+    pick = sorted(random.Random(12).sample(range(len(rows)), 12))
+    primes = [rows[i]["p"] for i in pick] + oracle.next_primes(1 << 24, 8) + \
+        oracle.next_primes(70001, 4)
+    with rabung_amd.Engine() as eng:
+        normal = eng.profile_block(primes)
+        assert eng.last_timing()["fallbacks"] == 0
+        dense = eng.profile_block(primes, method=rabung_amd.METHOD_DENSE)
+    with rabung_amd.Engine(extend_limit=1) as eng:
+        forced = eng.profile_block(primes)
+        t = eng.last_timing()
+    assert t["fallbacks"] > 0 and t["dense_primes"] >= t["fallbacks"]
+    assert forced == normal == dense
+    got = as_rows(forced[:12])
+    for j, i in enumerate(pick):
+        assert got[j] == rows[i] and forced[j]["A4"] == a4[i]
+
+
+@pytest.mark.parametrize("q", [12, 24, 63])
+def test_cover_threshold_does_not_change_records(q):
+    """Any certified cover gives the same exact profiles: threshold caps Q of
+    12, 24 and 63 against the default 48 (the seeded primes span the sparse
+    domain)."""
+    rng = random.Random(q)
+    primes = sorted({rabung_amd.primes_from(int(2.0 ** rng.uniform(17, 32)), 1)[0]
+                     for _ in range(24)})
+    with rabung_amd.Engine() as eng:
+        ref = eng.profile_block(primes)
+    with rabung_amd.Engine(cover_q=q) as eng:
+        got = eng.profile_block(primes)
+        t = eng.last_timing()
+    assert got == ref
+    assert t["sparse_primes"] > 0
+
+
+def test_cover_threshold_outside_range_refused():
+    with rabung_amd.Engine(cover_q=64) as eng:
+        with pytest.raises(rabung_amd.RabungError) as e:
+            eng.profile_block([101])
+    assert e.value.status == 1
 
 def test_methods_agree_and_are_deterministic(engine):
     primes = oracle.next_primes(3_000_000_000, 16) + [4294967291, 4273768801]
@@ -147,17 +271,31 @@ def test_refusals(engine):
 
 
 def test_baseline_bridge():
-    rc, out, pre = rabung_amd.baseline_profile(4294967291, 0, True)
-    assert rc == 0
-    exp = oracle.profile_many([1000003])[0]
-    rc, out, pre = rabung_amd.baseline_profile(1000003, 0, True)
+    """baseline_profile keeps the reference bridge's contract
+    (baseline_bridge.cu:21-27): only the caller's orders are written, 2 for
+    a mask it cannot serve, and a failure code other than 2."""
+    exp = oracle.profile_many([1000003])[0]  # 1000002 = 2 * 3 * 166667: r = 2, 3, 6
+    adm = exp["mask"]
+    unset = 0xFFFFFFFF
+    rc, out, pre = rabung_amd.baseline_profile(1000003, 0xFF, True, fill=unset)
     assert rc == 0
     for r in range(2, 10):
-        if exp["mask"] & (1 << (r - 2)):
+        if adm & (1 << (r - 2)):
             assert out[r] == exp["M"][r - 2] and pre[r] == exp["L"][r - 2]
-    rc, out, pre = rabung_amd.baseline_profile(1000003, 1 << (7 - 2), False)
-    assert rc == 2  # 7 does not divide 1000002
-
+        else:
+            assert out[r] == 0 and pre[r] == 0  # asked for, not admissible
+    assert out[0] == out[1] == unset
+    rc, out, pre = rabung_amd.baseline_profile(1000003, 1 << (3 - 2), True, fill=unset)
+    assert rc == 0 and out[3] == exp["M"][1] and pre[3] == exp["L"][1]
+    assert all(out[r] == unset for r in range(10) if r != 3)
+    rc, out, pre = rabung_amd.baseline_profile(1000003, (1 << 0) | (1 << 4), False, fill=unset)
+    assert rc == 0 and (out[2], out[6]) == (exp["M"][0], exp["M"][4]) and out[3] == unset
+    assert rabung_amd.baseline_profile(1000003, 1 << (7 - 2), False)[0] == 2  # 7 does not divide
+    assert rabung_amd.baseline_profile(1000003, 0, False)[0] == 2
+    assert rabung_amd.baseline_profile(1000001, 1, True)[0] == -1  # 1000001 = 101 * 9901
+    rc, out, pre = rabung_amd.baseline_profile(4294967291, 0xFF, True)
+    rows, _, _ = golden_data.sample_block(49900)
+    assert rc == 0 and out[2] == rows[-1]["profiles"][0]["max"]
 
 def test_sparse_block49900_large_subset(engine):
     """Sparse engine on 400 seeded primes just below 2^32 (profiles and A4)."""

@@ -143,3 +143,31 @@ def test_reference_moment_worker_a4_near_2_24():
     pick = sorted(random.Random(24).sample(primes, 32))
     recs = oracle.profile_many(pick)
     assert [r["A4"] for r in recs] == moment_worker_a4(pick)
+
+
+def test_block2_24_fixture_consistent():
+    """The committed 2^24 block fixture: the prime list (count, ends, hash),
+    the masks, the output contract's bounds and K, and the oracle itself on
+    a seeded subset (the fixture was written by the oracle and cross-checked
+    against the reference's CPU programs when it was made)."""
+    import hashlib
+    import struct
+    from rabung_amd import codec
+    rows, a4, doc = golden_data.block2_24()
+    primes = [r["p"] for r in rows]
+    assert primes == oracle.next_primes(1 << 24, 4096)
+    assert hashlib.sha256(b"".join(struct.pack("<I", p) for p in primes)).hexdigest() == \
+        doc["primes_sha256"]
+    assert sum(len(r["profiles"]) for r in rows) == doc["profiles"] == 13741
+    for row, mask_row, value in zip(rows, doc["rows"], a4):
+        p = row["p"]
+        assert mask_row[1] == codec.admissible_mask(p)
+        assert 0 <= value <= (p - 3) // 2
+        for v in row["profiles"]:
+            assert 1 <= v["prefix"] <= v["max"] <= (p - 1) // v["r"]
+            assert 1 <= v["witness"] <= (p - 1) // 2
+            assert v["K"] == codec.threshold_k(p, v["r"], v["max"], v["prefix"])
+    pick = sorted(random.Random(4096).sample(range(4096), 12))
+    recs = oracle.profile_many([primes[i] for i in pick])
+    assert oracle.rows(recs) == [rows[i] for i in pick]
+    assert [r["A4"] for r in recs] == [a4[i] for i in pick]

@@ -12,6 +12,7 @@ from pathlib import Path
 
 import pytest
 
+import a4_certificate
 import golden_data
 
 WORKER = Path(__file__).resolve().parent.parent / "rabung-atlas32_amd" / "rbg_worker"
@@ -59,8 +60,7 @@ def test_a4_protocol_matches_sample_frames():
     assert (h0[8], h0[9]) == (b0[0], b0[-1]) and (h1[8], h1[9]) == (b1[0], b1[-1])
     cert = json.loads(c1)
     assert h1[3] == len(c1) and cert["p"] == b1[5] and cert["quadruples"] == v1[5]
-    assert cert["hasse_lower"] <= cert["elliptic_points"] <= cert["hasse_upper"]
-    assert cert["elliptic_points"] % cert["order_lcm"] == 0
+    assert a4_certificate.verify(cert)
 
 
 @pytest.mark.parametrize("text", ["0 -1 2 7 5\nQUIT\n", "0 -1 2 7 9\nQUIT\n",
@@ -83,7 +83,7 @@ def test_profile_protocol_matches_sample_frames():
     b0, b1 = rows0[1000:1128], rows1[-40:]
     text = "0 %s\n1 %s\nQUIT\n" % (" ".join(str(r["p"]) for r in b0),
                                    " ".join(str(r["p"]) for r in b1))
-    res = run("profile", text)
+    res = run("profile", text, "--with-a4")
     assert res.returncode == 0, res.stderr
     lines = _profile_rows(res.stdout)
     meas = [x for x in lines if x["type"] == "measurement"]
@@ -107,15 +107,61 @@ def test_profile_protocol_refuses(text):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("block,count", [(0, 256), (49900, 130)])
-def test_recompute_sample_block(block, count):
-    """The recompute harness (the reference's recompute_small.py) on the
-    first 256 primes (2..1619) and on the block below 2^32."""
+@pytest.mark.parametrize("args,primes,frame", [
+    (["--block", "0", "--first", "256", "--route", "worker"], 256, None),
+    (["--block", "49900", "--sample", "130", "--seed", "7"], 130, None),
+    (["--block", "0", "--method", "plain"], 4096, True),
+])
+def test_recompute_sample_block(args, primes, frame):
+    """The recompute harness: the first 256 primes (2..1619) through the
+    worker protocols, a seeded sample of the block below 2^32 through the
+    library, and all of block 0 with its frame re-encoded byte-identically."""
     import sys
     root = Path(__file__).resolve().parent.parent
-    res = subprocess.run([sys.executable, "-m", "rabung_amd.recompute", "--count", str(count),
-                          "--block", str(block)], cwd=root / "rabung-atlas32_amd",
-                         capture_output=True, text=True, timeout=300)
+    res = subprocess.run([sys.executable, "-m", "rabung_amd.recompute", *args],
+                         cwd=root / "rabung-atlas32_amd", capture_output=True, text=True,
+                         timeout=300)
     assert res.returncode == 0, res.stderr[-2000:]
     out = json.loads(res.stdout.strip().splitlines()[-1])
-    assert out["status"] == "PASS" and out["primes"] == count
+    assert out["result"] == "identical" and out["differences"] == 0
+    assert out["primes"] == primes and out["frame_sha256_match"] == frame
+
+
+@pytest.mark.gpu
+def test_profile_capacity_refusal_replays_in_halves():
+    """A workspace budget (8 MB) below what a 64-prime batch near 2^32 needs
+    makes the library refuse with RBG_ECAPACITY; the worker replays the
+    batch in halves (the reference's replay rule, GPU_CONFIG.json:164-170)
+    and its output equals an unconstrained run's."""
+    rows, a4, _ = golden_data.sample_block(49900)
+    batch = rows[-64:]
+    text = "0 %s\nQUIT\n" % " ".join(str(r["p"]) for r in batch)
+    free = run("profile", text)
+    tight = run("profile", text, "--hbm-budget", "8000000")
+    assert free.returncode == 0 and tight.returncode == 0, tight.stderr
+    assert b"capacity refusal of 64 primes" in tight.stderr
+    assert b"capacity refusal" not in free.stderr
+
+    def profiles(res):
+        return [(m["p"], m["profiles"]) for m in _profile_rows(res.stdout)
+                if m["type"] == "measurement"]
+    assert profiles(tight) == profiles(free) == [
+        (r["p"], [{k: v[k] for k in ("r", "max", "prefix", "K", "witness")}
+                  for v in r["profiles"]]) for r in batch]
+
+
+@pytest.mark.gpu
+def test_profile_protocol_floor_and_a4_field():
+    """The default floor is the GPU worker's 1000 (worker32.cu:24); --floor 2
+    takes the CPU worker's small primes; A4 is reported only on request."""
+    res = run("profile", "0 101\nQUIT\n")
+    assert res.returncode == 2 and b"worker prime range" in res.stderr
+    res = run("profile", "0 101 1009\nQUIT\n", "--floor", "2", "--with-a4")
+    assert res.returncode == 0, res.stderr
+    meas = [m for m in _profile_rows(res.stdout) if m["type"] == "measurement"]
+    assert [m["p"] for m in meas] == [101, 1009] and all("A4" in m for m in meas)
+    rows0, a40, _ = golden_data.sample_block(0)
+    want = {r["p"]: v for r, v in zip(rows0, a40)}
+    assert [m["A4"] for m in meas] == [want[101], want[1009]]
+    res = run("profile", "0 1009\nQUIT\n")
+    assert all("A4" not in m for m in _profile_rows(res.stdout))
