@@ -143,6 +143,22 @@ int rbg_estimate_bytes(const rbg_ctx *ctx, const uint32_t *primes, uint32_t n,
 int rbg_profile_block_device(rbg_ctx *ctx, const uint32_t *d_primes, uint32_t n,
                              uint32_t flags, rbg_record *d_out);
 
+/* rbg_profile_block_device without any device allocation, for callers that
+ * cannot allocate (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  No counterpart in the reference.  Same
+ * inputs, outputs and results; every device buffer beyond d_primes and d_out
+ * is carved from a static array of 128 MiB in the library's code object,
+ * which each GPU holds from the first context opened on it until the process
+ * ends, and which keeps the last call's contents (the call clears what must
+ * start at zero).  Calls on one GPU share that array: they serialise on a
+ * per-device lock inside the process, so calls that overlap on one GPU wait
+ * for each other (the call returns only when its work is complete).  Returns
+ * RBG_ECAPACITY when the block needs more workspace than the array holds
+ * (about 90 primes near 2^32 with the batched method), RBG_EINVAL for the
+ * RBG_GRAPH flag; RBG_CERTIFY is ignored as in rbg_profile_block_device. */
+int rbg_profile_block_noalloc(rbg_ctx *ctx, const uint32_t *d_primes, uint32_t n,
+                              uint32_t flags, rbg_record *d_out);
+
 /* The wider arithmetic domain: primes 1000 <= p < 2^33 (and any smaller
  * prime).  Replaces the reference's 33-bit worker and engine
  * (reproducibility/gpu_method/source/worker33.cu:19-41 calculate_prime /

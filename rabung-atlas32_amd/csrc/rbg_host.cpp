@@ -67,6 +67,8 @@ extern "C" hipError_t rbg_sparse_filter(const SparsePrime *P, const SparseProf *
                                         uint32_t *failed, unsigned long long *stats,
                                         uint32_t *queue, int wide, uint32_t grid, uint32_t group,
                                         uint32_t extend_limit, hipStream_t stream);
+extern "C" size_t rbg_scratch_capacity(void);
+extern "C" hipError_t rbg_scratch_setup(int device, void **address);
 extern "C" hipError_t rbg_sparse_finalize(const SparsePrime *P, uint32_t nprimes,
                                           const SparseProf *prof, const unsigned long long *best,
                                           const uint32_t *a4, rbg_record *out, hipStream_t stream);
@@ -246,11 +248,27 @@ bool choose_cover(u64 p, u32 Q, u32 &B, u32 &D) {
     return true;
 }
 
+// Device memory carved from a caller-provided region instead of hipMalloc
+// (the allocation-free entry point): a bump pointer over [base, base + cap).
+struct Arena {
+    char *base = nullptr;
+    size_t cap = 0, used = 0;
+};
+
 struct DevBuf {
     void *ptr = nullptr;
     size_t bytes = 0;
+    Arena *arena = nullptr;  // set: carve from the arena, never allocate or free
     hipError_t ensure(size_t want) {
         if (want <= bytes) return hipSuccess;
+        if (arena) {
+            const size_t at = (arena->used + 255) & ~(size_t)255;
+            if (at > arena->cap || want > arena->cap - at) return hipErrorOutOfMemory;
+            ptr = arena->base + at;
+            bytes = want;
+            arena->used = at + want;
+            return hipSuccess;
+        }
         if (ptr) (void)hipFree(ptr);
         ptr = nullptr;
         bytes = 0;
@@ -260,7 +278,7 @@ struct DevBuf {
         return e;
     }
     void release() {
-        if (ptr) (void)hipFree(ptr);
+        if (ptr && !arena) (void)hipFree(ptr);
         ptr = nullptr;
         bytes = 0;
     }
@@ -309,6 +327,21 @@ struct rbg_ctx {
     u32 nprof32 = 0;  // profiles of primes below 2^32 (they come first)
     // timing of the last call
     rbg_timing timing{};
+    // the allocation-free entry point's scratch region while one of its calls
+    // runs (null otherwise)
+    Arena *arena = nullptr;
+    // this device's copy of the library's static scratch array (device
+    // address, resolved when the context is opened) or the reason it is not
+    // available
+    void *scratch = nullptr;
+    std::string scratch_error;
+
+    // every device buffer of the context
+    std::vector<DevBuf *> buffers() {
+        return {&descs, &tab_val, &tab_val64, &tab_code, &hdr, &runs, &recs, &flags,
+                &sp_primes, &sp_profs, &sp_val, &sp_val64, &sp_lab, &sp_labels, &sp_inv,
+                &sp_inv64, &sp_planes, &sp_best, &sp_failed, &sp_a4, &sp_stats, &sp_segs};
+    }
 
     int fail(int code, const std::string &msg) {
         error = msg;
@@ -319,6 +352,14 @@ struct rbg_ctx {
 namespace {
 
 int hip_fail(rbg_ctx *c, hipError_t e, const char *what) {
+    if (c->arena && e == hipErrorOutOfMemory) {
+        char msg[160];
+        snprintf(msg, sizeof msg,
+                 "the call needs more device workspace than the %zu-byte scratch of "
+                 "rbg_profile_block_noalloc",
+                 c->arena->cap);
+        return c->fail(RBG_ECAPACITY, msg);
+    }
     return c->fail(RBG_EDEVICE, std::string(what) + ": " + hipGetErrorString(e));
 }
 
@@ -1018,6 +1059,8 @@ rbg_ctx *rbg_open(const rbg_config *cfg) {
     c->cus = prop.multiProcessorCount;
     c->clock_khz = prop.clockRate;
     c->hbm_bytes = prop.totalGlobalMem;
+    const hipError_t se = rbg_scratch_setup(c->device, &c->scratch);
+    if (se != hipSuccess) c->scratch_error = hipGetErrorString(se);
     bool ok = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) == hipSuccess;
     for (int i = 0; i < kNumEv && ok; ++i) ok = hipEventCreate(&c->ev[i]) == hipSuccess;
     if (!ok) {
@@ -1164,6 +1207,57 @@ int rbg_profile_block_device(rbg_ctx *c, const uint32_t *d_primes, uint32_t n, u
     return run_block(c, wp.data(), n, flags & ~RBG_CERTIFY, d_out, nullptr);
 }
 
+// One lock per device: calls of the allocation-free entry point share that
+// device's scratch array and run one at a time.
+static std::mutex g_scratch_users[64];
+
+int rbg_profile_block_noalloc(rbg_ctx *c, const uint32_t *d_primes, uint32_t n, uint32_t flags,
+                              rbg_record *d_out) {
+    if (!c || (!d_primes && n) || (!d_out && n)) return RBG_EINVAL;
+    if (n > RBG_MAX_BLOCK) return c->fail(RBG_EINVAL, "more than 4096 primes in one block");
+    if (flags & RBG_GRAPH)
+        return c->fail(RBG_EINVAL, "graph capture is not available without allocation");
+    if (!c->scratch || c->device < 0 || c->device >= 64)
+        return c->fail(RBG_EDEVICE, "no scratch array on this device: " + c->scratch_error);
+    HIPCK(c, hipSetDevice(c->device));
+    std::vector<u32> h(n);
+    if (n) HIPCK(c, hipMemcpy(h.data(), d_primes, n * 4, hipMemcpyDeviceToHost));
+    const std::vector<u64> wp = widen(h.data(), n);
+    const size_t cap = rbg_scratch_capacity();
+    // refuse before any work when the planned workspace cannot fit (primes
+    // that the sparse engine refuses later need the dense workspace too,
+    // which the carving checks again)
+    const u32 tile_pos = (u32)rbg_waves_per_block() * tile_chunks(c) * 64u;
+    u64 need = 0;
+    for (u32 i = 0; i < n; ++i)
+        need += prime_bytes(wp[i], flags & RBG_METHOD_MASK, tile_pos, cover_q(c));
+    if (need > cap) {
+        char msg[160];
+        snprintf(msg, sizeof msg, "block needs about %llu bytes of device workspace; "
+                 "rbg_profile_block_noalloc has %zu", (unsigned long long)need, cap);
+        return c->fail(RBG_ECAPACITY, msg);
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_users[c->device]);
+    Arena arena;
+    arena.base = (char *)c->scratch;
+    arena.cap = cap;
+    std::vector<DevBuf *> bufs = c->buffers();
+    std::vector<DevBuf> kept;
+    for (DevBuf *b : bufs) {
+        kept.push_back(*b);
+        *b = DevBuf{};
+        b->arena = &arena;
+    }
+    c->arena = &arena;
+    int rc = run_block(c, wp.data(), n, flags & ~RBG_CERTIFY, d_out, nullptr);
+    // nothing of this call may still run on the scratch when the next one
+    // takes it
+    if (rc) (void)hipStreamSynchronize(c->stream);
+    for (size_t i = 0; i < bufs.size(); ++i) *bufs[i] = kept[i];
+    c->arena = nullptr;
+    return rc;
+}
+
 const char *rbg_last_error(const rbg_ctx *c) { return c ? c->error.c_str() : "no context"; }
 
 int rbg_last_timing(const rbg_ctx *c, rbg_timing *out) {
@@ -1183,11 +1277,7 @@ int rbg_device_info(const rbg_ctx *c, uint32_t *cus, uint32_t *clock_khz, uint64
 void rbg_close(rbg_ctx *c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
-    for (DevBuf *b : {&c->descs, &c->tab_val, &c->tab_val64, &c->tab_code, &c->hdr, &c->runs,
-                      &c->recs, &c->flags, &c->sp_primes, &c->sp_profs, &c->sp_val, &c->sp_val64,
-                      &c->sp_lab, &c->sp_labels, &c->sp_inv, &c->sp_inv64, &c->sp_planes,
-                      &c->sp_best, &c->sp_failed, &c->sp_a4, &c->sp_stats, &c->sp_segs})
-        b->release();
+    for (DevBuf *b : c->buffers()) b->release();
     for (int i = 0; i < kNumEv; ++i)
         if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
     for (hipGraphExec_t &x : c->gexec)

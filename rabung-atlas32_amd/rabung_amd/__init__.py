@@ -67,7 +67,8 @@ class Config(ctypes.Structure):
                 ("extend_limit", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
 
 
-EXPORTS = ["rbg_open", "rbg_profile_block", "rbg_profile_block_device", "rbg_profile_batch",
+EXPORTS = ["rbg_open", "rbg_profile_block", "rbg_profile_block_device",
+           "rbg_profile_block_noalloc", "rbg_profile_batch",
            "rbg_estimate_bytes", "rbg_autotune", "rbg_last_error",
            "rbg_last_timing", "rbg_device_info", "rbg_close", "rbg_mask", "rbg_is_prime",
            "rbg_version", "rbg_a4_host", "rbg_primes_from", "rbg_encode_scan_frame",
@@ -106,6 +107,9 @@ def load():
     lib.rbg_profile_block_device.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32,
                                              ctypes.c_uint32, ctypes.c_void_p]
     lib.rbg_profile_block_device.restype = ctypes.c_int
+    lib.rbg_profile_block_noalloc.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32,
+                                              ctypes.c_uint32, ctypes.c_void_p]
+    lib.rbg_profile_block_noalloc.restype = ctypes.c_int
     lib.rbg_profile_batch.argtypes = [ctypes.c_void_p, P(ctypes.c_uint32), ctypes.c_uint32,
                                       ctypes.c_uint32, P(Record)]
     lib.rbg_profile_batch.restype = ctypes.c_int
@@ -331,6 +335,16 @@ class Engine:
         """Device-resident variant: pointers to n uint32 primes and n records."""
         rc = self._lib.rbg_profile_block_device(self._ctx, ctypes.c_void_p(d_primes_ptr), n,
                                                 method, ctypes.c_void_p(d_out_ptr))
+        if rc:
+            raise RabungError(rc, self._lib.rbg_last_error(self._ctx).decode())
+
+    def profile_block_noalloc(self, d_primes_ptr, n, d_out_ptr, method=METHOD_BATCHED,
+                              graph=False):
+        """rbg_profile_block_noalloc: the device-resident call with every
+        workspace carved from the library's static scratch array."""
+        flags = method | (GRAPH if graph else 0)
+        rc = self._lib.rbg_profile_block_noalloc(self._ctx, ctypes.c_void_p(d_primes_ptr), n,
+                                                 flags, ctypes.c_void_p(d_out_ptr))
         if rc:
             raise RabungError(rc, self._lib.rbg_last_error(self._ctx).decode())
 
