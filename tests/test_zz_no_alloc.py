@@ -21,21 +21,62 @@ pytestmark = pytest.mark.gpu
 RECORD = ctypes.sizeof(rabung_amd.Record)
 
 
-def _device_block(primes):
-    import torch
-    host = np.asarray(primes, dtype=np.uint32).view(np.int32)
-    d_primes = torch.from_numpy(host.copy()).to("cuda:0")
-    return d_primes
+def _hip():
+    """The HIP runtime the library itself runs on (the libamdhip64 mapped
+    into this process once librabung is loaded): the test allocates its
+    input and output buffers with it, as a caller of the entry point would."""
+    rabung_amd.load()
+    with open("/proc/self/maps") as f:
+        paths = {line.split()[-1] for line in f if "libamdhip64" in line}
+    assert paths, "libamdhip64 not mapped"
+    hip = ctypes.CDLL(sorted(paths)[0])
+    hip.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+    hip.hipFree.argtypes = [ctypes.c_void_p]
+    hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    return hip
 
 
-def _records(eng, primes, method, noalloc):
-    import torch
-    d_primes = _device_block(primes)
-    d_out = torch.full((len(primes) * RECORD,), 0xA5, dtype=torch.uint8, device="cuda:0")
-    torch.cuda.synchronize()
-    call = eng.profile_block_noalloc if noalloc else eng.profile_block_device
-    call(d_primes.data_ptr(), len(primes), d_out.data_ptr(), method=method)
-    return d_out.cpu().numpy().tobytes()
+class DeviceBuffer:
+    """Caller-owned device memory (hipMalloc / hipMemcpy / hipFree)."""
+
+    H2D, D2H = 1, 2
+
+    def __init__(self, hip, nbytes):
+        self.hip, self.nbytes = hip, max(nbytes, 1)
+        self.ptr = ctypes.c_void_p()
+        assert hip.hipMalloc(ctypes.byref(self.ptr), self.nbytes) == 0
+
+    def write(self, data):
+        assert len(data) <= self.nbytes
+        assert self.hip.hipMemcpy(self.ptr, data, len(data), self.H2D) == 0
+
+    def read(self, nbytes):
+        out = ctypes.create_string_buffer(nbytes)
+        assert self.hip.hipMemcpy(out, self.ptr, nbytes, self.D2H) == 0
+        return out.raw
+
+    def free(self):
+        if self.ptr:
+            self.hip.hipFree(self.ptr)
+            self.ptr = ctypes.c_void_p()
+
+
+def _records(eng, primes, method, noalloc, graph=False):
+    hip = _hip()
+    n = len(primes)
+    d_primes, d_out = DeviceBuffer(hip, 4 * n), DeviceBuffer(hip, RECORD * n)
+    try:
+        d_primes.write(np.asarray(primes, dtype=np.uint32).tobytes())
+        d_out.write(b"\xa5" * (RECORD * n))  # stale bytes the call must overwrite
+        if noalloc:
+            eng.profile_block_noalloc(d_primes.ptr.value, n, d_out.ptr.value, method=method,
+                                      graph=graph)
+        else:
+            eng.profile_block_device(d_primes.ptr.value, n, d_out.ptr.value, method=method)
+        return d_out.read(RECORD * n)
+    finally:
+        d_primes.free()
+        d_out.free()
 
 
 def _top(n):
@@ -68,11 +109,8 @@ def test_noalloc_refuses_beyond_scratch_and_graph():
         with pytest.raises(rabung_amd.RabungError) as e:
             _records(eng, _top(256), rabung_amd.METHOD_BATCHED, noalloc=True)
         assert e.value.status == 4 and "rbg_profile_block_noalloc" in str(e.value)
-        import torch
-        d_primes = _device_block([101])
-        d_out = torch.zeros(RECORD, dtype=torch.uint8, device="cuda:0")
         with pytest.raises(rabung_amd.RabungError) as e:
-            eng.profile_block_noalloc(d_primes.data_ptr(), 1, d_out.data_ptr(), graph=True)
+            _records(eng, [101], rabung_amd.METHOD_BATCHED, noalloc=True, graph=True)
         assert e.value.status == 1
         # the context still works after the refusals
         small = [101, 1000003]
