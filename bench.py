@@ -62,39 +62,64 @@ BATCH_DESC = ("batching: 64 blocks of U{1..4096} consecutive primes from log-uni
               "[2^20, 2^32) (seed 2026), all admissible r, one profile_batch call")
 
 
-def bench_batch64(args):
-    """The cross-block batching case: one rbg_profile_batch call per step."""
+BATCH64_SHA256 = "f4ebfa462d8f7a50593e5a31f4265609788fdb4492f929ad2323afb7c5b1b8e1"
+
+
+def measure_batch64(steps, warmup):
+    """The cross-block batching workload: one rbg_profile_batch call per step
+    over the 64 seeded blocks (records returned as the C record array in host
+    memory).  Returns the measurement as a dict."""
+    import hashlib
+    import struct
     import torch
     import rabung_amd
     from rabung_amd import workloads
     blocks = workloads.batch_blocks(seed=2026, nblocks=64)
     flat = [p for b in blocks for p in b]
+    digest = hashlib.sha256(b"".join(struct.pack("<I", p) for p in flat)).hexdigest()
+    if digest != BATCH64_SHA256:
+        raise SystemExit("bench: the batch64 prime list differs from the committed hash")
     profiles = sum(bin(rabung_amd.mask(p)).count("1") for p in flat)
     with rabung_amd.Engine(0, hbm_budget=64 << 30) as eng:
-        for _ in range(args.warmup):
+        for _ in range(warmup):
             eng.profile_batch_raw(flat)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        for _ in range(args.steps):
+        for _ in range(steps):
             raw = eng.profile_batch_raw(flat)  # the C records, host buffers
         torch.cuda.synchronize()
-        dt = (time.perf_counter() - t0) / args.steps
+        dt = (time.perf_counter() - t0) / steps
         t = eng.last_timing()
+        first = [rabung_amd.record_to_dict(raw[i]) for i in range(len(blocks[0]))]
+        if eng.profile_block(blocks[0]) != first:
+            raise SystemExit("bench: batch64 records differ from the per-block call")
     recs = [rabung_amd.record_to_dict(raw[i]) for i in range(len(flat))]
     checked = golden_check(flat, recs)
-    line = {"metric": METRIC, "value": round(profiles / dt, 3), "unit": "profiles/s", "n_gpus": 1,
-            "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(dt * 1000.0, 3),
+    return {"workload": BATCH_DESC, "profiles_per_s": round(profiles / dt, 3),
+            "ms_per_step": round(dt * 1000.0, 3), "steps": steps, "warmup": warmup,
+            "blocks": len(blocks), "primes": len(flat), "profiles": profiles,
+            "primes_sha256": digest,
+            "records": "rbg_record array in host memory (the C-ABI output, no per-prime "
+                       "Python objects in the timed region)",
+            "sparse_primes": t["sparse_primes"], "dense_primes": t["dense_primes"],
+            "fallbacks": t["fallbacks"], "golden_checked": checked,
+            "stage_ms": {"prep": round(t["sparse_prep_ms"], 3),
+                         "seed": round(t["sparse_seed_ms"], 3),
+                         "filter": round(t["sparse_filter_ms"], 3),
+                         "dense": round(t["dense_scan_ms"], 3),
+                         "device_total": round(t["total_ms"], 3)}}
+
+
+def bench_batch64(args):
+    """The batching workload as the bench line's own workload."""
+    m = measure_batch64(args.steps, args.warmup)
+    line = {"metric": METRIC, "value": m["profiles_per_s"], "unit": "profiles/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": m["ms_per_step"],
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u32",
             "data": "synthetic: seeded prime blocks",
-            "config": {"workload": BATCH_DESC, "records": "rbg_record array in host memory "
-                       "(the C-ABI output, no per-prime Python objects in the timed region)",
-                       "blocks": len(blocks), "primes": len(flat),
-                       "profiles": profiles, "sparse_primes": t["sparse_primes"],
-                       "dense_primes": t["dense_primes"], "fallbacks": t["fallbacks"],
-                       "golden_checked": checked},
-            "stage_ms": {"prep": round(t["sparse_prep_ms"], 3), "seed": round(t["sparse_seed_ms"], 3),
-                         "filter": round(t["sparse_filter_ms"], 3),
-                         "dense": round(t["dense_scan_ms"], 3), "device_total": round(t["total_ms"], 3)}}
+            "config": {k: v for k, v in m.items() if k not in ("profiles_per_s", "ms_per_step",
+                                                               "stage_ms", "steps", "warmup")},
+            "stage_ms": m["stage_ms"]}
     print(json.dumps(line), flush=True)
     return 0
 
@@ -283,30 +308,95 @@ def golden_check(primes, records):
     return checked
 
 
-def cpu_baseline(primes, seconds_budget=30.0):
-    """The CPU oracle (a port of the reference's dense CPU worker) on a bounded
-    sample of the same workload: whole primes from the block, all host threads."""
+def host_threads():
+    """Host cores this process may run on (the box's CPU share)."""
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
+def _small_cpu_seconds(primes):
+    """Wall time of the reference's own CPU worker (src/census/small_cpu.cpp,
+    built from its sources into oracle/_ref) over `primes` through its line
+    protocol (batches of 128), with its output rows; None if not built."""
+    import subprocess
+    exe = ROOT / "oracle" / "_ref" / "small_cpu"
+    if not exe.exists():
+        return None, None
+    text = "".join("%d %s\n" % (k, " ".join(map(str, primes[i:i + 128])))
+                   for k, i in enumerate(range(0, len(primes), 128))) + "QUIT\n"
+    t0 = time.perf_counter()
+    done = subprocess.run([str(exe)], input=text, capture_output=True, text=True, timeout=300)
+    dt = time.perf_counter() - t0
+    if done.returncode:
+        return None, None
+    rows = [json.loads(x) for x in done.stdout.splitlines()]
+    return dt, [{"p": r["p"], "profiles": r["profiles"]} for r in rows if r["type"] == "measurement"]
+
+
+def cpu_baseline(eng, top):
+    """The CPU oracle (a port of the reference's dense CPU worker: discrete-log
+    walk + full-interval scan, threaded) timed on the host cores of this run,
+    with its records compared with the GPU's:
+      * headline: a seeded 8-prime subset of the top-256 workload;
+      * the first 256 primes (2..1619), also with the reference's own CPU
+        worker when it has been built from its sources;
+      * a seeded 256-prime sample of the 4096-prime block from 2^24.
+    """
     sys.path.insert(0, str(ROOT / "oracle"))
     import oracle
     import random
-    threads = min(16, os.cpu_count() or 1)
-    rng = random.Random(2026)
-    sample = []
-    done_profiles = 0
-    t0 = time.perf_counter()
-    pool = list(primes)
-    rng.shuffle(pool)
-    for p in pool:
-        rec = oracle.profile(p, threads)
-        sample.append(p)
-        done_profiles += bin(rec["mask"]).count("1")
-        if time.perf_counter() - t0 > seconds_budget * 0.5:
-            break
-    dt = time.perf_counter() - t0
-    return {"value": done_profiles / dt, "unit": "profiles/s", "cores": threads, "kind": "port",
-            "sample": "%d prime(s) of the workload (%s), %d profiles, %.1f s, dense "
-                      "discrete-log walk + full-interval scan" % (
-                          len(sample), ",".join(map(str, sample)), done_profiles, dt)}
+    from rabung_amd import codec
+    threads = host_threads()
+
+    def timed(primes, many):
+        t0 = time.perf_counter()
+        recs = oracle.profile_many(primes, threads) if many else \
+            [oracle.profile(p, threads) for p in primes]
+        dt = time.perf_counter() - t0
+        if eng.profile_block(primes) != recs:
+            raise SystemExit("bench: CPU oracle and GPU records differ")
+        return recs, dt
+
+    def nprof(recs):
+        return sum(bin(r["mask"]).count("1") for r in recs)
+
+    print("bench: CPU baseline, 8 seeded top-of-range primes on %d threads" % threads,
+          file=sys.stderr, flush=True)
+    pick = sorted(random.Random(5).sample(top, 8))
+    recs, dt = timed(pick, many=False)
+    out = {"value": round(nprof(recs) / dt, 4), "unit": "profiles/s", "cores": threads,
+           "kind": "port",
+           "sample": "seeded 8 of the 256 largest primes below 2^32 (%s), %d profiles in %.1f s "
+                     "on %d threads (oracle: dense discrete-log walk + full-interval scan, "
+                     "threaded over each prime's walk); records equal to the GPU's"
+                     % (",".join(map(str, pick)), nprof(recs), dt, threads),
+           "os_cpu_count": os.cpu_count(), "threads_used": threads, "matches_gpu": True}
+    print("bench: CPU baseline, first 256 primes and a 2^24 sample", file=sys.stderr, flush=True)
+    c1 = oracle.next_primes(2, 256)
+    recs1, dt1 = timed(c1, many=True)
+    out["first_256_primes"] = {"profiles": nprof(recs1), "seconds": round(dt1, 4),
+                               "profiles_per_s": round(nprof(recs1) / dt1, 1), "kind": "port",
+                               "matches_gpu": True}
+    ref_dt, ref_rows = _small_cpu_seconds(c1)
+    if ref_dt is not None:
+        if ref_rows != codec.records_to_rows(recs1):
+            raise SystemExit("bench: the reference CPU worker disagrees on the first 256 primes")
+        out["first_256_primes_reference_worker"] = {
+            "seconds": round(ref_dt, 4), "profiles_per_s": round(nprof(recs1) / ref_dt, 1),
+            "kind": "reference", "cores": 1, "matches_gpu": True,
+            "program": "src/census/small_cpu.cpp built from its sources (oracle/_ref/small_cpu), "
+                       "process start included"}
+    block = oracle.next_primes(1 << 24, 4096)
+    sample = sorted(random.Random(24).sample(block, 256))
+    recs2, dt2 = timed(sample, many=True)
+    out["block_2_24_sample"] = {"primes": len(sample), "profiles": nprof(recs2),
+                                "seconds": round(dt2, 3),
+                                "profiles_per_s": round(nprof(recs2) / dt2, 2),
+                                "full_block_seconds_estimate": round(dt2 * len(block) / len(sample), 1),
+                                "kind": "port", "cores": threads, "matches_gpu": True}
+    return out
 
 
 def calibrate_copy(nbytes=256 << 20, reps=3):
@@ -358,6 +448,8 @@ def main():
                     choices=sorted(WORKLOADS) + ["batch64", "holdout", "wide33"])
     ap.add_argument("--no-plain", action="store_true", help="skip timing the plain method")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-batch64", action="store_true",
+                    help="skip the batching workload's sub-measurement")
     ap.add_argument("--serve-seconds", type=float, default=10.0,
                     help="length of the Poisson serving run (0 skips it)")
     ap.add_argument("--qps", type=float, default=20.0, help="serving arrival rate")
@@ -415,6 +507,8 @@ def main():
         dt = time.perf_counter() - t0
         return dt, timings
 
+    print("bench: batched method, %d primes, %d steps" % (n, args.steps), file=sys.stderr,
+          flush=True)
     dt, timings = run(rabung_amd.METHOD_BATCHED, args.steps, args.warmup)
     timing = timings[-1]
     raw = d_out.cpu().numpy().tobytes()
@@ -436,7 +530,8 @@ def main():
              "montgomery_products_per_s": round(rabung_amd.microbench(4), 0)}
     plain = None
     if not args.no_plain:
-        psteps = max(1, args.steps // 3)
+        psteps = max(1, args.steps // 10)
+        print("bench: plain method, %d step(s)" % psteps, file=sys.stderr, flush=True)
         pdt, ptimings = run(rabung_amd.METHOD_PLAIN, psteps, 0)
         raw2 = d_out.cpu().numpy().tobytes()
         if raw2 != raw:
@@ -493,6 +588,7 @@ def main():
                     "algorithmic_ops_per_launch": ops, "peak_basis": peak_basis}
     serving = None
     if args.serve_seconds > 0:
+        print("bench: serving", file=sys.stderr, flush=True)
         from rabung_amd import serve, workloads
         stream = workloads.serving_stream(seed=4, qps=args.qps, seconds=args.serve_seconds)
         with serve.Server(slots=2, chunk=args.serve_chunk, max_primes=args.serve_max) as srv:
@@ -506,7 +602,11 @@ def main():
                                "at most %d primes" % (args.qps, args.serve_seconds, args.serve_chunk,
                                                       args.serve_max),
                    **{k: (round(v, 3) if isinstance(v, float) else v) for k, v in summ.items()}}
-    cpu = None if args.no_cpu else cpu_baseline(primes)
+    batch64 = None
+    if not args.no_batch64:
+        print("bench: batch64 workload", file=sys.stderr, flush=True)
+        batch64 = measure_batch64(max(1, args.steps // 10), 1)
+    cpu = None if args.no_cpu else cpu_baseline(eng, primes)
     line = {
         "metric": METRIC, "value": round(value, 3), "unit": "profiles/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 3),
@@ -519,6 +619,7 @@ def main():
         "plain": plain,
         "batched_vs_plain": round(value / plain["profiles_per_s"], 2) if plain else None,
         "roofline": roofline,
+        "batch64": batch64,
         "serving": serving,
         "cpu_baseline": cpu,
     }
