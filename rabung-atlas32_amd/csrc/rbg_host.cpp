@@ -739,12 +739,12 @@ int sparse_plan(rbg_ctx *c, const u64 *primes, const std::vector<u32> &idx_in,
                 f.prime = (u32)c->h_sp.size();
                 f.r = r;
                 f.np = 32u - (u32)__builtin_clz(r - 1);
-                f.wpp = (2u * B + 1u + 160u + 31u) / 32u + 2u;
-                f.plane_off = (u32)words;
+                f.wpp = rbg::row_words(1, B);
+                f.plane_off = (u32)words;  // units of 16 words
                 f.sigma0 = (((p - 1) / 2) % r) == 0;
                 f.pblk = (u32)pblk;
-                words += (u64)f.np * f.wpp;
-                pblk += ((f.wpp + 1ull) / 2ull + kPlanePairsPerBlock - 1ull) / kPlanePairsPerBlock;
+                words += rbg::row_offset(rbg::kRows, B, f.np) / 16u;
+                for (u32 row = 0; row < rbg::kRows; ++row) pblk += rbg::row_blocks(row, B);
                 c->h_prof.push_back(f);
                 ++s.nprof;
                 if (!wide) ++c->nprof32;
@@ -774,8 +774,9 @@ int sparse_run(rbg_ctx *c, rbg_record *recs, u32 *err, std::vector<u32> &failed_
     }
     const u32 segs_per_prof = rbg_seed_segments((u32)bmax);
     for (const SparseProf &f : c->h_prof) {
-        words += (u64)f.np * f.wpp;
-        pblk += ((f.wpp + 1ull) / 2ull + kPlanePairsPerBlock - 1ull) / kPlanePairsPerBlock;
+        const u32 B = c->h_sp[f.prime].B;
+        words += rbg::row_offset(rbg::kRows, B, f.np);  // 32-bit words
+        for (u32 row = 0; row < rbg::kRows; ++row) pblk += rbg::row_blocks(row, B);
     }
     const u64 bytes = np * sizeof(SparsePrime) + nf * sizeof(SparseProf) + c->h_sval.size() * 4 +
                       c->h_sval64.size() * 8 + c->h_slab.size() * 2 + lab * 2 + inv * 4 +
@@ -903,7 +904,7 @@ u64 prime_bytes(u64 p, u32 flags, u32 tile_pos, u32 q) {
         bytes += 2ull * (B + 1) + vbytes * (D + 1) + 88;
         for (u32 r = 2; r <= 9; ++r)
             if (mask & (1u << (r - 2)))
-                bytes += 4ull * (32u - (u32)__builtin_clz(r - 1)) * ((2u * B + 192u) / 32u + 2u) + 48;
+                bytes += 4ull * rbg::row_offset(rbg::kRows, B, 32u - (u32)__builtin_clz(r - 1)) + 48;
         return bytes;
     }
     const u64 tiles = ((p - 1) / 2 + tile_pos - 1) / tile_pos;

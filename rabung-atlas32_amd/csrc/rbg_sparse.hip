@@ -173,25 +173,34 @@ struct SmallMod {
     __device__ __forceinline__ u32 operator()(u32 x) const { return x - r * __umulhi(x, g); }
 };
 
-// Colour bit planes of one order over the signed table: bit i of plane k is
-// bit k of cl(i - B).  One wave builds 64 bits (two words) of every plane per
-// step; lane q keeps the words of step q, and after 64 steps the lanes store
-// them together (one 512-byte store per plane word).
+// Colour rows of one order (the full signed table and its five wheel
+// decimations, rbg_sparse.hpp): bit k of plane b of a row is bit b of the
+// colour of the numerator y = rho + g k - B, cl(-y) = cl(y) + cl(-1), and 0
+// for numerators outside [-B, B] and for y = 0.  One wave builds 64 bits
+// (two words) of every plane of one row per step; lane q keeps the words of
+// step q, and after 64 steps the lanes store them together (one 512-byte
+// store per plane word).
 __global__ __launch_bounds__(256) void sp_planes(const SparsePrime *__restrict__ P, u32 nprimes,
                                                  const SparseProf *__restrict__ prof, u32 nprof,
                                                  const u16 *__restrict__ labels,
                                                  u32 *__restrict__ planes) {
     // block -> profile through the per-profile block offsets kept in pblk of
-    // the profile records (searched linearly from the prime's first profile)
+    // the profile records (searched linearly from the prime's first
+    // profile), then -> row by the rows' block counts
     const u32 pi = __builtin_amdgcn_readfirstlane(find_prime(P, nprimes, blockIdx.x, 2));
     const SparsePrime s = P[pi];
     u32 k = s.prof_begin;
     while (k + 1 < s.prof_begin + s.nprof && prof[k + 1].pblk <= blockIdx.x) ++k;
     const SparseProf f = prof[k];
+    u32 local = blockIdx.x - f.pblk, row = 0;
+    while (row + 1 < kRows && local >= row_blocks(row, s.B)) local -= row_blocks(row++, s.B);
+    row = __builtin_amdgcn_readfirstlane(row);
+    const u32 g = row_g(row), wpr = row_words(g, s.B);
+    const int64_t rho = row_rho(row, s.B);
     const u32 lane = threadIdx.x & 63u;
     const u32 wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const u32 pairs = (f.wpp + 1) / 2;
-    const u32 first = ((blockIdx.x - f.pblk) * 4u + wave) * kPlanePairsPerWave;
+    const u32 pairs = wpr / 2;
+    const u32 first = (local * 4u + wave) * kPlanePairsPerWave;
     u64 mine[4] = {0, 0, 0, 0};  // this lane's step: words of planes 0..np-1
     const SmallMod mod_r(f.r), mod_m(s.m);
     const u32 nq = first < pairs ? min(kPlanePairsPerWave, pairs - first) : 0u;
@@ -204,7 +213,7 @@ __global__ __launch_bounds__(256) void sp_planes(const SparsePrime *__restrict__
         int64_t yv[8];
 #pragma unroll
         for (u32 u = 0; u < 8u; ++u) {
-            yv[u] = (int64_t)(first + q0 + u) * 64 + lane - (int64_t)s.B;
+            yv[u] = rho + (int64_t)g * ((int64_t)(first + q0 + u) * 64 + lane) - (int64_t)s.B;
             const u32 a = (u32)(yv[u] < 0 ? -yv[u] : yv[u]);
             jv[u] = lab[(yv[u] != 0 && a <= s.B) ? a : 1u];
         }
@@ -224,10 +233,11 @@ __global__ __launch_bounds__(256) void sp_planes(const SparsePrime *__restrict__
     }
     const u32 pair = first + lane;
     if (pair < pairs) {
+        u32 *base = planes + (u64)f.plane_off * 16u + row_offset(row, s.B, f.np);
         for (u32 b = 0; b < f.np; ++b) {
-            u32 *pl = planes + f.plane_off + (u64)b * f.wpp;
+            u32 *pl = base + (u64)b * wpr;
             pl[2 * pair] = (u32)mine[b];
-            if (2 * pair + 1 < f.wpp) pl[2 * pair + 1] = (u32)(mine[b] >> 32);
+            pl[2 * pair + 1] = (u32)(mine[b] >> 32);
         }
     }
 }
@@ -397,60 +407,78 @@ __device__ __forceinline__ u64 window64(const u32 *pl, u32 i) {
 
 constexpr u32 kXcds = 8;  // XCDs of an MI355X
 
+// A task's rows (wave-uniform).  Denominator d tests only anchors b coprime
+// to g = gcd(d, 6) (the others share a factor with d and are tested with a
+// smaller denominator), so its anchors fall into the classes c mod g with
+// gcd(c, g) = 1: one class for g = 1 (every b, the full row) and g = 2
+// (odd b), two for g = 3 (b = 1, 2 mod 3) and g = 6 (b = 1, 5 mod 6).  An
+// anchor b of class ci is row bit k = (b + B - rho_ci) / g, and b - j d is
+// row bit k - j d / g.  Plane pb of the first class's row starts at
+// plb[pb]; the second class's row lies `cstride` bits further in every
+// plane, so row bit k of class ci is bit k + ci cstride from plb[pb] and
+// every window of the task is read through the same plane pointers.
+struct TaskRows {
+    u32 g, dg, nc;      // decimation, d / g, classes
+    u32 cstride;        // bits from the first class's row to the second's
+    u32 rho[2];         // (c + B) mod g of each class
+    u32 nbits[2];       // row bits that carry colours: numerators in [-B, B]
+};
+
 // Bilateral screen (every placement of a q-window containing the anchor;
 // reference lemma GPU_METHOD_EN.tex:457-480, vdw_cut::screen): with
 // h = q - t extra positions needed, a candidate anchor is kept iff it has
 // L matching positions to its left and R to its right with L + R >= h.  In
 // numerator space (negated, so equality is unchanged) position x-k reads
-// b + k d and position x+t-1+k reads b - (t-1+k) d; numerators outside
-// [-B, B] are unknown and never reject.  R is kept as a bit-sliced count
-// (5 planes, h <= 31; h <= 24 for q <= 48, t = ceil(q/2)) so that every
-// candidate of the word is screened at once.
+// b + k d and position x+t-1+k reads b - (t-1+k) d; in row space these are
+// row bits kk + k dg and kk - (t-1+k) dg.  Bits outside the class's
+// colours [start, lim) (numerators outside [-B, B]) are unknown and never
+// reject.  R is kept as a bit-sliced count (5 planes, h <= 31; the cover
+// cap Q <= 63 and t = ceil(q/2) keep h there) so that every candidate of
+// the word is screened at once.
 template <int NP>
-__device__ __forceinline__ u64 match_left(const u32 *const *plb, const u64 *base, u32 b0, u32 B,
-                                          u32 d, u32 k, u32 &n_windows) {
-    const int64_t lo = (int64_t)b0 + (int64_t)k * d, Bs = B;
-    if (lo > Bs) return ~0ull;
+__device__ __forceinline__ u64 match_left(const u32 *const *plb, const u64 *base, u32 kk, u32 lim,
+                                          u32 dg, u32 k, u32 &n_windows) {
+    const u32 lo = kk + k * dg;
+    if (lo >= lim) return ~0ull;
     u64 mm = 0;
-    const u32 idx = (u32)(lo + Bs);
 #pragma unroll
-    for (int pb = 0; pb < NP; ++pb) mm |= base[pb] ^ window64(plb[pb], idx);
+    for (int pb = 0; pb < NP; ++pb) mm |= base[pb] ^ window64(plb[pb], lo);
     n_windows += 2u * NP;  // counted in 32-anchor plane dwords
-    const int64_t known = Bs - lo + 1;  // bits 0..known-1 lie in the table
-    if (known < 64) mm &= (1ull << known) - 1ull;
+    const u32 known = lim - lo;  // bits 0..known-1 carry colours
+    if (known < 64u) mm &= (1ull << known) - 1ull;
     return ~mm;
 }
 
 template <int NP>
-__device__ __forceinline__ u64 match_right(const u32 *const *plb, const u64 *base, u32 b0, u32 B,
-                                           u32 d, u32 t, u32 k, u32 &n_windows) {
-    const int64_t lo = (int64_t)b0 - (int64_t)(t - 1 + k) * d, Bs = B;
-    if (lo + 63 < -Bs) return ~0ull;
+__device__ __forceinline__ u64 match_right(const u32 *const *plb, const u64 *base, u32 kk,
+                                           u32 start, u32 dg, u32 t, u32 k, u32 &n_windows) {
+    const int64_t lo = (int64_t)kk - (int64_t)(t - 1 + k) * dg;
+    if (lo + 63 < (int64_t)start) return ~0ull;
     u64 mm = 0;
-    if (lo >= -Bs) {
-        const u32 idx = (u32)(lo + Bs);
+    if (lo >= (int64_t)start) {
 #pragma unroll
-        for (int pb = 0; pb < NP; ++pb) mm |= base[pb] ^ window64(plb[pb], idx);
+        for (int pb = 0; pb < NP; ++pb) mm |= base[pb] ^ window64(plb[pb], (u32)lo);
     } else {
-        const u32 shift = (u32)(-Bs - lo);  // 1..63 low bits are below -B: unknown
+        const u32 shift = (u32)((int64_t)start - lo);  // 1..63 low bits lie before the row
 #pragma unroll
-        for (int pb = 0; pb < NP; ++pb) mm |= base[pb] ^ (window64(plb[pb], 0) << shift);
+        for (int pb = 0; pb < NP; ++pb) mm |= base[pb] ^ (window64(plb[pb], start) << shift);
         mm &= ~((1ull << shift) - 1ull);
     }
     n_windows += 2u * NP;
     return ~mm;
 }
 
-// Screen of the candidates `cand` of the word at b0 (the anchors that passed
-// every stage); candidates outside `cand` are never reported, so the counts
-// follow `cand` only and the loops end as soon as those runs end.
+// Screen of the candidates `cand` of the word at row bit kk (the anchors
+// that passed every stage); candidates outside `cand` are never reported,
+// so the counts follow `cand` only and the loops end as soon as those runs
+// end.
 template <int NP>
-__device__ __forceinline__ u64 screen(const u32 *const *plb, const u64 *base, u64 cand, u32 b0,
-                                      u32 B, u32 d, u32 t, u32 h, u32 &n_windows) {
+__device__ __forceinline__ u64 screen(const u32 *const *plb, const u64 *base, u64 cand, u32 kk,
+                                      u32 start, u32 lim, u32 dg, u32 t, u32 h, u32 &n_windows) {
     // R: consecutive right matches per candidate, bit-sliced (r0 = LSB)
     u64 r0 = 0, r1 = 0, r2 = 0, r3 = 0, r4 = 0, cr = cand;
     for (u32 k = 1; k <= h; ++k) {
-        cr &= match_right<NP>(plb, base, b0, B, d, t, k, n_windows);
+        cr &= match_right<NP>(plb, base, kk, start, dg, t, k, n_windows);
         if (!cr) break;
         u64 c = cr, x;
         x = r0 & c; r0 ^= c; c = x;
@@ -462,7 +490,7 @@ __device__ __forceinline__ u64 screen(const u32 *const *plb, const u64 *base, u6
     u64 pass = 0, cl = cand;
     for (u32 k = 0; k <= h; ++k) {
         if (k) {
-            cl &= match_left<NP>(plb, base, b0, B, d, k, n_windows);
+            cl &= match_left<NP>(plb, base, kk, lim, dg, k, n_windows);
             if (!cl) break;
         }
         // candidates with R >= h - k (compare the sliced count with a constant)
@@ -491,13 +519,15 @@ template <class D>
 struct FilterCtx {
     const SparsePrime *s;
     const SparseProf *f;
-    const u32 *pl;
+    const u32 *pl;    // the full row, plane 0 (completion reads colours there)
+    const u32 *rows;  // plane 0 of the task's first class row
     const typename D::V *tab_val;
     const u16 *tab_lab;
     unsigned long long *best;
     u32 *failed;
     typename D::V dinv;
-    u32 d, t, q, B, bmin, wpp, ext_limit;
+    u32 d, t, q, B, bmin, wpr, ext_limit;
+    TaskRows R;
 };
 
 // Scheduled anchors of one task as spans.  The schedule keeps the anchors
@@ -509,7 +539,7 @@ struct FilterCtx {
 //   t even: S(m) = [m d + 1, m d + d - 1]; c = d/2 (d even) is not scheduled;
 //   t odd:  S(m) = [m d + d/2 + 1, (m+1) d + (d-1)/2]; (m+1) d is not.
 // At stage j every span reads the span j d below it, so consecutive words of
-// a span are consecutive in the colour planes at every stage.  For d <= 2
+// a span are consecutive in the colour rows at every stage.  For d <= 2
 // one span [max(bmin, t or t-1), B] holds every anchor.
 struct Spans {
     u32 s_lo, width, excl;  // span offset and width; excluded offset (~0u: none)
@@ -544,29 +574,38 @@ __device__ __forceinline__ Spans spans_of(u32 d, u32 t, u32 sp, u32 phi, u32 bmi
     return S;
 }
 
-// Words are 96 consecutive anchors of a span: a stage reads one 96-bit
-// window per plane, at any bit offset, with one four-dword load.
+// Words are 96 consecutive bits of one class row (96 anchors b, b + g, ...,
+// b + 95 g of one span): a stage reads one 96-bit window per plane, at any
+// bit offset, with one four-dword load.
 constexpr u32 kWordBits = 96;
 constexpr u32 kQueue = 128;  // entries of a wave's queue of passing words (two per word at most)
 
-// Screen and complete the queued words (one lane per word).
+// Screen and complete the queued words (one lane per entry: a row bit kk and
+// the 64 anchors from it that passed every stage).
 template <class D, int NP>
-__device__ __forceinline__ void drain_queue(const FilterCtx<D> &c, const u32 *const *plb, u32 qn, const u32 *qb,
-                            const u64 *qs, u32 lane, u32 &n_windows, u32 &n_done) {
+__device__ __forceinline__ void drain_queue(const FilterCtx<D> &c, const u32 *const *plb, u32 qn,
+                                            const u32 *qb, const u64 *qs, u32 lane,
+                                            u32 &n_windows, u32 &n_done) {
+    const TaskRows &R = c.R;
     for (u32 e = lane; e < qn; e += 64u) {
-        const u32 b0 = qb[e];
+        const u32 kk = qb[e];
         u64 surv = qs[e];
         if (!surv) continue;
+        const u32 ci = kk >= R.cstride && R.nc > 1 ? 1u : 0u;
+        const u32 start = ci ? R.cstride : 0u;
         if (c.q > c.t) {
             u64 base[NP];
 #pragma unroll
-            for (int pb = 0; pb < NP; ++pb) base[pb] = window64(plb[pb], b0 + c.B);
-            surv &= screen<NP>(plb, base, surv, b0, c.B, c.d, c.t, c.q - c.t, n_windows);
+            for (int pb = 0; pb < NP; ++pb) base[pb] = window64(plb[pb], kk);
+            surv &= screen<NP>(plb, base, surv, kk, start, start + R.nbits[ci], R.dg, c.t,
+                               c.q - c.t, n_windows);
         }
+        // anchor of bit 0: b0 = g (kk - start) + rho - B
+        const u32 b0 = R.g * (kk - start) + R.rho[ci] - c.B;
         while (surv) {
             const u32 bit = (u32)__builtin_ctzll(surv);
             surv &= surv - 1ull;
-            complete_anchor<D>(c.s, c.f, c.pl, b0 + bit, c.d, c.dinv, c.t, c.tab_val,
+            complete_anchor<D>(c.s, c.f, c.pl, b0 + R.g * bit, c.d, c.dinv, c.t, c.tab_val,
                                c.tab_lab, c.best, c.failed, c.ext_limit);
             ++n_done;
         }
@@ -584,22 +623,56 @@ __device__ __forceinline__ void wave_lds_fence() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Per class of a task: where its words lie and which of their bits are
+// anchors (wave-uniform).  Every span holds the same anchors of the class
+// relative to its start (g | d), so a word is named by (span, w) and
+//   k0 = klo + span * spk + 96 w     (row bit of its first anchor),
+// with `n` anchors of the class per span in W words; bits at row bits
+// below kmin (anchors below bmin) or above kmax (above B) are not anchors,
+// nor is bit exb of word exw (the span's unscheduled point).
+struct ClassWords {
+    u32 n, W, klo, kmin, kmax, exw, exb, start;
+};
+
+__device__ __forceinline__ ClassWords class_words(const Spans &S, const TaskRows &R, u32 ci,
+                                                  u32 d, u32 B, u32 bmin) {
+    const u32 g = R.g;
+    const u32 cls = g == 1 ? 0u : (g == 2 ? 1u : (ci ? g - 1u : 1u));  // 1, or 1 and g-1
+    const u32 delta = (cls + g - S.s_lo % g) % g;  // first anchor of the class in a span
+    ClassWords w;
+    w.n = S.width > delta ? (S.width - 1u - delta) / g + 1u : 0u;
+    w.W = (w.n + kWordBits - 1u) / kWordBits;
+    w.start = ci * R.cstride;
+    const u32 rho = R.rho[ci];
+    w.klo = w.start + (S.m_first * d + S.s_lo + delta + B - rho) / g;
+    w.kmin = w.start + (bmin + B - rho + g - 1u) / g;
+    w.kmax = w.start + (2u * B - rho) / g;
+    w.exw = ~0u;
+    w.exb = 0;
+    if (S.excl != ~0u && S.excl >= delta && (S.excl - delta) % g == 0) {
+        const u32 i = (S.excl - delta) / g;
+        w.exw = i / kWordBits;
+        w.exb = i % kWordBits;
+    }
+    return w;
+}
+
 // Every scheduled word of one (profile, denominator) task for NP planes.
 //
-// Each lane holds one word (96 consecutive anchors b) and tests it G stages
-// per step: for stages j the 96-bit windows at numerators b - j d must equal
-// the base window at b.  The base words of a new word are loaded when the
-// lane takes it and aligned only after the step's stage loads are issued,
-// so taking a word costs no extra memory round trip.  A lane whose word
-// dies or passes takes the next word at once (words are handed out in order
-// with a ballot), so the lanes stay busy however differently the words end.
-// Passing words (anchors that survived every stage) go to a per-wave LDS
-// queue, one entry per half (anchors b0..b0+63, b0+64..b0+95) that has
+// Each lane holds one word (96 anchors of one class row) and tests it G
+// stages per step: for stages j the 96-bit windows at row bits k - j dg must
+// equal the base window at k.  The base words of a new word are loaded when
+// the lane takes it and aligned only after the step's stage loads are
+// issued, so taking a word costs no extra memory round trip.  A lane whose
+// word dies or passes takes the next word at once (words are handed out in
+// order with a ballot), so the lanes stay busy however differently the
+// words end.  Passing words (anchors that survived every stage) go to a
+// per-wave LDS queue, one entry per half (anchors 0..63, 64..95) that has
 // survivors; the queue is screened and completed with all lanes at once
 // outside the word loop, at the end of the task or when it is full (the
 // lanes whose words did not fit keep them and the word loop stops handing
 // out words until the queue has been drained).  Stages past t-1 re-read
-// stage t-1, which keeps every address inside the table.
+// stage t-1, which keeps every address inside the row.
 constexpr u32 kRefill = 48;  // smallest cohort of idle lanes that takes new words
 
 template <class D, int NP>
@@ -609,42 +682,30 @@ __device__ void filter_task(const FilterCtx<D> &c, u32 sp, u32 phi, u32 *qb, u64
     constexpr u32 NW = 4;  // dwords per window load
     constexpr int G = NP == 1 ? 6 : NP == 2 ? 3 : NP == 3 ? 2 : 1;  // stages per step (register budget)
     const u32 lane = threadIdx.x & 63u;
-    const u32 d = c.d, t = c.t, B = c.B, bmin = c.bmin;
-    const Spans S = spans_of(d, t, sp, phi, bmin, B);
-    const u32 W = (S.width + WB - 1u) / WB;  // words per span
-    const u32 total = S.nsb * W;  // < 2^23: at most B/96 + nsb words
-    const float rW = 1.0f / (float)max(W, 1u);
-    const u32 span_step = sp * d;
+    const TaskRows &R = c.R;
+    const u32 t = c.t, dg = R.dg;
+    const Spans S = spans_of(c.d, t, sp, phi, c.bmin, c.B);
+    const ClassWords C0 = class_words(S, R, 0, c.d, c.B, c.bmin);
+    ClassWords C1 = C0;
+    C1.W = 0;
+    if (R.nc > 1) C1 = class_words(S, R, 1, c.d, c.B, c.bmin);
+    const u32 Wsum = C0.W + C1.W;  // words per span
+    const u32 total = S.nsb * Wsum;  // < 2^23: at most 2B/96 + 2 nsb words
+    const float rW = 1.0f / (float)max(Wsum, 1u);
+    const u32 spk = sp * dg;  // row bits from one span to the next
     const u32 *plb[NP];
 #pragma unroll
-    for (int pb = 0; pb < NP; ++pb) plb[pb] = c.pl + (u32)pb * c.wpp;
-    // Wheel: anchors b with 2 | b (d even) or 3 | b (3 | d) share a factor
-    // with d and are tested with a smaller denominator.  Every word start b0
-    // = m_first d + s_lo + span sp d + 96 w is = s_lo mod 2 when d is even
-    // and mod 3 when 3 | d, so the wheel is one mask per task (bit i of
-    // dword k stands for b0 + 32 k + i; 32 = 2 mod 3).
-    u32 wheel[3] = {~0u, ~0u, ~0u};
-    if ((d & 1u) == 0) {
-        const u32 par = (S.s_lo & 1u) ? 0x55555555u : 0xAAAAAAAAu;
-        wheel[0] = wheel[1] = wheel[2] = par;
-    }
-    if (d % 3u == 0) {
-        const u32 r = S.s_lo % 3u;
-#pragma unroll
-        for (u32 k = 0; k < 3; ++k) wheel[k] &= ~(0x49249249u << ((6u - r - 2u * k) % 3u));
-    }
-    // the span's unscheduled point lies in word excl / 96 of every span
-    const u32 ex_w = S.excl == ~0u ? ~0u : S.excl / WB, ex_b = S.excl % WB;
+    for (int pb = 0; pb < NP; ++pb) plb[pb] = c.rows + (u32)pb * c.wpr;
     u32 next = 0, qn = 0;
     bool pend = false;  // a passing word that did not fit the queue
-    u32 b0 = 0;
-    u64 live = 0;    // surviving anchors b0 .. b0+63
-    u32 live_hi = 0; // surviving anchors b0+64 .. b0+95
+    u32 k0 = 0;
+    u64 live = 0;    // surviving anchors 0 .. 63 of the word
+    u32 live_hi = 0; // surviving anchors 64 .. 95
     for (;;) {
         bool stall = false;
         bool have = false;
         u32 j = 1, sh = 0;
-        int io = 0, io_last = 0;  // table index of stage j and of stage t-1
+        int io = 0, io_last = 0;  // row bit of stage j and of stage t-1
         u32 braw[NP][NW];
 #pragma unroll
         for (int pb = 0; pb < NP; ++pb)
@@ -661,29 +722,32 @@ __device__ void filter_task(const FilterCtx<D> &c, u32 sp, u32 phi, u32 *qb, u64
                     const u32 my = next + lanes_below(idle);
                     if (my < total) {
                         // exact quotient: the float estimate is at most one
-                        // low (|error| < 1/W because total < 2^23)
+                        // low (|error| < 1/Wsum because total < 2^23)
                         u32 span = (u32)((float)my * rW);
-                        if ((span + 1u) * W <= my) ++span;
-                        const u32 w = my - span * W;
-                        const u32 lo = S.m_first * d + S.s_lo + span * span_step;
-                        b0 = lo + WB * w;
-                        // anchors [first, last] of the word in the span and
-                        // in [bmin, B], as offsets from b0
-                        const u32 first_b = max(max(lo, bmin), b0);
-                        const u32 last_b = min(min(lo + S.width - 1u, B), b0 + WB - 1u);
-                        const u32 first = first_b - b0, last = last_b - b0;
+                        if ((span + 1u) * Wsum <= my) ++span;
+                        const u32 rem = my - span * Wsum;
+                        const bool second = rem >= C0.W;
+                        const ClassWords &C = second ? C1 : C0;
+                        const u32 w = second ? rem - C0.W : rem;
+                        k0 = C.klo + span * spk + WB * w;
+                        // anchors of the word: row bits [lo_k, hi_k], the
+                        // class's anchors of the span clipped to [kmin, kmax]
+                        // (compared as row bits first, so the offsets below
+                        // are only formed for a word that holds anchors)
+                        const u32 nvalid = min(WB, C.n - WB * w);
+                        const u32 lo_k = max(k0, C.kmin);
+                        const u32 hi_k = min(k0 + nvalid - 1u, C.kmax);
                         u64 valid = 0;
                         u32 valid_hi = 0;
-                        if (first_b <= last_b) {
+                        if (lo_k <= hi_k) {
+                            const u32 first = lo_k - k0, last = hi_k - k0;
                             if (first < 64u)
                                 valid = (~0ull << first) & (~0ull >> (63u - min(last, 63u)));
                             if (last >= 64u)
                                 valid_hi = (~0u << (max(first, 64u) - 64u)) & (~0u >> (95u - last));
-                            valid &= ((u64)wheel[1] << 32) | wheel[0];
-                            valid_hi &= wheel[2];
-                            if (w == ex_w) {
-                                if (ex_b < 64u) valid &= ~(1ull << ex_b);
-                                else valid_hi &= ~(1u << (ex_b - 64u));
+                            if (w == C.exw) {
+                                if (C.exb < 64u) valid &= ~(1ull << C.exb);
+                                else valid_hi &= ~(1u << (C.exb - 64u));
                             }
                         }
                         if (valid | valid_hi) {
@@ -691,11 +755,12 @@ __device__ void filter_task(const FilterCtx<D> &c, u32 sp, u32 phi, u32 *qb, u64
                             live = valid;
                             live_hi = valid_hi;
                             j = 1;
-                            const u32 ib = b0 + B;
-                            io = (int)ib - (int)d;
-                            io_last = (int)ib - (int)((t - 1u) * d);
-                            sh = ib & 31u;
-                            const u32 wo = ib >> 5;
+                            io = (int)k0 - (int)dg;
+                            // anchors are >= bmin, so stage t-1 stays in the
+                            // row; bits below bmin only need an address in it
+                            io_last = max((int)k0 - (int)((t - 1u) * dg), (int)C.start);
+                            sh = k0 & 31u;
+                            const u32 wo = k0 >> 5;
 #pragma unroll
                             for (int pb = 0; pb < NP; ++pb)
 #pragma unroll
@@ -717,7 +782,7 @@ __device__ void filter_task(const FilterCtx<D> &c, u32 sp, u32 phi, u32 *qb, u64
                 u32 raw[G][NP][NW], shg[G];
 #pragma unroll
                 for (int g = 0; g < G; ++g) {
-                    const u32 ig = (u32)max(io - g * (int)d, io_last);
+                    const u32 ig = (u32)max(io - g * (int)dg, io_last);
                     shg[g] = ig & 31u;
 #pragma unroll
                     for (int pb = 0; pb < NP; ++pb) {
@@ -744,7 +809,7 @@ __device__ void filter_task(const FilterCtx<D> &c, u32 sp, u32 phi, u32 *qb, u64
                 live &= ~mism;
                 live_hi &= ~mism_hi;
                 j += G;
-                io -= G * (int)d;
+                io -= G * (int)dg;
                 if (!(live | live_hi)) {
                     have = false;
                 } else if (j >= t) {
@@ -753,7 +818,7 @@ __device__ void filter_task(const FilterCtx<D> &c, u32 sp, u32 phi, u32 *qb, u64
                 }
             }
             // a passing word takes one queue entry per half with survivors:
-            // anchors b0 .. b0+63 and b0+64 .. b0+95
+            // anchors 0 .. 63 (row bit k0) and 64 .. 95 (row bit k0 + 64)
             const u64 fin_lo = ballot64(fin && live != 0);
             const u64 fin_hi = ballot64(fin && live_hi != 0);
             if (fin_lo | fin_hi) {
@@ -762,11 +827,11 @@ __device__ void filter_task(const FilterCtx<D> &c, u32 sp, u32 phi, u32 *qb, u64
                     const u32 need = (live != 0 ? 1u : 0u) + (live_hi != 0 ? 1u : 0u);
                     if (slot + need <= kQueue) {
                         if (live) {
-                            qb[slot] = b0;
+                            qb[slot] = k0;
                             qs[slot++] = live;
                         }
                         if (live_hi) {
-                            qb[slot] = b0 + 64u;
+                            qb[slot] = k0 + 64u;
                             qs[slot] = live_hi;
                         }
                     } else {
@@ -792,11 +857,11 @@ __device__ void filter_task(const FilterCtx<D> &c, u32 sp, u32 phi, u32 *qb, u64
             if (pend) {
                 u32 slot = lanes_below(pm_lo) + lanes_below(pm_hi);
                 if (live) {
-                    qb[slot] = b0;
+                    qb[slot] = k0;
                     qs[slot++] = live;
                 }
                 if (live_hi) {
-                    qb[slot] = b0 + 64u;
+                    qb[slot] = k0 + 64u;
                     qs[slot] = live_hi;
                 }
                 pend = false;
@@ -821,6 +886,56 @@ __device__ __forceinline__ u32 next_task(u32 &k, u32 &chunk, u32 claim) {
         k = 0;
     }
     return chunk + k;
+}
+
+// Context of one (profile, denominator) task; false when the task has
+// nothing to do (d beyond the prime's cover, or a threshold below 3).
+template <class D>
+__device__ __forceinline__ bool task_setup(const SparsePrime *P, const SparseProf *fg, u32 dd,
+                                           const u32 *planes, const typename D::V *inv,
+                                           const typename D::V *tab_val, const u16 *tab_lab,
+                                           unsigned long long *best_slot, u32 *failed,
+                                           u32 ext_limit, FilterCtx<D> &c, u32 &sp, u32 &phi) {
+    const SparsePrime *sg = P + fg->prime;
+    if (dd > sg->D) return false;
+    const u64 key = __hip_atomic_load(best_slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const u32 q = min((u32)(key >> 32), sg->Q);
+    if (q < 3u) return false;
+    u32 t = max(3u, (q + 1u) / 2u);
+    if (t > q) t = q;
+    sp = q - t + 1u;  // schedule period
+    phi = ((t - 1u) / 2u) % sp;
+    c.s = sg;
+    c.f = fg;
+    // rows of g = gcd(d, 6): the full row for g = 1, else the class rows
+    const u32 B = sg->B, np = fg->np;
+    const u32 wg = ((dd & 1u) ? 1u : 2u) * (dd % 3u ? 1u : 3u);
+    const u32 row0 = first_row_of(wg);
+    const u32 *prof_rows = planes + (u64)fg->plane_off * 16u;
+    c.pl = prof_rows;
+    c.rows = prof_rows + row_offset(row0, B, np);
+    c.wpr = row_words(wg, B);
+    c.R.g = wg;
+    c.R.dg = dd / wg;
+    c.R.nc = wg >= 3 ? 2u : 1u;
+    c.R.cstride = np * c.wpr * 32u;
+    for (u32 ci = 0; ci < 2; ++ci) {
+        const u32 row = row0 + (ci < c.R.nc ? ci : 0u);
+        c.R.rho[ci] = row_rho(row, B);
+        c.R.nbits[ci] = row_bits(row, B);
+    }
+    c.tab_val = tab_val;
+    c.tab_lab = tab_lab;
+    c.best = best_slot;
+    c.failed = failed;
+    c.d = dd;
+    c.dinv = inv[(u64)sg->inv_off + dd];
+    c.t = t;
+    c.q = q;
+    c.B = B;
+    c.bmin = ((t - 1u) * dd + 1u) / 2u;
+    c.ext_limit = ext_limit;
+    return true;
 }
 
 // D = Dom32 for the profiles of primes below 2^32, Dom33 for the wider
@@ -884,32 +999,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void s
         const u32 local = g * kGroup + rem % kGroup;
         if (local >= nprof_x) continue;
         const u32 kk = __builtin_amdgcn_readfirstlane(xcd + local * kXcds);
-        const SparseProf *fg = prof + kk;
-        const SparsePrime *sg = P + fg->prime;
-        if (dd > sg->D) continue;
-        const u64 key = __hip_atomic_load(&best[kk], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        const u32 q = min((u32)(key >> 32), sg->Q);
-        if (q < 3u) continue;
-        u32 t = max(3u, (q + 1u) / 2u);
-        if (t > q) t = q;
-        const u32 sp = q - t + 1u;  // schedule period
-        const u32 phi = ((t - 1u) / 2u) % sp;
         FilterCtx<D> c;
-        c.s = sg;
-        c.f = fg;
-        c.pl = planes + fg->plane_off;
-        c.wpp = fg->wpp;
-        c.tab_val = tab_val;
-        c.tab_lab = tab_lab;
-        c.best = &best[kk];
-        c.failed = failed;
-        c.d = dd;
-        c.dinv = inv[(u64)sg->inv_off + dd];
-        c.t = t;
-        c.q = q;
-        c.B = sg->B;
-        c.bmin = ((t - 1u) * dd + 1u) / 2u;
-        c.ext_limit = ext_limit;
+        u32 sp = 0, phi = 0;
+        if (!task_setup<D>(P, prof + kk, dd, planes, inv, tab_val, tab_lab, &best[kk], failed,
+                           ext_limit, c, sp, phi))
+            continue;
+        const SparseProf *fg = c.f;
         switch (fg->np) {
         case 1: filter_task<D, 1>(c, sp, phi, qb, qs, n_windows, n_words, n_done); break;
         case 2: filter_task<D, 2>(c, sp, phi, qb, qs, n_windows, n_words, n_done); break;

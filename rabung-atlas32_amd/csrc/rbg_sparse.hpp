@@ -27,11 +27,52 @@ struct SparseProf {
     u32 prime;      // index into the SparsePrime array
     u32 r;          // order
     u32 np;         // colour bit planes (bits of r-1)
-    u32 wpp;        // 32-bit words per plane
-    u32 plane_off;  // first word of plane 0
+    u32 wpp;        // 32-bit words per plane of the full row (row 0)
+    u32 plane_off;  // first word of the profile's rows, in units of 16 words
     u32 prefix;     // initial run of the identity colour (seed kernel)
     u32 sigma0;     // r | (p-1)/2
     u32 pblk;       // first plane block
 };
+
+// Colour rows of one profile.  Row 0 is the full signed table: bit i of
+// plane b is bit b of the colour of the numerator i - B, i in [0, 2B].
+// Rows 1-5 decimate it by the wheel: row (g, c) holds the numerators
+// a = c (mod g), c coprime to g, in increasing order -- bit k is the full
+// row's bit rho + g k with rho = (c + B) mod g.  A denominator d with
+// gcd(d, 6) = g > 1 tests only anchors b coprime to g (the others share a
+// factor with d), and b - j d stays in b's class, so a word of 96
+// consecutive bits of one row holds 96 anchors, all live, and every stage
+// window is 96 consecutive bits of the same row.  Each row is padded by 256
+// bits (a four-dword load at its last bit stays inside) and every row
+// starts on a 16-word boundary; the two rows of one g have equal sizes, so
+// the second class lies a fixed number of bits after the first.
+constexpr u32 kRows = 6;
+
+RBG_HD u32 row_g(u32 row) { return row == 0 ? 1u : row == 1 ? 2u : row <= 3 ? 3u : 6u; }
+
+RBG_HD u32 row_class(u32 row) { return row == 3 ? 2u : row == 5 ? 5u : row == 0 ? 0u : 1u; }
+
+RBG_HD u32 row_rho(u32 row, u32 B) { return (row_class(row) + B) % row_g(row); }
+
+// class numerators in [-B, B]: row bits that carry colours
+RBG_HD u32 row_bits(u32 row, u32 B) { return (2u * B - row_rho(row, B)) / row_g(row) + 1u; }
+
+// 32-bit words of one plane of a row of decimation g (a multiple of 16)
+RBG_HD u32 row_words(u32 g, u32 B) { return ((2u * B / g + 1u + 256u + 31u) / 32u + 15u) & ~15u; }
+
+// words from a profile's first word to plane 0 of `row`
+RBG_HD u32 row_offset(u32 row, u32 B, u32 np) {
+    u32 off = 0;
+    for (u32 r = 0; r < row; ++r) off += np * row_words(row_g(r), B);
+    return off;
+}
+
+// blocks of the plane kernel for one row (kPlanePairsPerBlock 64-bit steps each)
+RBG_HD u32 row_blocks(u32 row, u32 B) {
+    return (row_words(row_g(row), B) / 2u + kPlanePairsPerBlock - 1u) / kPlanePairsPerBlock;
+}
+
+// first row of decimation g and the number of its classes
+RBG_HD u32 first_row_of(u32 g) { return g == 1 ? 0u : g == 2 ? 1u : g == 3 ? 2u : 4u; }
 
 }  // namespace rbg

@@ -134,6 +134,90 @@ def span_words(d, t, sp, phi, bmin, B, word=96):
     return got
 
 
+ROWS = [(1, 0), (2, 1), (3, 1), (3, 2), (6, 1), (6, 5)]  # (g, class) of rows 0..5
+
+
+def row_layout(B, np_):
+    """Row sizes of one profile as the kernels lay them out (rbg_sparse.hpp):
+    per row (g, class, rho, colour bits, words per plane, first word)."""
+    out, off = [], 0
+    for g, c in ROWS:
+        rho = (c + B) % g
+        words = ((2 * B // g + 1 + 256 + 31) // 32 + 15) // 16 * 16
+        out.append({"g": g, "c": c, "rho": rho, "bits": (2 * B - rho) // g + 1,
+                    "words": words, "off": off})
+        off += np_ * words
+    return out
+
+
+def class_row_words(d, t, sp, phi, bmin, B, np_=1, word=96):
+    """The filter's words over class rows (class_words and the word hand-out
+    of rbg_sparse.hip, integer for integer): for g = gcd(d, 6), each span's
+    anchors of a class c coprime to g are cut into words of `word`
+    consecutive row bits; word w of span `span` starts at row bit
+    klo + span * sp * (d / g) + word * w, bits below kmin / above kmax and
+    the unscheduled point are cleared.  Returns {anchor b: (class index,
+    row bit k)} for every valid bit, k counted from the first class's row
+    (the second class lies cstride bits further)."""
+    g = (1 if d % 2 else 2) * (1 if d % 3 else 3)
+    row0 = {1: 0, 2: 1, 3: 2, 6: 4}[g]
+    nc = 2 if g >= 3 else 1
+    lay = row_layout(B, np_)
+    cstride = np_ * lay[row0]["words"] * 32
+    dg = d // g
+    if d <= 2:
+        lo_b = max(bmin, t if d == 1 else t - 1)
+        s_lo, width, excl, m_first, nsb = lo_b, (B - lo_b + 1 if lo_b <= B else 0), None, 0, \
+            (1 if lo_b <= B else 0)
+    else:
+        odd_t = t % 2 == 1
+        rm = (phi + sp - 1) % sp if odd_t else phi
+        s_lo = d // 2 + 1 if odd_t else 1
+        width = d + (d - 1) // 2 - d // 2 if odd_t else d - 1
+        excl = d - s_lo if odd_t else (d // 2 - 1 if d % 2 == 0 else None)
+        x = bmin + 1 - s_lo - width
+        m_lo = 0 if x <= 0 else (x + d - 1) // d
+        m_first = m_lo + (rm + sp - m_lo % sp) % sp
+        m_hi = (B - s_lo) // d
+        nsb = 0 if m_first > m_hi else (m_hi - m_first) // sp + 1
+    classes = []
+    for ci in range(nc):
+        cls = 0 if g == 1 else (1 if g == 2 else (g - 1 if ci else 1))
+        rho = lay[row0 + ci]["rho"]
+        assert rho == (cls + B) % g
+        delta = (cls + g - s_lo % g) % g
+        n = (width - 1 - delta) // g + 1 if width > delta else 0
+        start = ci * cstride
+        ex = None
+        if excl is not None and excl >= delta and (excl - delta) % g == 0:
+            ex = divmod((excl - delta) // g, word)
+        classes.append({"n": n, "W": (n + word - 1) // word, "start": start, "rho": rho,
+                        "klo": start + (m_first * d + s_lo + delta + B - rho) // g,
+                        "kmin": start + (bmin + B - rho + g - 1) // g,
+                        "kmax": start + (2 * B - rho) // g, "ex": ex})
+    wsum = sum(c["W"] for c in classes)
+    got = {}
+    for my in range(nsb * wsum):
+        span, rem = divmod(my, wsum)
+        ci = 1 if rem >= classes[0]["W"] else 0
+        C = classes[ci]
+        w = rem - (classes[0]["W"] if ci else 0)
+        k0 = C["klo"] + span * sp * dg + word * w
+        nvalid = min(word, C["n"] - word * w)
+        first = C["kmin"] - k0 if C["kmin"] > k0 else 0
+        if C["kmax"] < k0:
+            continue
+        last = min(nvalid - 1, C["kmax"] - k0)
+        for i in range(first, last + 1):
+            if C["ex"] == (w, i):
+                continue
+            k = k0 + i
+            b = g * (k - C["start"]) + C["rho"] - B
+            assert b not in got
+            got[b] = (ci, k)
+    return got, {"g": g, "dg": dg, "cstride": cstride, "row0": row0, "layout": lay}
+
+
 def anchors(ranges):
     got = set()
     for lo, hi, *ex in ranges:

@@ -89,3 +89,41 @@ def test_word_masks_keep_the_scheduled_anchors_coprime_to_the_wheel():
                         if not (d % 2 == 0 and b % 2 == 0) and not (d % 3 == 0 and b % 3 == 0)}
                 got = sparse_model.span_words(d, t, sp, phi, bmin, B)
                 assert got == want, (p, q, d)
+
+
+def test_class_row_words_hold_the_scheduled_anchors():
+    """The filter's words over the wheel's class rows (one row per class c
+    mod g coprime to g = gcd(d, 6)) keep exactly the scheduled anchors that
+    gcd(b, d) in {2, 3} does not exclude; every anchor's row bit maps back to
+    it, and its stage windows b - j d (j < t) are row bits k - j d/g inside
+    its class's colours (numerators >= -B), as are the screen's reads up to
+    q - t positions beyond."""
+    import random
+    rng = random.Random(6)
+    for p in (65537 + 2, 1000003, 16777259, 4294967291):
+        B, D = sparse_model.cover(p)
+        # d = 1, 2 (one span over [bmin, B]) only where B is small
+        ds = sorted(set(range(1 if B < 40000 else 3, 30)) | {D, D - 1, D - 2, D - 3, D - 4, D - 5} |
+                    {rng.randint(3, D) for _ in range(5)})
+        for q in ((3, 4, 17, 24, 25, 48) if p < 1 << 30 else (3, 24, 25, 48)):
+            t = min(q, max(3, (q + 1) // 2))
+            sp = q - t + 1
+            phi = ((t - 1) // 2) % sp
+            for d in ds:
+                bmin = ((t - 1) * d + 1) // 2
+                want = {b for b in sparse_model.anchors(
+                            sparse_model.span_ranges(d, t, sp, phi, bmin, B))
+                        if not (d % 2 == 0 and b % 2 == 0) and not (d % 3 == 0 and b % 3 == 0)}
+                for np_ in ((1, 3) if d % 6 == 0 else (1,)):
+                    got, info = sparse_model.class_row_words(d, t, sp, phi, bmin, B, np_)
+                    assert set(got) == want, (p, q, d, np_)
+                    g, dg, lay = info["g"], info["dg"], info["layout"]
+                    for b, (ci, k) in got.items():
+                        row = lay[info["row0"] + ci]
+                        start = ci * info["cstride"]
+                        assert (b + B - row["rho"]) % g == 0 and (b + B - row["rho"]) // g == k - start
+                        # stage t-1 and the screen's right reads stay in the row
+                        low = k - (t - 1) * dg
+                        assert low >= start, (p, q, d)
+                        assert k - start < row["bits"]
+                        assert row["words"] * 32 >= row["bits"] + 256
