@@ -173,13 +173,13 @@ struct SmallMod {
     __device__ __forceinline__ u32 operator()(u32 x) const { return x - r * __umulhi(x, g); }
 };
 
-// Colour rows of one order (the full signed table and its five wheel
-// decimations, rbg_sparse.hpp): bit k of plane b of a row is bit b of the
-// colour of the numerator y = rho + g k - B, cl(-y) = cl(y) + cl(-1), and 0
-// for numerators outside [-B, B] and for y = 0.  One wave builds 64 bits
-// (two words) of every plane of one row per step; lane q keeps the words of
-// step q, and after 64 steps the lanes store them together (one 512-byte
-// store per plane word).
+// Full colour row (row 0) of one order: bit k of plane b is bit b of the
+// colour of the numerator y = k - B, cl(-y) = cl(y) + cl(-1), and 0 for
+// numerators outside [-B, B] and for y = 0.  One wave builds 64 bits (two
+// words) of every plane per step; lane q keeps the words of step q, and
+// after 64 steps the lanes store them together (one 512-byte store per plane
+// word).  The kernel is written for any row (g, rho) but launched blocks of
+// the decimated rows return at once: sp_rows copies their bits out of row 0.
 __global__ __launch_bounds__(256) void sp_planes(const SparsePrime *__restrict__ P, u32 nprimes,
                                                  const SparseProf *__restrict__ prof, u32 nprof,
                                                  const u16 *__restrict__ labels,
@@ -195,6 +195,7 @@ __global__ __launch_bounds__(256) void sp_planes(const SparsePrime *__restrict__
     u32 local = blockIdx.x - f.pblk, row = 0;
     while (row + 1 < kRows && local >= row_blocks(row, s.B)) local -= row_blocks(row++, s.B);
     row = __builtin_amdgcn_readfirstlane(row);
+    if (row != 0) return;  // the decimated rows come from row 0 (sp_rows)
     const u32 g = row_g(row), wpr = row_words(g, s.B);
     const int64_t rho = row_rho(row, s.B);
     const u32 lane = threadIdx.x & 63u;
@@ -238,6 +239,66 @@ __global__ __launch_bounds__(256) void sp_planes(const SparsePrime *__restrict__
             u32 *pl = base + (u64)b * wpr;
             pl[2 * pair] = (u32)mine[b];
             pl[2 * pair + 1] = (u32)(mine[b] >> 32);
+        }
+    }
+}
+
+// 32 bits of a decimated row from the full row: bit b of the result is bit
+// g b of the stream s[0], s[1], ... (the row-0 bits from rho on, already
+// aligned by alignbit).
+template <u32 G>
+__device__ __forceinline__ u32 every_gth_bit(const u32 *s) {
+    u32 out = 0;
+#pragma unroll
+    for (u32 b = 0; b < 32; ++b) {
+        constexpr u32 kStride = G;
+        const u32 j = kStride * b;
+        out |= ((s[j >> 5] >> (j & 31u)) & 1u) << b;
+    }
+    return out;
+}
+
+template <u32 G>
+__device__ __forceinline__ u32 decimated_word(const u32 *full, u32 wfull, u32 rho, u32 w) {
+    // row bit 32 w + b is row-0 bit rho + G (32 w + b) = 32 G w + rho + G b
+    u32 in[G + 1];
+#pragma unroll
+    for (u32 k = 0; k <= G; ++k) in[k] = G * w + k < wfull ? full[G * w + k] : 0u;
+    u32 s[G];
+#pragma unroll
+    for (u32 k = 0; k < G; ++k) s[k] = __builtin_amdgcn_alignbit(in[k + 1], in[k], rho);
+    return every_gth_bit<G>(s);
+}
+
+// The five decimated rows of every profile (rbg_sparse.hpp) from its full
+// row: row (g, c) bit k is row-0 bit rho + g k.  Same grid and block map as
+// sp_planes; blocks of row 0 return at once, the others write 512 words of
+// each plane (two per thread).
+__global__ __launch_bounds__(256) void sp_rows(const SparsePrime *__restrict__ P, u32 nprimes,
+                                               const SparseProf *__restrict__ prof,
+                                               u32 *__restrict__ planes) {
+    const u32 pi = __builtin_amdgcn_readfirstlane(find_prime(P, nprimes, blockIdx.x, 2));
+    const SparsePrime s = P[pi];
+    u32 k = s.prof_begin;
+    while (k + 1 < s.prof_begin + s.nprof && prof[k + 1].pblk <= blockIdx.x) ++k;
+    const SparseProf f = prof[k];
+    u32 local = blockIdx.x - f.pblk, row = 0;
+    while (row + 1 < kRows && local >= row_blocks(row, s.B)) local -= row_blocks(row++, s.B);
+    row = __builtin_amdgcn_readfirstlane(row);
+    if (row == 0) return;
+    const u32 g = row_g(row), rho = row_rho(row, s.B);
+    const u32 wpr = row_words(g, s.B), wfull = row_words(1, s.B);
+    u32 *prof_rows = planes + (u64)f.plane_off * 16u;
+    for (u32 half = 0; half < 2; ++half) {
+        const u32 w = local * 2u * kPlanePairsPerBlock + half * 256u + threadIdx.x;
+        if (w >= wpr) break;
+        for (u32 b = 0; b < f.np; ++b) {
+            const u32 *full = prof_rows + (u64)b * wfull;
+            u32 v;
+            if (g == 2) v = decimated_word<2>(full, wfull, rho, w);
+            else if (g == 3) v = decimated_word<3>(full, wfull, rho, w);
+            else v = decimated_word<6>(full, wfull, rho, w);
+            prof_rows[row_offset(row, s.B, f.np) + (u64)b * wpr + w] = v;
         }
     }
 }
@@ -1079,6 +1140,8 @@ extern "C" hipError_t rbg_sparse_prepare(const SparsePrime *P, uint32_t nprimes,
                        inv, inv64);
     hipLaunchKernelGGL(rbg::sp_planes, dim3(plane_blocks), dim3(256), 0, stream, P, nprimes, prof,
                        nprof, labels, planes);
+    hipLaunchKernelGGL(rbg::sp_rows, dim3(plane_blocks), dim3(256), 0, stream, P, nprimes, prof,
+                       planes);
     return hipGetLastError();
 }
 
