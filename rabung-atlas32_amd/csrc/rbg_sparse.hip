@@ -737,7 +737,7 @@ __device__ __forceinline__ ClassWords class_words(const Spans &S, const TaskRows
 constexpr u32 kRefill = 48;  // smallest cohort of idle lanes that takes new words
 
 template <class D, int NP>
-__device__ void filter_task(const FilterCtx<D> &c, u32 sp, u32 phi, u32 *qb, u64 *qs,
+__device__ void filter_task(const FilterCtx<D> &c, u32 sp, u32 phi, u32 *qb, u64 *qs, u32 *cw,
                             u32 &n_windows, u32 &n_words, u32 &n_done) {
     constexpr u32 WB = kWordBits;
     constexpr u32 NW = 4;  // dwords per window load
@@ -746,11 +746,31 @@ __device__ void filter_task(const FilterCtx<D> &c, u32 sp, u32 phi, u32 *qb, u64
     const TaskRows &R = c.R;
     const u32 t = c.t, dg = R.dg;
     const Spans S = spans_of(c.d, t, sp, phi, c.bmin, c.B);
-    const ClassWords C0 = class_words(S, R, 0, c.d, c.B, c.bmin);
-    ClassWords C1 = C0;
-    C1.W = 0;
-    if (R.nc > 1) C1 = class_words(S, R, 1, c.d, c.B, c.bmin);
-    const u32 Wsum = C0.W + C1.W;  // words per span
+    // both classes' word parameters go to the wave's LDS table (a lane reads
+    // its class's eight words when it takes a word), only W0 stays in a
+    // register
+    u32 W0, Wsum;
+    {
+        const ClassWords C0 = class_words(S, R, 0, c.d, c.B, c.bmin);
+        ClassWords C1 = C0;
+        C1.W = 0;
+        if (R.nc > 1) C1 = class_words(S, R, 1, c.d, c.B, c.bmin);
+        W0 = C0.W;
+        Wsum = C0.W + C1.W;  // words per span
+        if (lane < 2u) {
+            const ClassWords &C = lane ? C1 : C0;
+            u32 *row = cw + 8u * lane;
+            row[0] = C.n;
+            row[1] = C.klo;
+            row[2] = C.kmin;
+            row[3] = C.kmax;
+            row[4] = C.exw;
+            row[5] = C.exb;
+            row[6] = C.start;
+            row[7] = 0;
+        }
+        wave_lds_fence();
+    }
     const u32 total = S.nsb * Wsum;  // < 2^23: at most 2B/96 + 2 nsb words
     const float rW = 1.0f / (float)max(Wsum, 1u);
     const u32 spk = sp * dg;  // row bits from one span to the next
@@ -780,6 +800,10 @@ __device__ void filter_task(const FilterCtx<D> &c, u32 sp, u32 phi, u32 *qb, u64
             const u32 nidle = (u32)__builtin_popcountll(idle);
             if (!stall && idle && next < total && (nidle >= kRefill || nidle == 64u)) {
                 if (!have && !pend) {
+                    // the stages this lane's last word took (t - 1 when it
+                    // passed) are counted now, once per word
+                    n_windows += (u32)NP * (WB / 32u) * min(j - 1u, t - 1u);
+                    j = 1;
                     const u32 my = next + lanes_below(idle);
                     if (my < total) {
                         // exact quotient: the float estimate is at most one
@@ -787,9 +811,18 @@ __device__ void filter_task(const FilterCtx<D> &c, u32 sp, u32 phi, u32 *qb, u64
                         u32 span = (u32)((float)my * rW);
                         if ((span + 1u) * Wsum <= my) ++span;
                         const u32 rem = my - span * Wsum;
-                        const bool second = rem >= C0.W;
-                        const ClassWords &C = second ? C1 : C0;
-                        const u32 w = second ? rem - C0.W : rem;
+                        const bool second = rem >= W0;
+                        const uint4 *cp = (const uint4 *)(cw + (second ? 8u : 0u));
+                        const uint4 ca = cp[0], cb = cp[1];
+                        ClassWords C;
+                        C.n = ca.x;
+                        C.klo = ca.y;
+                        C.kmin = ca.z;
+                        C.kmax = ca.w;
+                        C.exw = cb.x;
+                        C.exb = cb.y;
+                        C.start = cb.z;
+                        const u32 w = second ? rem - W0 : rem;
                         k0 = C.klo + span * spk + WB * w;
                         // anchors of the word: row bits [lo_k, hi_k], the
                         // class's anchors of the span clipped to [kmin, kmax]
@@ -866,7 +899,6 @@ __device__ void filter_task(const FilterCtx<D> &c, u32 sp, u32 phi, u32 *qb, u64
                     for (int g = 0; g < G; ++g)
                         mism_hi |= base_hi ^ __builtin_amdgcn_alignbit(raw[g][pb][3], raw[g][pb][2], shg[g]);
                 }
-                n_windows += (u32)NP * min((u32)G, t - j) * (WB / 32u);
                 live &= ~mism;
                 live_hi &= ~mism_hi;
                 j += G;
@@ -880,9 +912,9 @@ __device__ void filter_task(const FilterCtx<D> &c, u32 sp, u32 phi, u32 *qb, u64
             }
             // a passing word takes one queue entry per half with survivors:
             // anchors 0 .. 63 (row bit k0) and 64 .. 95 (row bit k0 + 64)
-            const u64 fin_lo = ballot64(fin && live != 0);
-            const u64 fin_hi = ballot64(fin && live_hi != 0);
-            if (fin_lo | fin_hi) {
+            if (ballot64(fin)) {
+                const u64 fin_lo = ballot64(fin && live != 0);
+                const u64 fin_hi = ballot64(fin && live_hi != 0);
                 if (fin) {
                     u32 slot = qn + lanes_below(fin_lo) + lanes_below(fin_hi);
                     const u32 need = (live != 0 ? 1u : 0u) + (live_hi != 0 ? 1u : 0u);
@@ -906,8 +938,9 @@ __device__ void filter_task(const FilterCtx<D> &c, u32 sp, u32 phi, u32 *qb, u64
                 qn = min(qn + n, kQueue);
             }
         }
-        // no word in flight: screen and complete the queue, then requeue the
-        // lanes that were held back
+        // no word in flight: count the last words' stages, screen and
+        // complete the queue, then requeue the lanes that were held back
+        n_windows += (u32)NP * (WB / 32u) * min(j - 1u, t - 1u);
         wave_lds_fence();
         if (qn) drain_queue<D, NP>(c, plb, qn, qb, qs, lane, n_windows, n_done);
         wave_lds_fence();
@@ -1020,6 +1053,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void s
     // per-wave queue of passing words: start b0 and surviving anchors
     __shared__ u32 s_qb[kFilterWaves][kQueue];
     __shared__ u64 s_qs[kFilterWaves][kQueue];
+    // per-wave word parameters of the current task's classes (16 words)
+    __shared__ __attribute__((aligned(16))) u32 s_cw[kFilterWaves][16];
     if (threadIdx.x < 3) s_stats[threadIdx.x] = 0;
     __syncthreads();
     // u32 per lane, added to the block's totals before they can overflow: a
@@ -1029,6 +1064,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void s
     const u32 wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     u32 *qb = s_qb[wave];
     u64 *qs = s_qs[wave];
+    u32 *cw = s_cw[wave];
     // XCD-aware schedule: workgroups are dispatched round-robin over the 8
     // XCDs, so block b runs on XCD b % 8.  XCD x owns the profiles
     // k = x, x+8, x+16, ... and walks them in groups of kGroup profiles (a
@@ -1067,10 +1103,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void s
             continue;
         const SparseProf *fg = c.f;
         switch (fg->np) {
-        case 1: filter_task<D, 1>(c, sp, phi, qb, qs, n_windows, n_words, n_done); break;
-        case 2: filter_task<D, 2>(c, sp, phi, qb, qs, n_windows, n_words, n_done); break;
-        case 3: filter_task<D, 3>(c, sp, phi, qb, qs, n_windows, n_words, n_done); break;
-        default: filter_task<D, 4>(c, sp, phi, qb, qs, n_windows, n_words, n_done); break;
+        case 1: filter_task<D, 1>(c, sp, phi, qb, qs, cw, n_windows, n_words, n_done); break;
+        case 2: filter_task<D, 2>(c, sp, phi, qb, qs, cw, n_windows, n_words, n_done); break;
+        case 3: filter_task<D, 3>(c, sp, phi, qb, qs, cw, n_windows, n_words, n_done); break;
+        default: filter_task<D, 4>(c, sp, phi, qb, qs, cw, n_windows, n_words, n_done); break;
         }
         if ((n_windows | n_done) >= 0x80000000u) {
             atomicAdd(&s_stats[0], (unsigned long long)n_windows);
