@@ -450,9 +450,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-batch64", action="store_true",
                     help="skip the batching workload's sub-measurement")
-    ap.add_argument("--serve-seconds", type=float, default=10.0,
-                    help="length of the Poisson serving run (0 skips it)")
-    ap.add_argument("--qps", type=float, default=20.0, help="serving arrival rate")
+    ap.add_argument("--serve-seconds", type=float, default=8.0,
+                    help="length of each Poisson serving run (0 skips serving)")
+    ap.add_argument("--qps", default="20,60",
+                    help="serving arrival rates, comma separated (one run each)")
     ap.add_argument("--serve-chunk", type=int, default=64, help="primes per request piece")
     ap.add_argument("--serve-max", type=int, default=256, help="primes per serving batch")
     ap.add_argument("--microbench", action="store_true",
@@ -588,20 +589,29 @@ def main():
                     "algorithmic_ops_per_launch": ops, "peak_basis": peak_basis}
     serving = None
     if args.serve_seconds > 0:
-        print("bench: serving", file=sys.stderr, flush=True)
         from rabung_amd import serve, workloads
-        stream = workloads.serving_stream(seed=4, qps=args.qps, seconds=args.serve_seconds)
-        with serve.Server(slots=2, chunk=args.serve_chunk, max_primes=args.serve_max) as srv:
-            serve.run_stream(srv, workloads.serving_stream(seed=5, qps=args.qps, seconds=1.0))
-            srv.batches.clear()
-            _, summ = serve.run_stream(srv, stream)
-        serving = {"workload": "Poisson arrivals at %.1f QPS for %.0f s (seed 4): 80%% single "
-                               "primes (next prime >= uniform u32), 20%% intervals of U{16..4096} "
-                               "consecutive primes from log-uniform starts in [2^16, 2^32); "
-                               "2 slots, continuous batching, pieces of %d primes, batches of "
-                               "at most %d primes" % (args.qps, args.serve_seconds, args.serve_chunk,
-                                                      args.serve_max),
-                   **{k: (round(v, 3) if isinstance(v, float) else v) for k, v in summ.items()}}
+        serving = []
+        for qps in [float(q) for q in args.qps.split(",") if q]:
+            print("bench: serving at %.1f QPS" % qps, file=sys.stderr, flush=True)
+            stream = workloads.serving_stream(seed=4, qps=qps, seconds=args.serve_seconds,
+                                              intervals=True)
+            with serve.Server(slots=2, chunk=args.serve_chunk, max_primes=args.serve_max) as srv:
+                serve.run_stream(srv, workloads.serving_stream(seed=5, qps=qps, seconds=1.0,
+                                                               intervals=True))
+                srv.batches.clear()
+                _, summ = serve.run_stream(srv, stream)
+            serving.append({
+                "workload": "Poisson arrivals at %.1f QPS for %.0f s (seed 4): 80%% single "
+                            "primes (next prime >= uniform u32), 20%% interval queries of "
+                            "U{16..4096} consecutive primes from log-uniform starts in "
+                            "[2^16, 2^32), resolved by rank on the device; 2 slots, continuous "
+                            "batching, pieces of %d primes, batches of at most %d primes"
+                            % (qps, args.serve_seconds, args.serve_chunk, args.serve_max),
+                "qps": qps,
+                # work offered per second of arrivals; a drain well above zero
+                # means the arrivals outran the server (saturation)
+                "offered_profiles_per_s": round(summ["profiles"] / args.serve_seconds, 1),
+                **{k: (round(v, 3) if isinstance(v, float) else v) for k, v in summ.items()}})
     batch64 = None
     if not args.no_batch64:
         print("bench: batch64 workload", file=sys.stderr, flush=True)

@@ -159,6 +159,34 @@ int rbg_profile_block_device(rbg_ctx *ctx, const uint32_t *d_primes, uint32_t n,
 int rbg_profile_block_noalloc(rbg_ctx *ctx, const uint32_t *d_primes, uint32_t n,
                               uint32_t flags, rbg_record *d_out);
 
+/* ---- device-resident prime table (interval queries) ---------------------
+ *
+ * Every prime below 2^32 as a bitmap of the odd numbers in HBM (256 MiB,
+ * plus 4 MiB of block counts and prefixes), built once per device by a
+ * segmented sieve on the device the first time a context needs it and kept
+ * until the process ends (outside rbg_config.hbm_budget).  Ranks count
+ * primes from 0: p_0 = 2, p_1 = 3, ...  No reference counterpart: the
+ * reference's supervisor hands its workers prime lists from a host sieve
+ * (src/census/prime_sieve.cpp:8-22); these calls let a serving loop resolve
+ * interval queries on the device and stream them in pieces by rank. */
+
+/* Builds the table if needed; *count = number of primes below 2^32. */
+int rbg_prime_table(rbg_ctx *ctx, uint64_t *count);
+
+/* *rank = number of primes below x (x <= 2^32). */
+int rbg_prime_rank(rbg_ctx *ctx, uint64_t x, uint64_t *rank);
+
+/* Primes of ranks first .. first+n-1 into out (host); *found = how many of
+ * them lie below 2^32 (n <= RBG_MAX_BATCH). */
+int rbg_primes_by_rank(rbg_ctx *ctx, uint64_t first, uint32_t n, uint32_t *out, uint32_t *found);
+
+/* Profiles of the primes of ranks first .. first+n-1 (n <= RBG_MAX_BLOCK),
+ * selected on the device from the table (no host primality test); out gets
+ * *found records.  Same flags and all-or-nothing contract as
+ * rbg_profile_block. */
+int rbg_profile_ranked(rbg_ctx *ctx, uint64_t first, uint32_t n, uint32_t flags,
+                       rbg_record *out, uint32_t *found);
+
 /* The wider arithmetic domain: primes 1000 <= p < 2^33 (and any smaller
  * prime).  Replaces the reference's 33-bit worker and engine
  * (reproducibility/gpu_method/source/worker33.cu:19-41 calculate_prime /

@@ -67,6 +67,15 @@ extern "C" hipError_t rbg_sparse_filter(const SparsePrime *P, const SparseProf *
                                         uint32_t *failed, unsigned long long *stats,
                                         uint32_t *queue, int wide, uint32_t grid, uint32_t group,
                                         uint32_t extend_limit, hipStream_t stream);
+extern "C" size_t rbg_pt_bytes(void);
+extern "C" hipError_t rbg_pt_build(const uint32_t *d_small, uint32_t nsmall, uint64_t *bits,
+                                   uint32_t *counts, uint32_t *prefix, uint32_t *d_total,
+                                   hipStream_t stream);
+extern "C" hipError_t rbg_pt_select(const uint64_t *bits, const uint32_t *counts,
+                                    const uint32_t *prefix, uint32_t nodd, uint64_t first,
+                                    uint32_t n, uint32_t *d_out, hipStream_t stream);
+extern "C" hipError_t rbg_pt_rank(const uint64_t *bits, const uint32_t *prefix, uint64_t x,
+                                  uint64_t *d_out, hipStream_t stream);
 extern "C" size_t rbg_scratch_capacity(void);
 extern "C" hipError_t rbg_scratch_setup(int device, void **address);
 extern "C" hipError_t rbg_sparse_finalize(const SparsePrime *P, uint32_t nprimes,
@@ -330,6 +339,8 @@ struct rbg_ctx {
     // the allocation-free entry point's scratch region while one of its calls
     // runs (null otherwise)
     Arena *arena = nullptr;
+    // primes selected from the device prime table (rank queries)
+    DevBuf sel;
     // this device's copy of the library's static scratch array (device
     // address, resolved when the context is opened) or the reason it is not
     // available
@@ -340,7 +351,7 @@ struct rbg_ctx {
     std::vector<DevBuf *> buffers() {
         return {&descs, &tab_val, &tab_val64, &tab_code, &hdr, &runs, &recs, &flags,
                 &sp_primes, &sp_profs, &sp_val, &sp_val64, &sp_lab, &sp_labels, &sp_inv,
-                &sp_inv64, &sp_planes, &sp_best, &sp_failed, &sp_a4, &sp_stats, &sp_segs};
+                &sp_inv64, &sp_planes, &sp_best, &sp_failed, &sp_a4, &sp_stats, &sp_segs, &sel};
     }
 
     int fail(int code, const std::string &msg) {
@@ -915,7 +926,8 @@ u64 prime_bytes(u64 p, u32 flags, u32 tile_pos, u32 q) {
 // 2^33 for the wide ones).  Records keep the low word of p; the wide entry
 // points widen them.
 int run_block(rbg_ctx *c, const u64 *h_primes, u32 n, u32 flags, rbg_record *d_out,
-              rbg_record *h_out, u32 limit = RBG_MAX_BLOCK, u64 pmax = 1ull << 32) {
+              rbg_record *h_out, u32 limit = RBG_MAX_BLOCK, u64 pmax = 1ull << 32,
+              bool trusted = false) {
     if (n > limit) return c->fail(RBG_EINVAL, "more than 4096 primes in one block");
     c->error.clear();
     c->timing = rbg_timing{};
@@ -927,7 +939,7 @@ int run_block(rbg_ctx *c, const u64 *h_primes, u32 n, u32 flags, rbg_record *d_o
     std::vector<u32> odd, dense, sparse, failed;
     for (u32 i = 0; i < n; ++i) {
         const u64 p = h_primes[i];
-        if (p >= pmax || !prime_u64(p)) {
+        if (p >= pmax || (!trusted && !prime_u64(p))) {
             char msg[128];
             snprintf(msg, sizeof msg, "not a prime below 2^%d: %llu (index %u)",
                      pmax > (1ull << 32) ? 33 : 32, (unsigned long long)p, i);
@@ -1206,6 +1218,134 @@ int rbg_profile_block_device(rbg_ctx *c, const uint32_t *d_primes, uint32_t n, u
     if (n) HIPCK(c, hipMemcpy(h.data(), d_primes, n * 4, hipMemcpyDeviceToHost));
     const std::vector<u64> wp = widen(h.data(), n);
     return run_block(c, wp.data(), n, flags & ~RBG_CERTIFY, d_out, nullptr);
+}
+
+// ---- device-resident prime table (one per device, kept until exit) --------
+
+namespace {
+
+struct PrimeTable {
+    std::mutex lock;
+    bool built = false;
+    u64 *bits = nullptr;
+    u32 *counts = nullptr, *prefix = nullptr, *total = nullptr;
+    u32 nodd = 0;  // odd primes below 2^32
+    u64 *rank_out = nullptr;
+};
+
+PrimeTable g_tables[64];
+
+// Odd primes 3 <= q < 2^16 (every odd composite below 2^32 has one of them
+// as a factor q <= its square root).
+std::vector<u32> small_odd_primes() {
+    std::vector<unsigned char> comp(1u << 16, 0);
+    std::vector<u32> out;
+    for (u32 q = 3; q < (1u << 16); q += 2) {
+        if (comp[q]) continue;
+        out.push_back(q);
+        for (u32 m = q * q; m < (1u << 16); m += 2 * q) comp[m] = 1;
+    }
+    return out;
+}
+
+// The table of ctx's device, built on first use (segmented sieve on the
+// device, about 258 MiB of HBM outside the context's workspace budget).
+int prime_table(rbg_ctx *c, PrimeTable **out) {
+    if (c->device < 0 || c->device >= 64) return c->fail(RBG_EDEVICE, "device ordinal");
+    PrimeTable &t = g_tables[c->device];
+    std::lock_guard<std::mutex> lock(t.lock);
+    if (!t.built) {
+        HIPCK(c, hipSetDevice(c->device));
+        const std::vector<u32> small = small_odd_primes();
+        const size_t words = (size_t)1 << 25, blocks = words / 64;
+        const size_t small_at = (rbg_pt_bytes() + 64 + 255) & ~(size_t)255;
+        char *mem = nullptr;
+        HIPCK(c, hipMalloc((void **)&mem, small_at + small.size() * 4));
+        u32 *d_small = (u32 *)(mem + small_at);
+        hipError_t e = hipMemcpy(d_small, small.data(), small.size() * 4, hipMemcpyHostToDevice);
+        u64 *bits = (u64 *)mem;
+        u32 *counts = (u32 *)(mem + words * 8), *prefix = counts + blocks,
+            *total = prefix + blocks;
+        if (e == hipSuccess)
+            e = rbg_pt_build(d_small, (u32)small.size(), bits, counts, prefix, total, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        if (e == hipSuccess) e = hipMemcpy(&t.nodd, total, 4, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) {
+            (void)hipFree(mem);
+            return hip_fail(c, e, "building the prime table");
+        }
+        t.bits = bits;
+        t.counts = counts;
+        t.prefix = prefix;
+        t.total = total;
+        t.rank_out = (u64 *)(mem + rbg_pt_bytes());
+        t.built = true;
+    }
+    *out = &t;
+    return RBG_OK;
+}
+
+}  // namespace
+
+int rbg_prime_table(rbg_ctx *c, uint64_t *count) {
+    if (!c) return RBG_EINVAL;
+    PrimeTable *t;
+    if (int rc = prime_table(c, &t)) return rc;
+    if (count) *count = (u64)t->nodd + 1;
+    return RBG_OK;
+}
+
+int rbg_prime_rank(rbg_ctx *c, uint64_t x, uint64_t *rank) {
+    if (!c || !rank) return RBG_EINVAL;
+    if (x > (1ull << 32)) return c->fail(RBG_EINVAL, "rank beyond 2^32");
+    PrimeTable *t;
+    if (int rc = prime_table(c, &t)) return rc;
+    std::lock_guard<std::mutex> lock(t->lock);  // one rank slot per table
+    HIPCK(c, rbg_pt_rank(t->bits, t->prefix, x, t->rank_out, c->stream));
+    HIPCK(c, hipMemcpyAsync(rank, t->rank_out, 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCK(c, hipStreamSynchronize(c->stream));
+    return RBG_OK;
+}
+
+namespace {
+
+// Primes of ranks first .. first+n-1 into the context's device buffer and
+// into h_out; *found = how many exist below 2^32.
+int select_ranks(rbg_ctx *c, u64 first, u32 n, u32 *h_out, u32 *found) {
+    PrimeTable *t;
+    if (int rc = prime_table(c, &t)) return rc;
+    const u64 total = (u64)t->nodd + 1;
+    const u32 m = first >= total ? 0u : (u32)std::min<u64>(n, total - first);
+    if (found) *found = m;
+    if (!m) return RBG_OK;
+    HIPCK(c, c->sel.ensure((size_t)m * 4));
+    HIPCK(c, rbg_pt_select(t->bits, t->counts, t->prefix, t->nodd, first, m, (u32 *)c->sel.ptr,
+                           c->stream));
+    HIPCK(c, hipMemcpyAsync(h_out, c->sel.ptr, (size_t)m * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCK(c, hipStreamSynchronize(c->stream));
+    return RBG_OK;
+}
+
+}  // namespace
+
+int rbg_primes_by_rank(rbg_ctx *c, uint64_t first, uint32_t n, uint32_t *out, uint32_t *found) {
+    if (!c || (!out && n) || !found) return RBG_EINVAL;
+    if (n > RBG_MAX_BATCH) return c->fail(RBG_EINVAL, "more than RBG_MAX_BATCH ranks");
+    return select_ranks(c, first, n, out, found);
+}
+
+int rbg_profile_ranked(rbg_ctx *c, uint64_t first, uint32_t n, uint32_t flags, rbg_record *out,
+                       uint32_t *found) {
+    if (!c || (!out && n) || !found) return RBG_EINVAL;
+    if (n > RBG_MAX_BLOCK) return c->fail(RBG_EINVAL, "more than 4096 primes in one block");
+    std::vector<u32> primes(n);
+    if (int rc = select_ranks(c, first, n, primes.data(), found)) return rc;
+    const std::vector<u64> wp(primes.begin(), primes.begin() + *found);
+    // primes of the sieve need no primality test
+    const int rc = run_block(c, wp.data(), *found, flags, nullptr, out, RBG_MAX_BLOCK,
+                             1ull << 32, true);
+    if (rc && out) memset(out, 0, sizeof(rbg_record) * n);
+    return rc;
 }
 
 // One lock per device: calls of the allocation-free entry point share that

@@ -75,7 +75,8 @@ EXPORTS = ["rbg_open", "rbg_profile_block", "rbg_profile_block_device",
            "rbg_encode_a4_frame", "rbg_science_hash", "rbg_prime_hash", "rbg_a4_certificate", "rbg_microbench",
            "rbg_profile_block_wide", "rbg_profile_batch_wide", "rbg_estimate_bytes_wide",
            "rbg_a4_host_wide", "rbg_is_prime_wide", "rbg_mask_wide", "rbg_primes_from_wide",
-           "rbg_check_wide", "rbg_certify_records",
+           "rbg_check_wide", "rbg_certify_records", "rbg_prime_table", "rbg_prime_rank",
+           "rbg_primes_by_rank", "rbg_profile_ranked",
            "baseline_init",
            "baseline_profile"]
 
@@ -182,6 +183,15 @@ def load():
     lib.rbg_certify_records.argtypes = [P(Record), ctypes.c_uint32, ctypes.c_uint32,
                                         P(ctypes.c_uint32)]
     lib.rbg_certify_records.restype = ctypes.c_int
+    lib.rbg_prime_table.argtypes = [ctypes.c_void_p, P(ctypes.c_uint64)]
+    lib.rbg_prime_rank.argtypes = [ctypes.c_void_p, ctypes.c_uint64, P(ctypes.c_uint64)]
+    lib.rbg_primes_by_rank.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint32,
+                                       P(ctypes.c_uint32), P(ctypes.c_uint32)]
+    lib.rbg_profile_ranked.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint32,
+                                       ctypes.c_uint32, P(Record), P(ctypes.c_uint32)]
+    for name in ("rbg_prime_table", "rbg_prime_rank", "rbg_primes_by_rank",
+                 "rbg_profile_ranked"):
+        getattr(lib, name).restype = ctypes.c_int
     _lib = lib
     return lib
 
@@ -347,6 +357,45 @@ class Engine:
                                                  flags, ctypes.c_void_p(d_out_ptr))
         if rc:
             raise RabungError(rc, self._lib.rbg_last_error(self._ctx).decode())
+
+    def _check(self, rc):
+        if rc:
+            raise RabungError(rc, self._lib.rbg_last_error(self._ctx).decode())
+
+    def prime_count(self):
+        """Number of primes below 2^32, from the device prime table (built on
+        first use: about 258 MiB of HBM kept until the process ends)."""
+        out = ctypes.c_uint64()
+        self._check(self._lib.rbg_prime_table(self._ctx, ctypes.byref(out)))
+        return out.value
+
+    def prime_rank(self, x):
+        """Number of primes below x (x <= 2^32), resolved on the device."""
+        out = ctypes.c_uint64()
+        self._check(self._lib.rbg_prime_rank(self._ctx, int(x), ctypes.byref(out)))
+        return out.value
+
+    def primes_by_rank(self, first, n):
+        """Primes of ranks first .. first+n-1 (rank 0 is 2) below 2^32."""
+        out = (ctypes.c_uint32 * max(1, n))()
+        found = ctypes.c_uint32()
+        self._check(self._lib.rbg_primes_by_rank(self._ctx, int(first), n, out,
+                                                 ctypes.byref(found)))
+        return list(out[:found.value])
+
+    def profile_ranked_raw(self, first, n, method=METHOD_BATCHED, certify=False, graph=False):
+        """Records of the primes of ranks first .. first+n-1 (n <= 4096),
+        selected on the device; returns (Record array, count)."""
+        flags = method | (CERTIFY if certify else 0) | (GRAPH if graph else 0)
+        out = (Record * max(1, n))()
+        found = ctypes.c_uint32()
+        self._check(self._lib.rbg_profile_ranked(self._ctx, int(first), n, flags, out,
+                                                 ctypes.byref(found)))
+        return out, found.value
+
+    def profile_ranked(self, first, n, method=METHOD_BATCHED, certify=False, graph=False):
+        out, k = self.profile_ranked_raw(first, n, method, certify, graph)
+        return [record_to_dict(out[i]) for i in range(k)]
 
     def device_info(self):
         cus = ctypes.c_uint32()

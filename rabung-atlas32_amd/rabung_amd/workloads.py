@@ -68,8 +68,12 @@ def batch_blocks(seed=2026, nblocks=64):
     return blocks
 
 
-def serving_stream(seed=4, qps=20.0, seconds=10.0):
-    """[(arrival_seconds, [primes...]), ...] sorted by arrival."""
+def serving_stream(seed=4, qps=20.0, seconds=10.0, intervals=False):
+    """[(arrival_seconds, [primes...]), ...] sorted by arrival.  With
+    `intervals`, an interval query is given as serve.Interval(start, n) (the
+    server resolves it on the device) instead of its list of primes; the
+    random draws, and so the queries, are the same."""
+    from .serve import Interval
     rng = random.Random(seed)
     t = 0.0
     out = []
@@ -81,6 +85,10 @@ def serving_stream(seed=4, qps=20.0, seconds=10.0):
             primes = [_prime_near(rng.randrange(2, TOP))]
         else:
             n = rng.randint(16, 4096)
-            primes = block(_log_uniform(rng, 16, 32), n) or [4294967291]
+            start = _log_uniform(rng, 16, 32)
+            if intervals:
+                primes = Interval(start, n)
+            else:
+                primes = block(start, n) or [4294967291]
         out.append((t, primes))
     return out
