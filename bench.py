@@ -107,6 +107,8 @@ def measure_batch64(steps, warmup):
                          "seed": round(t["sparse_seed_ms"], 3),
                          "filter": round(t["sparse_filter_ms"], 3),
                          "dense": round(t["dense_scan_ms"], 3),
+                         "a4_host": round(t["a4_host_ms"], 3),
+                         "a4_wait": round(t["a4_wait_ms"], 3),
                          "device_total": round(t["total_ms"], 3)}}
 
 
@@ -255,6 +257,8 @@ def bench_wide(args):
             "stage_ms": {"prep": round(t["sparse_prep_ms"], 3),
                          "seed": round(t["sparse_seed_ms"], 3),
                          "filter": round(t["sparse_filter_ms"], 3),
+                         "a4_host": round(t["a4_host_ms"], 3),
+                         "a4_wait": round(t["a4_wait_ms"], 3),
                          "device_total": round(t["total_ms"], 3)}}
     eng.close()
     print(json.dumps(line), flush=True)
@@ -548,6 +552,12 @@ def main():
                               "peak": round(peak_tops, 3), "unit": "TOP/s",
                               "frac": round(ops / scan_s / 1e12 / peak_tops, 4),
                               "algorithmic_ops_per_step": ops,
+                              # priced at the 6 instructions this build's
+                              # REDC issues; at the 8 of a textbook REDC
+                              # the same time gives frac x 8/6
+                              "ops_per_product": OPS_PER_PRODUCT,
+                              "frac_at_8_ops_per_product": round(
+                                  ops * 8 / OPS_PER_PRODUCT / scan_s / 1e12 / peak_tops, 4),
                               "scan_ms_per_step": round(scan_s * 1000.0, 3),
                               "peak_basis": peak_basis,
                               # products/s against the measured product rate
@@ -576,6 +586,11 @@ def main():
                                  "seed": round(timing["sparse_seed_ms"], 3),
                                  "filter": round(timing["sparse_filter_ms"], 3),
                                  "dense_fallback": round(timing["dense_scan_ms"], 3),
+                                 # A4 point counts run on host threads beside
+                                 # the device; a4_wait is the part the call
+                                 # waited for after the device was done
+                                 "a4_host": round(timing["a4_host_ms"], 3),
+                                 "a4_wait": round(timing["a4_wait_ms"], 3),
                                  "call_total": round(timing["total_ms"], 3)},
                     "peak_basis": peak_basis, "probe": probe}
     else:

@@ -220,6 +220,8 @@ typedef struct rbg_timing {
     uint64_t filter_windows; /* plane windows compared by the filter, in 32-anchor dwords */
     uint64_t filter_words;   /* 64- or 96-anchor words the filter examined */
     uint64_t filter_completions; /* surviving anchors extended to their runs */
+    double a4_host_ms;       /* elliptic A4 counts on host threads (beside the device) */
+    double a4_wait_ms;       /* of that, time the call waited after the device was done */
 } rbg_timing;
 
 int rbg_last_timing(const rbg_ctx *ctx, rbg_timing *out);
@@ -263,6 +265,17 @@ int rbg_a4_certificate(uint32_t p, uint32_t *a4, uint64_t *attempts, char *out, 
  * L2-resident buffer, 8 independent loads in flight per wave (issue cost,
  * not latency). */
 int rbg_microbench(int device, int kind, uint32_t param, double *out);
+
+/* Test hook for the 32-bit residue arithmetic of the dense scan, on
+ * `device` for odd p >= 3 (prime or not): kind 0 writes out[i] =
+ * a[i] b[i] 2^-32 mod p (one Montgomery product; a[i], b[i] < p), kind 1
+ * out[i] = a[i]^b[i] mod p computed in the Montgomery domain (a[i] < p),
+ * kind 2 the Montgomery form (a[i] + b[i]) 2^32 mod p reached by the scan's
+ * position recurrence (b[i] additions of 2^32 mod p, b[i] <= 4096).
+ * Replaces the reference's device Montgomery self-check
+ * (src/census/engine32/build/original_scan_gpu.cu:550-574). */
+int rbg_check_mont32(int device, const uint32_t *a, const uint32_t *b, uint32_t n, uint32_t p,
+                     int kind, uint32_t *out);
 
 /* Test hook for the wide residue arithmetic: on `device`, kind 0 writes
  * out[i] = a[i] b[i] 2^-32 mod p (the kernels' 33-bit Montgomery product),

@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -813,7 +814,13 @@ int sparse_run(rbg_ctx *c, rbg_record *recs, u32 *err, std::vector<u32> &failed_
     std::vector<u32> a4(np, 0);
     for (u32 i = 0; i < np; ++i) sp_p[i] = c->h_sp[i].p | ((u64)(c->h_sp[i].hi & 1u) << 32);
     int a4_rc = RBG_OK;
-    std::thread a4_thread([&] { a4_rc = rbg_a4_host_wide(sp_p.data(), np, a4.data(), 0); });
+    double a4_ms = 0.0;
+    std::thread a4_thread([&] {
+        const auto t0 = std::chrono::steady_clock::now();
+        a4_rc = rbg_a4_host_wide(sp_p.data(), np, a4.data(), 0);
+        a4_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0)
+                    .count();
+    });
     struct Joiner {
         std::thread &t;
         ~Joiner() {
@@ -885,7 +892,11 @@ int sparse_run(rbg_ctx *c, rbg_record *recs, u32 *err, std::vector<u32> &failed_
     HIPCK(c, hipMemcpyAsync(h_failed.data(), failed, np * 4, hipMemcpyDeviceToHost, s));
     HIPCK(c, hipMemcpyAsync(h_stats, c->sp_stats.ptr, sizeof h_stats, hipMemcpyDeviceToHost, s));
     HIPCK(c, hipStreamSynchronize(s));
+    const auto wait0 = std::chrono::steady_clock::now();
     a4_thread.join();
+    c->timing.a4_wait_ms +=
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - wait0).count();
+    c->timing.a4_host_ms += a4_ms;
     if (a4_rc) return c->fail(a4_rc, "elliptic A4 count failed");
     HIPCK(c, hipMemcpyAsync(c->sp_a4.ptr, a4.data(), np * 4, hipMemcpyHostToDevice, s));
     HIPCK(c, rbg_sparse_finalize(dP, np, dF, best, (const u32 *)c->sp_a4.ptr, recs, s));
@@ -1030,6 +1041,8 @@ int run_batch(rbg_ctx *c, const u64 *primes, u32 n, u32 flags, rbg_record *out, 
         sum.filter_windows += t.filter_windows;
         sum.filter_words += t.filter_words;
         sum.filter_completions += t.filter_completions;
+        sum.a4_host_ms += t.a4_host_ms;
+        sum.a4_wait_ms += t.a4_wait_ms;
         first = last;
     }
     c->timing = sum;

@@ -83,6 +83,28 @@ __global__ void wide_check(const u64 *a, const u64 *b, u32 n, u64 p, u32 ninv, u
     out[i] = kind == 0 ? mont_mul33(a[i], b[i], p, ninv) : mont_pow33(a[i], b[i], p, ninv, one);
 }
 
+// 32-bit residue arithmetic of the dense scan on given operands (odd p):
+// kind 0 out[i] = a[i] b[i] 2^-32 mod p (one product, Dom32::mul);
+// kind 1 out[i] = a[i]^b[i] mod p through the Montgomery domain (into the
+// domain by r2 = 2^64 mod p, Dom32::pow, out by a product with 1);
+// kind 2 out[i] = Montgomery form of a[i] + b[i] reached as the scan walks
+// its positions: the form of a[i], then b[i] additions of step = 2^32 mod p.
+__global__ void mont32_check(const u32 *a, const u32 *b, u32 n, u32 p, u32 ninv, u32 r2, u32 one,
+                             int kind, u32 *out) {
+    const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const Dom32 dm{p, pos_inverse(ninv)};
+    if (kind == 0) {
+        out[i] = dm.mul(a[i], b[i]);
+    } else if (kind == 1) {
+        out[i] = dm.mul(dm.pow(dm.mul(a[i], r2), b[i], one), 1u);
+    } else {
+        u32 x = dm.mul(a[i], r2);
+        for (u32 k = 0; k < b[i]; ++k) x = dm.add(x, one);
+        out[i] = x;
+    }
+}
+
 }  // namespace rbg
 
 namespace {
@@ -182,6 +204,35 @@ extern "C" int rbg_microbench(int device, int kind, uint32_t param, double *out)
         rc = RBG_EINVAL;
     }
     (void)hipFree(sink);
+    return rc;
+}
+
+// Device check of the 32-bit residue arithmetic (a test hook: the dense
+// scan's Montgomery product, power and position recurrence on caller
+// operands, host arrays).
+extern "C" int rbg_check_mont32(int device, const uint32_t *a, const uint32_t *b, uint32_t n,
+                                uint32_t p, int kind, uint32_t *out) {
+    if (!a || !b || !out || !n || p < 3 || !(p & 1) || kind < 0 || kind > 2) return RBG_EINVAL;
+    for (uint32_t i = 0; i < n; ++i)
+        if (a[i] >= p || (kind == 0 && b[i] >= p) || (kind == 2 && b[i] > 4096)) return RBG_EINVAL;
+    if (hipSetDevice(device) != hipSuccess) return RBG_EDEVICE;
+    const uint32_t ninv = rbg::neg_inverse32(p);
+    const uint32_t one = (uint32_t)((1ull << 32) % p);
+    const uint32_t r2 = (uint32_t)(((unsigned __int128)1 << 64) % p);
+    uint32_t *d = nullptr;
+    if (hipMalloc(&d, 3ull * n * 4) != hipSuccess) return RBG_EDEVICE;
+    int rc = RBG_OK;
+    if (hipMemcpy(d, a, n * 4ull, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(d + n, b, n * 4ull, hipMemcpyHostToDevice) != hipSuccess) {
+        rc = RBG_EDEVICE;
+    } else {
+        hipLaunchKernelGGL(rbg::mont32_check, dim3((n + 255) / 256), dim3(256), 0, 0, d, d + n, n,
+                           p, ninv, r2, one, kind, d + 2ull * n);
+        if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess ||
+            hipMemcpy(out, d + 2ull * n, n * 4ull, hipMemcpyDeviceToHost) != hipSuccess)
+            rc = RBG_EDEVICE;
+    }
+    (void)hipFree(d);
     return rc;
 }
 

@@ -57,7 +57,8 @@ class Timing(ctypes.Structure):
                 ("fallbacks", ctypes.c_uint64), ("positions", ctypes.c_uint64),
                 ("products", ctypes.c_uint64), ("launches", ctypes.c_uint64),
                 ("filter_windows", ctypes.c_uint64), ("filter_words", ctypes.c_uint64),
-                ("filter_completions", ctypes.c_uint64)]
+                ("filter_completions", ctypes.c_uint64), ("a4_host_ms", ctypes.c_double),
+                ("a4_wait_ms", ctypes.c_double)]
 
 
 class Config(ctypes.Structure):
@@ -76,7 +77,7 @@ EXPORTS = ["rbg_open", "rbg_profile_block", "rbg_profile_block_device",
            "rbg_profile_block_wide", "rbg_profile_batch_wide", "rbg_estimate_bytes_wide",
            "rbg_a4_host_wide", "rbg_is_prime_wide", "rbg_mask_wide", "rbg_primes_from_wide",
            "rbg_check_wide", "rbg_certify_records", "rbg_prime_table", "rbg_prime_rank",
-           "rbg_primes_by_rank", "rbg_profile_ranked",
+           "rbg_primes_by_rank", "rbg_profile_ranked", "rbg_check_mont32",
            "baseline_init",
            "baseline_profile"]
 
@@ -177,6 +178,10 @@ def load():
                                    ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int,
                                    P(ctypes.c_uint64)]
     lib.rbg_check_wide.restype = ctypes.c_int
+    lib.rbg_check_mont32.argtypes = [ctypes.c_int, P(ctypes.c_uint32), P(ctypes.c_uint32),
+                                     ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int,
+                                     P(ctypes.c_uint32)]
+    lib.rbg_check_mont32.restype = ctypes.c_int
     lib.rbg_a4_certificate.argtypes = [ctypes.c_uint32, P(ctypes.c_uint32), P(ctypes.c_uint64),
                                        ctypes.c_void_p, P(ctypes.c_size_t)]
     lib.rbg_a4_certificate.restype = ctypes.c_int
@@ -443,6 +448,21 @@ def check_wide(a, b, p, kind, device=0):
     rc = load().rbg_check_wide(device, A, Bv, n, p, kind, out)
     if rc:
         raise RabungError(rc, "rbg_check_wide")
+    return list(out)
+
+
+def check_mont32(a, b, p, kind, device=0):
+    """Device results of the dense scan's 32-bit residue arithmetic (kind 0:
+    Montgomery product a b 2^-32 mod p; kind 1: a^b mod p through the
+    Montgomery domain; kind 2: Montgomery form of a + b by the position
+    recurrence)."""
+    n = len(a)
+    A = (ctypes.c_uint32 * n)(*a)
+    Bv = (ctypes.c_uint32 * n)(*b)
+    out = (ctypes.c_uint32 * n)()
+    rc = load().rbg_check_mont32(device, A, Bv, n, p, kind, out)
+    if rc:
+        raise RabungError(rc, "rbg_check_mont32")
     return list(out)
 
 
