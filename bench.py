@@ -541,9 +541,16 @@ def main():
         raw2 = d_out.cpu().numpy().tobytes()
         if raw2 != raw:
             raise SystemExit("bench: plain and batched methods disagree")
-        ops = int32_ops(primes)
+        # the library's own count of the powers it computed: with label
+        # reuse only the seed positions and the positions coprime to 6 take
+        # one (the others add a label gathered from HBM)
+        ops = ptimings[-1]["products"] * OPS_PER_PRODUCT
         scan_s = sum(t["dense_scan_ms"] for t in ptimings) / len(ptimings) / 1000.0
-        plain = {"method": "dense Montgomery scan, one launch per prime",
+        plain = {"method": "dense Montgomery scan with label reuse (powers for the positions "
+                           "coprime to 6, labels of the others from x/2 and x/3), phase "
+                           "launches per prime",
+                 "positions_per_step": ptimings[-1]["positions"],
+                 "products_per_step": ptimings[-1]["products"],
                  "profiles_per_s": round(profiles / (pdt / psteps), 3),
                  "ms_per_step": round(pdt * 1000.0 / psteps, 3),
                  "scan_launches_per_step": ptimings[-1]["launches"],

@@ -84,7 +84,8 @@ typedef struct rbg_record_wide {
 
 typedef struct rbg_config {
     int32_t device;          /* HIP device ordinal */
-    uint32_t tile_chunks;    /* dense scan: 64-position chunks per wave segment; 0 = 64 */
+    uint32_t tile_chunks;    /* dense scan: 64-position chunks per wave segment; 0 = 64
+                                (the label-reuse scan rounds it up to a multiple of 12) */
     uint32_t blocks_per_cu;  /* persistent blocks (4 waves) per CU; 0 = 8 */
     uint64_t hbm_budget;     /* bytes of device workspace allowed; 0 = 16 GiB */
     uint32_t xcd_group;      /* sparse filter: profiles one XCD works on at a time; 0 = 1 */
@@ -97,7 +98,10 @@ typedef struct rbg_config {
                                 the dense scan; 0 = 2^22.  A small value forces the
                                 refusal path (the reference's queue-overflow refusal,
                                 gpu_pipeline.cuh:335,342,363) */
-    uint32_t reserved;
+    uint32_t floor_boost;    /* dense scan: added to every run floor (runs shorter than
+                                the floor are not tracked inside segments; a prime whose
+                                longest run stays below it is rescanned exactly); 0 for
+                                normal use.  A large value forces the rescan path */
 } rbg_config;
 
 typedef struct rbg_ctx rbg_ctx;
@@ -222,6 +226,8 @@ typedef struct rbg_timing {
     uint64_t filter_completions; /* surviving anchors extended to their runs */
     double a4_host_ms;       /* elliptic A4 counts on host threads (beside the device) */
     double a4_wait_ms;       /* of that, time the call waited after the device was done */
+    uint64_t rescans;        /* dense scans repeated with every run tracked (a longest
+                                run below its floor; expected never above small primes) */
 } rbg_timing;
 
 int rbg_last_timing(const rbg_ctx *ctx, rbg_timing *out);

@@ -47,6 +47,15 @@ __device__ __forceinline__ u32 add_mod(u32 a, u32 b, u32 p) {
     return (u32)(s >= p ? s - p : s);
 }
 
+// Orders the LDS writes of a wave before its following LDS reads of other
+// lanes' slots (the wave's LDS instructions execute in order; the fences
+// keep the compiler from moving accesses across).
+__device__ __forceinline__ void wave_lds_fence() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Index of y in the sorted table vals[0..m) (binary search, fixed trip count
 // for a given m so that a wave stays converged).
 template <class V>

@@ -38,6 +38,12 @@ using rbg::u16;
 using rbg::u32;
 using rbg::u64;
 
+extern "C" hipError_t rbg_launch_scan_reuse(const PrimeDesc *descs, uint32_t pi, uint32_t ntiles,
+                                            uint32_t chunks, const uint32_t *tab_val,
+                                            const uint16_t *tab_lab, uint16_t *lab,
+                                            uint32_t lab_from, uint32_t first_phase, uint32_t j2,
+                                            uint32_t j3, TileHdr *hdr, RunSum *runs, uint32_t *err,
+                                            uint32_t *launches, hipStream_t stream);
 extern "C" hipError_t rbg_launch_scan(const PrimeDesc *descs, uint32_t nprimes, uint32_t tile_first,
                                       uint32_t tile_end, uint32_t chunks, const void *tab_val,
                                       const uint16_t *tab_code, TileHdr *hdr, RunSum *runs,
@@ -326,6 +332,15 @@ struct rbg_ctx {
     std::vector<u32> h_val;
     std::vector<u64> h_val64;  // root tables of primes above 2^32
     std::vector<u16> h_code;
+    std::vector<u16> h_lab;  // sorted root-table index -> label j (primes below 2^32)
+    // per dense prime: tile chunks, and for the label-reuse scan the labels
+    // of 2 and 3 (reuse = 0: every position by its power)
+    struct DenseLaunch {
+        u32 chunks, reuse, j2, j3;
+    };
+    std::vector<DenseLaunch> h_launch;
+    bool exact_runs = false;  // dense scans with run floor 1 (the rescan)
+    DevBuf tab_lab, dense_lab;
     // sparse path
     DevBuf sp_primes, sp_profs, sp_val, sp_val64, sp_lab, sp_labels, sp_inv, sp_inv64, sp_planes,
         sp_best, sp_failed, sp_a4, sp_stats, sp_segs;
@@ -350,8 +365,8 @@ struct rbg_ctx {
 
     // every device buffer of the context
     std::vector<DevBuf *> buffers() {
-        return {&descs, &tab_val, &tab_val64, &tab_code, &hdr, &runs, &recs, &flags,
-                &sp_primes, &sp_profs, &sp_val, &sp_val64, &sp_lab, &sp_labels, &sp_inv,
+        return {&descs, &tab_val, &tab_val64, &tab_code, &tab_lab, &dense_lab, &hdr, &runs,
+                &recs, &flags, &sp_primes, &sp_profs, &sp_val, &sp_val64, &sp_lab, &sp_labels, &sp_inv,
                 &sp_inv64, &sp_planes, &sp_best, &sp_failed, &sp_a4, &sp_stats, &sp_segs, &sel};
     }
 
@@ -527,18 +542,51 @@ std::vector<u32> by_domain(const u64 *primes, const std::vector<u32> &idx) {
     return out;
 }
 
+// Run floor of order r for a half interval of nh positions: 1 + the
+// largest t with r^t <= nh / 4096, so that runs of that length are expected
+// about 4096 (1 - 1/r) times; segments then skip the bookkeeping of shorter
+// runs (most chunks hold none that long).  A prime whose longest run stays
+// below its floor is rescanned with floor 1, so results never depend on it.
+u32 run_floor(u64 nh, u32 r) {
+    u32 t = 1;
+    for (u64 v = nh / 4096; v >= r && t < 62; v /= r) ++t;
+    return t;
+}
+
+// Label-reuse scan (plain method, primes below 2^32): tiles of the first
+// phase, whose positions read the labels seeded by direct powers.
+constexpr u32 kReuseFirstPhase = 256;
+constexpr u32 kReuseChunkUnit = 12;  // segments start at 1 mod 6, passes of 12 chunks
+
+// Label j of v (x^e = z^j) from a prime's sorted root table.
+u32 label_of(u64 p, u32 e, u32 v, const u32 *vals, const u16 *labs, u32 m) {
+    const u32 y = (u32)(((unsigned __int128)rbg::powmod64((u32)v, e, (u32)p) << 32) % p);
+    const u32 *at = std::lower_bound(vals, vals + m, y);
+    return (at != vals + m && *at == y) ? labs[at - vals] : 0xFFFFFFFFu;
+}
+
 int dense_run(rbg_ctx *c, const u64 *primes, const std::vector<u32> &idx_in, bool plain,
               rbg_record *recs, u32 *err) {
-    const u32 tile_pos = (u32)rbg_waves_per_block() * tile_chunks(c) * 64u;
+    const u32 waves = (u32)rbg_waves_per_block();
+    const u32 reuse_chunks = (tile_chunks(c) + kReuseChunkUnit - 1) / kReuseChunkUnit *
+                             kReuseChunkUnit;
     const std::vector<u32> idx = by_domain(primes, idx_in);
     c->h_descs.clear();
     c->h_val.clear();
     c->h_val64.clear();
     c->h_code.clear();
-    u64 tiles = 0, runs = 0, tiles32 = 0;
+    c->h_lab.clear();
+    c->h_launch.clear();
+    u64 tiles = 0, runs = 0, tiles32 = 0, lab_bytes = 0;
     for (u32 i : idx) {
         const u64 p = primes[i];
         const bool wide = p >= (1ull << 32);
+        // the plain method scans the large primes below 2^32 with label reuse
+        const u64 nh = (p - 1) / 2;
+        const bool reuse = plain && !wide &&
+                           nh > (u64)kReuseFirstPhase * waves * reuse_chunks * 64u;
+        const u32 chunks = reuse ? reuse_chunks : tile_chunks(c);
+        const u32 tile_pos = waves * chunks * 64u;
         const MontConsts k = mont_consts(p);
         PrimeDesc d{};
         d.p = (u32)p;
@@ -561,28 +609,61 @@ int dense_run(rbg_ctx *c, const u64 *primes, const std::vector<u32> &idx_in, boo
         d.nr = (u32)__builtin_popcount(d.mask);
         d.sum_off = (u32)runs;
         d.index = i;
+        d.floors[0] = d.floors[1] = 0x01010101u;
+        if (!c->exact_runs)
+            for (u32 r = 2; r <= 9; ++r)
+                if (d.mask & (1u << (r - 2))) {
+                    const u32 f = std::min(62u, run_floor(d.nh, r) + c->cfg.floor_boost);
+                    d.floors[(r - 2) >> 2] += (f - 1u) << (8u * ((r - 2) & 3u));
+                }
         if (tiles + d.ntiles > 0xFFFFFFFFull || runs + (u64)d.ntiles * d.nr > 0xFFFFFFFFull)
             return c->fail(RBG_ECAPACITY, "block too large for one call");
         tiles += d.ntiles;
         if (!wide) tiles32 = tiles;
         runs += (u64)d.ntiles * d.nr;
         c->timing.positions += d.nh;
-        c->timing.products += (u64)d.nh * rbg::montpow_products(d.e);
         const bool ok = wide ? root_table(p, d.m, c->h_val64, &c->h_code, nullptr)
-                             : root_table(p, d.m, c->h_val, &c->h_code, nullptr);
+                             : root_table(p, d.m, c->h_val, &c->h_code, &c->h_lab);
         if (!ok) return c->fail(RBG_EINVAL, "no root of the required order");
+        if (wide) c->h_lab.resize(c->h_code.size(), 0);
+        rbg_ctx::DenseLaunch L{chunks, 0, 0, 0};
+        if (reuse) {
+            const u32 *vals = c->h_val.data() + d.tab_off;
+            const u16 *labs = c->h_lab.data() + d.tab_off;
+            L.j2 = label_of(p, d.e, 2, vals, labs, d.m);
+            L.j3 = label_of(p, d.e, 3, vals, labs, d.m);
+            if (L.j2 >= d.m || L.j3 >= d.m) return c->fail(RBG_EINVAL, "label of 2 or 3");
+            L.reuse = 1;
+            lab_bytes = std::max<u64>(lab_bytes, (u64)(d.nh / 2 + 1) * 2);
+            // powers: the seed, then the positions coprime to 6
+            const u64 seed = (u64)kReuseFirstPhase * tile_pos / 2;
+            c->timing.products += (seed + (d.nh + 5) / 6 + (d.nh + 1) / 6) *
+                                  rbg::montpow_products(d.e);
+        } else {
+            c->timing.products += (u64)d.nh * rbg::montpow_products(d.e);
+        }
+        c->h_launch.push_back(L);
         c->h_descs.push_back(d);
     }
     const u32 np = (u32)c->h_descs.size();
     c->timing.dense_primes = np;
     if (!np) return RBG_OK;
-    const u64 bytes = np * sizeof(PrimeDesc) + c->h_val.size() * 4 + c->h_val64.size() * 8 +
-                      c->h_code.size() * 2 + tiles * sizeof(TileHdr) + runs * sizeof(RunSum);
+    u64 bytes = np * sizeof(PrimeDesc) + c->h_val.size() * 4 + c->h_val64.size() * 8 +
+                c->h_code.size() * 4 + tiles * sizeof(TileHdr) + runs * sizeof(RunSum);
     if (bytes > budget(c)) return c->fail(RBG_ECAPACITY, "dense workspace exceeds the HBM budget");
+    // no room for labels (the HBM budget, or the static scratch of the
+    // allocation-free entry point, far below a label array's 2 GiB):
+    // every position by its power
+    if (bytes + lab_bytes > budget(c) || c->arena) {
+        for (rbg_ctx::DenseLaunch &L : c->h_launch) L.reuse = 0;
+        lab_bytes = 0;
+    }
     HIPCK(c, c->descs.ensure(np * sizeof(PrimeDesc)));
     HIPCK(c, c->tab_val.ensure(c->h_val.size() * 4));
     HIPCK(c, c->tab_val64.ensure(c->h_val64.size() * 8));
     HIPCK(c, c->tab_code.ensure(c->h_code.size() * 2));
+    HIPCK(c, c->tab_lab.ensure(c->h_lab.size() * 2));
+    if (lab_bytes) HIPCK(c, c->dense_lab.ensure(lab_bytes));
     HIPCK(c, c->hdr.ensure(tiles * sizeof(TileHdr)));
     HIPCK(c, c->runs.ensure(runs * sizeof(RunSum)));
     hipStream_t s = c->stream;
@@ -596,6 +677,8 @@ int dense_run(rbg_ctx *c, const u64 *primes, const std::vector<u32> &idx_in, boo
                                 hipMemcpyHostToDevice, s));
     HIPCK(c, hipMemcpyAsync(c->tab_code.ptr, c->h_code.data(), c->h_code.size() * 2,
                             hipMemcpyHostToDevice, s));
+    HIPCK(c, hipMemcpyAsync(c->tab_lab.ptr, c->h_lab.data(), c->h_lab.size() * 2,
+                            hipMemcpyHostToDevice, s));
     // tile counters of the two persistent launches (flags[0], flags[2])
     u32 *counter = (u32 *)c->flags.ptr, *counter64 = (u32 *)c->flags.ptr + 2;
     const PrimeDesc *dd = (const PrimeDesc *)c->descs.ptr;
@@ -608,9 +691,23 @@ int dense_run(rbg_ctx *c, const u64 *primes, const std::vector<u32> &idx_in, boo
         if (plain) {
             for (u32 i = 0; i < np; ++i) {
                 const PrimeDesc &d = c->h_descs[i];
+                const rbg_ctx::DenseLaunch &L = c->h_launch[i];
                 const int wide = (int)(d.hi & 1u);
-                HIPCK(c, rbg_launch_scan(dd, np, d.tile_begin, d.tile_begin + d.ntiles,
-                                         tile_chunks(c), wide ? c->tab_val64.ptr : c->tab_val.ptr,
+                if (L.reuse) {
+                    const u32 lab_from = kReuseFirstPhase * waves * L.chunks * 64u / 2u;
+                    u32 launches = 0;
+                    HIPCK(c, rbg_launch_scan_reuse(dd, i, d.ntiles, L.chunks,
+                                                   (const u32 *)c->tab_val.ptr,
+                                                   (const u16 *)c->tab_lab.ptr,
+                                                   (u16 *)c->dense_lab.ptr, lab_from,
+                                                   kReuseFirstPhase, L.j2, L.j3,
+                                                   (TileHdr *)c->hdr.ptr, (RunSum *)c->runs.ptr,
+                                                   err, &launches, s));
+                    c->timing.launches += launches;
+                    continue;
+                }
+                HIPCK(c, rbg_launch_scan(dd, np, d.tile_begin, d.tile_begin + d.ntiles, L.chunks,
+                                         wide ? c->tab_val64.ptr : c->tab_val.ptr,
                                          (const u16 *)c->tab_code.ptr, (TileHdr *)c->hdr.ptr,
                                          (RunSum *)c->runs.ptr, counter, err, 0, wide, d.ntiles,
                                          s));
@@ -990,6 +1087,18 @@ int run_block(rbg_ctx *c, const u64 *h_primes, u32 n, u32 flags, rbg_record *d_o
     if (rc) return rc;
     u32 h_err = 0;
     HIPCK(c, hipMemcpyAsync(&h_err, err, sizeof(u32), hipMemcpyDeviceToHost, s));
+    if (!dense.empty()) {
+        HIPCK(c, hipStreamSynchronize(s));
+        if (h_err == 4u) {  // a longest run below its floor: rescan the dense primes exactly
+            c->exact_runs = true;
+            HIPCK(c, hipMemsetAsync(err, 0, sizeof(u32), s));
+            rc = dense_run(c, h_primes, dense, method == RBG_METHOD_PLAIN, recs, err);
+            c->exact_runs = false;
+            if (rc) return rc;
+            ++c->timing.rescans;
+            HIPCK(c, hipMemcpyAsync(&h_err, err, sizeof(u32), hipMemcpyDeviceToHost, s));
+        }
+    }
     if (h_out && n)
         HIPCK(c, hipMemcpyAsync(h_out, recs, (size_t)n * sizeof(rbg_record), hipMemcpyDeviceToHost,
                                 s));
@@ -999,6 +1108,7 @@ int run_block(rbg_ctx *c, const u64 *h_primes, u32 n, u32 flags, rbg_record *d_o
     if (c->timing.dense_primes) c->timing.dense_scan_ms = elapsed(c, kDense0, kDense1);
     if (h_err & 1u) return c->fail(RBG_EDEVICE, "character value missing from the root table");
     if (h_err & 2u) return c->fail(RBG_EDEVICE, "incomplete coverage of the half interval");
+    if (h_err & 4u) return c->fail(RBG_EDEVICE, "longest run below its floor in an exact scan");
     if ((flags & RBG_CERTIFY) && h_out) return certify_all(c, h_out, h_primes, n);
     return RBG_OK;
 }
@@ -1043,6 +1153,7 @@ int run_batch(rbg_ctx *c, const u64 *primes, u32 n, u32 flags, rbg_record *out, 
         sum.filter_completions += t.filter_completions;
         sum.a4_host_ms += t.a4_host_ms;
         sum.a4_wait_ms += t.a4_wait_ms;
+        sum.rescans += t.rescans;
         first = last;
     }
     c->timing = sum;

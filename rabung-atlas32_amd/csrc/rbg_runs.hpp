@@ -46,15 +46,27 @@ __device__ __forceinline__ void interior_best(u64 N, u32 f, u32 h, u32 base, u32
         X = T;
         l = 1;
     } else {
-        // X: positions i with ones at i..i+best-1 (a run of length >= best+1)
+        // X: positions i with ones at i..i+best-1 (a run of length >= best+1).
+        // Doubling up to s, the largest power of two <= best, as straight-line
+        // code on wave-uniform branches: a chunk without s consecutive ones
+        // (most of them, once best is large) leaves after these few steps
         const u32 k = best;
         X = T;
-        u32 s = 1;
-        while (2 * s <= k) {
-            X &= X >> s;
-            s *= 2;
+        if (k >= 2) {
+            X &= X >> 1;
+            if (k >= 4) {
+                X &= X >> 2;
+                if (k >= 8) {
+                    X &= X >> 4;
+                    if (k >= 16) {
+                        X &= X >> 8;
+                        if (k >= 32) X &= X >> 16;
+                    }
+                }
+            }
         }
-        X &= X >> (k - s);
+        if (X == 0) return;
+        X &= X >> (k - (1u << (31 - __builtin_clz(k))));
         if (X == 0) return;
         l = k;
     }
@@ -76,10 +88,14 @@ struct OrderState {
     u32 prefix, cur, best, bstart;
 };
 
-__device__ __forceinline__ void order_init(OrderState &s) {
+// floor >= 1: runs shorter than `floor` are not recorded in best (it starts
+// at floor - 1, a placeholder that any run of length >= floor replaces).  A
+// summary whose final best stays below the floor is then not exact, and the
+// caller must rescan with floor 1.
+__device__ __forceinline__ void order_init(OrderState &s, u32 floor = 1) {
     s.prefix = 0;
     s.cur = 0;
-    s.best = 0;
+    s.best = floor ? floor - 1u : 0u;
     s.bstart = 0;
 }
 

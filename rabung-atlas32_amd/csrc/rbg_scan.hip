@@ -73,6 +73,73 @@ __device__ __forceinline__ void pow_many(const D &dm, const typename D::V *x, u3
     }
 }
 
+// Run bookkeeping of one wave over its segment: per admissible order the
+// run state, fed one 64-position chunk of colour codes at a time.
+struct WaveRuns {
+    OrderState st[8];
+    u32 a4;
+    u64 prevN;
+    u32 carry_code, first_code, last_code;
+};
+
+__device__ __forceinline__ void runs_init(WaveRuns &w, const PrimeDesc &d) {
+#pragma unroll
+    for (int k = 0; k < 8; ++k) order_init(w.st[k], run_floor(d, (u32)k));
+    w.a4 = 0;
+    w.prevN = 0;
+    w.carry_code = w.first_code = w.last_code = 0;
+}
+
+// Chunk c (positions base .. base+valid-1 of the segment) with this lane's
+// colour code: one DPP shift for the left neighbour, one ballot per order.
+__device__ __forceinline__ void runs_chunk(WaveRuns &w, u32 code, u32 c, u32 base, u32 valid,
+                                           u32 mask) {
+    const u64 vmask = valid == 64u ? ~0ull : ((1ull << valid) - 1ull);
+    const u32 left = shift_up_one(code, w.carry_code);
+    const u32 diff = code ^ left;
+    if (c == 0) w.first_code = __builtin_amdgcn_readfirstlane(code);
+    w.carry_code = __builtin_amdgcn_readlane(code, 63);
+    w.last_code = __builtin_amdgcn_readlane(code, valid - 1);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const u32 r = (u32)k + 2u;
+        if (!(mask & (1u << k))) continue;
+        u64 C = ballot64((diff & plane_bits(r)) != 0u);
+        if (c == 0) C |= 1ull;
+        C &= vmask;
+        const u64 N = ~C & vmask;
+        if (r == 2) {
+            u64 W = N & ((N << 1) | (w.prevN >> 63)) & ((N << 2) | (w.prevN >> 62));
+            w.a4 += (u32)__builtin_popcountll(W);
+            w.prevN = N;
+        }
+        order_update(w.st[k], C, N, base);
+    }
+}
+
+// Per-order summaries of the segment (positions relative to its start)
+// into out[r-2]; codes_out = first code | last code << 16.
+__device__ __forceinline__ void runs_finish(WaveRuns &w, u32 seg_len, u32 mask, RunSum *out,
+                                            u32 &codes_out) {
+    const u32 lane = threadIdx.x & 63u;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        if (!(mask & (1u << k))) continue;
+        OrderState &s = w.st[k];
+        const u32 suffix = order_finish(s, seg_len);
+        if (lane == 0) {
+            RunSum o;
+            o.prefix = s.prefix;
+            o.suffix = suffix;
+            o.best = s.best;
+            o.bstart = s.bstart;
+            o.a4 = (k == 0) ? w.a4 : 0;
+            out[k] = o;
+        }
+    }
+    codes_out = w.first_code | (w.last_code << 16);
+}
+
 // Scan one segment [seg_start, seg_start+seg_len) with the calling wave.
 // Writes per-order summaries (positions relative to seg_start) into out[r-2].
 template <class D>
@@ -82,12 +149,8 @@ __device__ void wave_scan(const PrimeDesc &d, const PrimeConsts<D> &k, u32 seg_s
     typedef typename D::V V;
     const u32 lane = threadIdx.x & 63u;
     const u32 e = d.e, m = d.m, mask = d.mask;
-    OrderState st[8];
-#pragma unroll
-    for (int k = 0; k < 8; ++k) order_init(st[k]);
-    u32 a4 = 0;
-    u64 prevN = 0;
-    u32 carry_code = 0, first_code = 0, last_code = 0;
+    WaveRuns w;
+    runs_init(w, d);
     // Montgomery form of this lane's first position
     V xm = k.dom.mul((V)(seg_start + lane), k.r2);
     const u32 nchunks = (seg_len + 63u) >> 6;
@@ -102,58 +165,180 @@ __device__ void wave_scan(const PrimeDesc &d, const PrimeConsts<D> &k, u32 seg_s
         pow_many<D, kPowChains>(k.dom, xs, e, k.one, ys);
         xm = k.dom.add(xs[kPowChains - 1], k.step);
 #pragma unroll
-    for (int ci = 0; ci < kPowChains; ++ci) {
-        const u32 c = c0 + (u32)ci;
-        if (c >= nchunks) break;
-        const u32 base = c << 6;
-        const u32 valid = min(64u, seg_len - base);
-        const u64 vmask = valid == 64u ? ~0ull : ((1ull << valid) - 1ull);
-        const V y = ys[ci];
-        const u32 lo = table_index(s_val, m, y);
-        bad |= (s_val[lo] != y) & (lane < valid);
-        u32 code = s_code[lo];
-        u32 left = shift_up_one(code, carry_code);
-        u32 diff = code ^ left;
-        if (c == 0) first_code = __builtin_amdgcn_readfirstlane(code);
-        carry_code = __builtin_amdgcn_readlane(code, 63);
-        last_code = __builtin_amdgcn_readlane(code, valid - 1);
-#pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            const u32 r = (u32)k + 2u;
-            if (!(mask & (1u << k))) continue;
-            u64 C = ballot64((diff & plane_bits(r)) != 0u);
-            if (c == 0) C |= 1ull;
-            C &= vmask;
-            const u64 N = ~C & vmask;
-            if (r == 2) {
-                u64 W = N & ((N << 1) | (prevN >> 63)) & ((N << 2) | (prevN >> 62));
-                a4 += (u32)__builtin_popcountll(W);
-                prevN = N;
-            }
-            order_update(st[k], C, N, base);
+        for (int ci = 0; ci < kPowChains; ++ci) {
+            const u32 c = c0 + (u32)ci;
+            if (c >= nchunks) break;
+            const u32 base = c << 6;
+            const u32 valid = min(64u, seg_len - base);
+            const V y = ys[ci];
+            const u32 lo = table_index(s_val, m, y);
+            bad |= (s_val[lo] != y) & (lane < valid);
+            runs_chunk(w, s_code[lo], c, base, valid, mask);
         }
-    }
     }
     if (__builtin_amdgcn_ballot_w64(bad != 0)) atomicOr(err, 1u);
+    runs_finish(w, seg_len, mask, out, codes_out);
+}
+
+// ---- label reuse (the plain method for primes below 2^32) ----------------
+//
+// The colour of x depends on its discrete-log label j(x) (x^e = z^j) only,
+// and labels are additive: j(q u) = j(q) + j(u) mod m.  So only positions
+// coprime to 6 need a Montgomery power: an even x takes j(x/2) + j(2) and
+// an odd multiple of 3 takes j(x/3) + j(3) from a label array in HBM (one
+// u16 per position up to nh/2), written by the tiles of lower positions.
+// The reference's V2b variant computes powers for odd positions only
+// (src/census/engine32/build/original_scan_gpu.cu:386-390); here the wheel
+// is 6, a third of the positions.
+//
+// Tiles run in phases of doubling position ranges (launches on one
+// stream): a phase reads only labels of earlier phases (x/2 <= end/2 <=
+// the previous phase's end) and of a seed range computed by direct powers.
+// In a wave, lane l computes the power of the l-th position coprime to 6 of
+// a 192-position group (x = g + 6 (l >> 1) + 4 (l & 1), g = 1 mod 6), four
+// groups at once (768 positions, 12 chunks); the labels of the other
+// positions are gathered from HBM before the powers start, so their latency
+// hides behind the products.  Segments start at 1 mod 6 (their length is a
+// multiple of 192).
+
+constexpr u32 kGroup = 192;                       // positions per 64 coprime ones
+constexpr u32 kReuseChunks = kPowChains * 3;      // chunks per pass (4 groups)
+
+__device__ __forceinline__ void wave_scan_reuse(const PrimeDesc &d, const PrimeConsts<Dom32> &k,
+                                                u32 seg_start, u32 seg_len, const u32 *s_val,
+                                                const u16 *s_lab, const u16 *s_code, u16 *s_jp,
+                                                u16 *lab, u32 lab_from, u32 lab_to, u32 j2, u32 j3,
+                                                RunSum *out, u32 &codes_out, u32 *err) {
+    const u32 lane = threadIdx.x & 63u;
+    const u32 e = d.e, m = d.m, mask = d.mask;
+    const Dom32 &dm = k.dom;
+    WaveRuns w;
+    runs_init(w, d);
+    // per chunk phase s = c mod 3: offset o = 64 s + lane in its group and
+    // the kind of x = g + o (0: coprime to 6, 1: even, 2: odd multiple of 3)
+    u32 kind[3];
 #pragma unroll
-    for (int k = 0; k < 8; ++k) {
-        if (!(mask & (1u << k))) continue;
-        OrderState &s = st[k];
-        const u32 suffix = order_finish(s, seg_len);
-        if (lane == 0) {
-            RunSum o;
-            o.prefix = s.prefix;
-            o.suffix = suffix;
-            o.best = s.best;
-            o.bstart = s.bstart;
-            o.a4 = (k == 0) ? a4 : 0;
-            out[k] = o;
-        }
+    for (u32 s = 0; s < 3; ++s) {
+        const u32 rr = (64u * s + lane) % 6u;
+        kind[s] = (rr == 0u || rr == 4u) ? 0u : (rr & 1u) ? 1u : 2u;
     }
-    codes_out = first_code | (last_code << 16);
+    const u32 step = dm.mul(kGroup, k.r2);  // Montgomery form of 192
+    // this lane's coprime position in each group: g + 6 (lane >> 1) + 4 (lane & 1)
+    const u32 coff0 = 6u * (lane >> 1) + 4u * (lane & 1u);
+    u32 xm = dm.mul(seg_start + coff0, k.r2);
+    const u32 nchunks = (seg_len + 63u) >> 6;
+    u32 bad = 0;
+    for (u32 c0 = 0; c0 < nchunks; c0 += kReuseChunks) {
+        // labels of the non-coprime positions (loads in flight during the powers)
+        u32 lv[kReuseChunks];
+#pragma unroll
+        for (u32 c = 0; c < kReuseChunks; ++c) {
+            const u32 off = ((c0 + c) << 6) + lane;
+            const u32 x = seg_start + off;
+            const u32 kd = kind[c % 3u];
+            lv[c] = 0;
+            if (kd != 0u && off < seg_len) lv[c] = lab[kd == 1u ? x >> 1 : x / 3u];
+        }
+        u32 xs[kPowChains], ys[kPowChains];
+        xs[0] = xm;
+#pragma unroll
+        for (int i = 1; i < kPowChains; ++i) xs[i] = dm.add(xs[i - 1], step);
+        pow_many<Dom32, kPowChains>(dm, xs, e, k.one, ys);
+        xm = dm.add(xs[kPowChains - 1], step);
+        // every label of the pass into s_jp (pass-relative positions)
+#pragma unroll
+        for (int i = 0; i < kPowChains; ++i) {
+            const u32 lo = table_index(s_val, m, ys[i]);
+            const u32 coff = (u32)i * kGroup + coff0;
+            bad |= (s_val[lo] != ys[i]) & ((c0 << 6) + coff < seg_len);
+            s_jp[coff] = s_lab[lo];
+        }
+#pragma unroll
+        for (u32 c = 0; c < kReuseChunks; ++c) {
+            const u32 kd = kind[c % 3u];
+            u32 jr = lv[c] + (kd == 1u ? j2 : j3);
+            jr = jr >= m ? jr - m : jr;
+            bad |= jr >= m;  // a label never written (a phase out of order)
+            if (kd != 0u) s_jp[(c << 6) + lane] = (u16)min(jr, m - 1u);
+        }
+        wave_lds_fence();
+        const u32 cend = min(kReuseChunks, nchunks - c0);
+        for (u32 c = 0; c < cend; ++c) {
+            const u32 cc = c0 + c;
+            const u32 base = cc << 6;
+            const u32 valid = min(64u, seg_len - base);
+            const u32 j = s_jp[(c << 6) + lane];
+            const u32 x = seg_start + base + lane;
+            if (lane < valid && x > lab_from && x <= lab_to) lab[x] = (u16)j;
+            runs_chunk(w, s_code[j], cc, base, valid, mask);
+        }
+        wave_lds_fence();  // the next pass rewrites s_jp
+    }
+    if (__builtin_amdgcn_ballot_w64(bad != 0)) atomicOr(err, 1u);
+    runs_finish(w, seg_len, mask, out, codes_out);
 }
 
 constexpr int kWavesPerBlock = 4;
+
+// Wave summaries of one tile, merged into the tile's summary.
+struct TileWaves {
+    RunSum run[kWavesPerBlock][8];
+    u32 codes[kWavesPerBlock];
+    u32 first[kWavesPerBlock];
+    u32 len[kWavesPerBlock];
+};
+
+// Every wave of the block calls this after its segment: the per-order
+// merge of the waves' summaries in position order, written as tile lt of
+// prime d (global tile number `tile`).  Ends with a block barrier.
+__device__ __forceinline__ void write_tile(TileWaves &tw, u32 wave, const PrimeDesc &d, u32 lt,
+                                           u32 tile, u32 seg_start, u32 seg_len, u32 codes,
+                                           TileHdr *__restrict__ hdr, RunSum *__restrict__ runs) {
+    if ((threadIdx.x & 63u) == 0) {
+        tw.codes[wave] = codes;
+        tw.first[wave] = seg_start;
+        tw.len[wave] = seg_len;
+    }
+    __syncthreads();
+    if (threadIdx.x < 8) {
+        const u32 k = threadIdx.x, r = k + 2;
+        if (d.mask & (1u << k)) {
+            const u32 planes = plane_bits(r);
+            Agg acc;
+            acc.len = 0;
+            for (int w = 0; w < kWavesPerBlock; ++w) {
+                Agg b;
+                b.first_x = tw.first[w];
+                b.len = tw.len[w];
+                b.fcode = tw.codes[w] & 0xFFFFu;
+                b.lcode = tw.codes[w] >> 16;
+                b.prefix = tw.run[w][k].prefix;
+                b.suffix = tw.run[w][k].suffix;
+                b.best = tw.run[w][k].best;
+                b.bstart = tw.run[w][k].bstart + tw.first[w];
+                b.a4 = tw.run[w][k].a4;
+                acc = merge(acc, b, planes);
+            }
+            const u32 kk = __builtin_popcount(d.mask & ((1u << k) - 1u));
+            RunSum o;
+            o.prefix = acc.prefix;
+            o.suffix = acc.suffix;
+            o.best = acc.best;
+            o.bstart = acc.bstart;
+            o.a4 = acc.a4;
+            runs[(u64)d.sum_off + (u64)lt * d.nr + kk] = o;
+            if (k == 0) {
+                TileHdr h;
+                h.first_x = acc.first_x;
+                h.len = acc.len;
+                h.codes = acc.fcode | (acc.lcode << 16);
+                h.pad = 0;
+                hdr[tile] = h;
+            }
+        }
+    }
+    __syncthreads();
+}
 
 // D = Dom32 for primes below 2^32, Dom33 for 2^32 < p < 2^33 (the tiles of
 // one launch belong to primes of one domain).
@@ -165,10 +350,7 @@ __global__ __launch_bounds__(kWavesPerBlock * 64) void scan_tiles(
     u32 *__restrict__ err, int persistent) {
     __shared__ typename D::V s_val[kMaxM];
     __shared__ u16 s_code[kMaxM];
-    __shared__ RunSum s_run[kWavesPerBlock][8];
-    __shared__ u32 s_codes[kWavesPerBlock];
-    __shared__ u32 s_first[kWavesPerBlock];
-    __shared__ u32 s_len[kWavesPerBlock];
+    __shared__ TileWaves s_tw;
     __shared__ u32 s_tile, s_pi;
     // wave-uniform: everything derived from it (segment bounds, chunk masks,
     // the run state of every order) then lives in scalar registers and runs
@@ -218,53 +400,69 @@ __global__ __launch_bounds__(kWavesPerBlock * 64) void scan_tiles(
         const u32 seg_len =
             seg_start > d.nh ? 0u : (u32)min((u64)seg_positions, (u64)d.nh - seg_start + 1ull);
         u32 codes = 0;
-        wave_scan<D>(d, consts_of(d, D{}), (u32)seg_start, seg_len, s_val, s_code, s_run[wave],
+        wave_scan<D>(d, consts_of(d, D{}), (u32)seg_start, seg_len, s_val, s_code, s_tw.run[wave],
                      codes, err);
-        if ((threadIdx.x & 63u) == 0) {
-            s_codes[wave] = codes;
-            s_first[wave] = (u32)seg_start;
-            s_len[wave] = seg_len;
-        }
-        __syncthreads();
-        if (threadIdx.x < 8) {
-            const u32 k = threadIdx.x, r = k + 2;
-            if (d.mask & (1u << k)) {
-                const u32 planes = plane_bits(r);
-                Agg acc;
-                acc.len = 0;
-                for (int w = 0; w < kWavesPerBlock; ++w) {
-                    Agg b;
-                    b.first_x = s_first[w];
-                    b.len = s_len[w];
-                    b.fcode = s_codes[w] & 0xFFFFu;
-                    b.lcode = s_codes[w] >> 16;
-                    b.prefix = s_run[w][k].prefix;
-                    b.suffix = s_run[w][k].suffix;
-                    b.best = s_run[w][k].best;
-                    b.bstart = s_run[w][k].bstart + s_first[w];
-                    b.a4 = s_run[w][k].a4;
-                    acc = merge(acc, b, planes);
-                }
-                const u32 kk = __builtin_popcount(d.mask & ((1u << k) - 1u));
-                RunSum o;
-                o.prefix = acc.prefix;
-                o.suffix = acc.suffix;
-                o.best = acc.best;
-                o.bstart = acc.bstart;
-                o.a4 = acc.a4;
-                runs[(u64)d.sum_off + (u64)lt * d.nr + kk] = o;
-                if (k == 0) {
-                    TileHdr h;
-                    h.first_x = acc.first_x;
-                    h.len = acc.len;
-                    h.codes = acc.fcode | (acc.lcode << 16);
-                    h.pad = 0;
-                    hdr[tile] = h;
-                }
-            }
-        }
-        __syncthreads();
+        write_tile(s_tw, wave, d, lt, tile, (u32)seg_start, seg_len, codes, hdr, runs);
     }
+}
+
+// Label seed of the reuse phases: lab[x] = j(x) for 1 <= x <= n by direct
+// powers (the positions below the first phase's reads).
+__global__ __launch_bounds__(256) void label_seed(const PrimeDesc *__restrict__ descs, u32 pi,
+                                                  const u32 *__restrict__ tab_val,
+                                                  const u16 *__restrict__ tab_lab,
+                                                  u16 *__restrict__ lab, u32 n,
+                                                  u32 *__restrict__ err) {
+    __shared__ u32 s_val[kMaxM];
+    __shared__ u16 s_lab[kMaxM];
+    const PrimeDesc d = descs[pi];
+    for (u32 j = threadIdx.x; j < d.m; j += blockDim.x) {
+        s_val[j] = tab_val[d.tab_off + j];
+        s_lab[j] = tab_lab[d.tab_off + j];
+    }
+    __syncthreads();
+    const PrimeConsts<Dom32> k = consts_of(d, Dom32{});
+    u32 bad = 0;
+    for (u32 x = 1u + blockIdx.x * blockDim.x + threadIdx.x; x <= n; x += gridDim.x * blockDim.x) {
+        const u32 y = k.dom.pow(k.dom.mul(x, k.r2), d.e, k.one);
+        const u32 lo = table_index(s_val, d.m, y);
+        bad |= s_val[lo] != y;
+        lab[x] = s_lab[lo];
+    }
+    if (bad) atomicOr(err, 1u);
+}
+
+// One phase of the label-reuse scan of prime pi: tiles tile_lo + blockIdx.x
+// (prime-local tile numbers, one tile per workgroup).  lab holds the labels
+// of positions 1 .. lab_from (seed) and of every position of the earlier
+// phases up to lab_to = nh / 2; this phase adds its own up to lab_to.
+__global__ __launch_bounds__(kWavesPerBlock * 64) void scan_reuse(
+    const PrimeDesc *__restrict__ descs, u32 pi, u32 tile_lo, u32 chunks,
+    const u32 *__restrict__ tab_val, const u16 *__restrict__ tab_lab, u16 *lab, u32 lab_from,
+    u32 j2, u32 j3, TileHdr *__restrict__ hdr, RunSum *__restrict__ runs, u32 *__restrict__ err) {
+    __shared__ u32 s_val[kMaxM];
+    __shared__ u16 s_lab[kMaxM];   // sorted root-table index -> label j
+    __shared__ u16 s_code[kMaxM];  // label j -> colour code
+    __shared__ u16 s_jp[kWavesPerBlock][kReuseChunks * 64];
+    __shared__ TileWaves s_tw;
+    const u32 wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const PrimeDesc d = descs[pi];
+    for (u32 j = threadIdx.x; j < d.m; j += blockDim.x) {
+        s_val[j] = tab_val[d.tab_off + j];
+        s_lab[j] = tab_lab[d.tab_off + j];
+        s_code[j] = (u16)colour_code(j, d.m);
+    }
+    __syncthreads();
+    const u32 lt = tile_lo + blockIdx.x;
+    const u32 seg_positions = chunks * 64u;
+    const u64 seg = (u64)lt * kWavesPerBlock + wave;
+    const u64 seg_start = 1ull + seg * seg_positions;
+    const u32 seg_len =
+        seg_start > d.nh ? 0u : (u32)min((u64)seg_positions, (u64)d.nh - seg_start + 1ull);
+    u32 codes = 0;
+    wave_scan_reuse(d, consts_of(d, Dom32{}), (u32)seg_start, seg_len, s_val, s_lab, s_code,
+                    s_jp[wave], lab, lab_from, d.nh / 2u, j2, j3, s_tw.run[wave], codes, err);
+    write_tile(s_tw, wave, d, lt, d.tile_begin + lt, (u32)seg_start, seg_len, codes, hdr, runs);
 }
 
 constexpr int kMergeThreads = 256;
@@ -325,6 +523,9 @@ __global__ __launch_bounds__(kMergeThreads) void merge_primes(const PrimeDesc *_
         if (t == 0) {
             const Agg S = s_agg[0];
             if (S.len != d.nh || S.first_x != 1u) atomicOr(err, 2u);
+            // runs below the floor were not tracked: exact only if a run
+            // reached it
+            if (S.best < run_floor(d, (u32)k)) atomicOr(err, 4u);
             const u64 p = (u64)d.p | ((u64)(d.hi & 1u) << 32);
             const bool sigma0 = (((p - 1u) / 2u) % r) == 0u;
             u32 M = S.best, W = S.bstart;
@@ -380,6 +581,35 @@ extern "C" hipError_t rbg_launch_scan(const rbg::PrimeDesc *descs, uint32_t npri
                            0, stream, descs, nprimes, tile_first, tile_end, chunks,
                            (const uint32_t *)tab_val, tab_code, hdr, runs, counter, err,
                            persistent);
+    return hipGetLastError();
+}
+
+// Label-reuse scan of descriptor pi (a prime below 2^32): the seed of
+// labels 1 .. lab_from, then phases of doubling tile ranges, each phase
+// reading only labels of positions below its first tile.  first_phase = the
+// tiles of the first phase (2 lab_from >= first_phase * tile positions).
+extern "C" hipError_t rbg_launch_scan_reuse(const rbg::PrimeDesc *descs, uint32_t pi,
+                                            uint32_t ntiles, uint32_t chunks,
+                                            const uint32_t *tab_val, const uint16_t *tab_lab,
+                                            uint16_t *lab, uint32_t lab_from,
+                                            uint32_t first_phase, uint32_t j2, uint32_t j3,
+                                            rbg::TileHdr *hdr, rbg::RunSum *runs, uint32_t *err,
+                                            uint32_t *launches, hipStream_t stream) {
+    if (chunks % rbg::kReuseChunks || !first_phase ||
+        2ull * lab_from < (uint64_t)first_phase * rbg::kWavesPerBlock * chunks * 64u)
+        return hipErrorInvalidValue;
+    const uint32_t seed_blocks = (lab_from + 255u) / 256u < 2048u ? (lab_from + 255u) / 256u : 2048u;
+    if (lab_from)
+        hipLaunchKernelGGL(rbg::label_seed, dim3(seed_blocks), dim3(256), 0, stream, descs, pi,
+                           tab_val, tab_lab, lab, lab_from, err);
+    *launches = lab_from ? 1u : 0u;
+    for (uint32_t lo = 0, hi = first_phase < ntiles ? first_phase : ntiles; lo < ntiles;
+         lo = hi, hi = 2u * hi < ntiles ? 2u * hi : ntiles) {
+        hipLaunchKernelGGL(rbg::scan_reuse, dim3(hi - lo), dim3(rbg::kWavesPerBlock * 64), 0,
+                           stream, descs, pi, lo, chunks, tab_val, tab_lab, lab, lab_from, j2, j3,
+                           hdr, runs, err);
+        ++*launches;
+    }
     return hipGetLastError();
 }
 

@@ -678,12 +678,6 @@ __device__ __forceinline__ u64 window_of(u32 a, u32 b, u32 c, u32 sh) {
     return ((u64)__builtin_amdgcn_alignbit(c, b, sh) << 32) | __builtin_amdgcn_alignbit(b, a, sh);
 }
 
-__device__ __forceinline__ void wave_lds_fence() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
 // Per class of a task: where its words lie and which of their bits are
 // anchors (wave-uniform).  Every span holds the same anchors of the class
 // relative to its start (g | d), so a word is named by (span, w) and

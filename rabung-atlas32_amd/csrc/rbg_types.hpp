@@ -67,7 +67,13 @@ struct PrimeDesc {
                     // p, one, r2 and step above hold the low words
     u32 tab64;      // 2^32 < p: offset of the root-table values in the u64
                     // value array (codes stay at tab_off)
+    u32 floors[2];  // run floor of order r in byte r-2 (bytes 0-3 of [0],
+                    // 4-7 of [1]): runs shorter than the floor are not
+                    // tracked inside a segment (1: every run)
 };
+
+// Run floor of order k + 2.
+RBG_HD u32 run_floor(const PrimeDesc &d, u32 k) { return (d.floors[k >> 2] >> (8u * (k & 3u))) & 0xFFu; }
 
 // Per (tile, admissible order) run summary of a contiguous stretch of the
 // half interval, the associative run monoid of the reference's dense scans

@@ -230,6 +230,19 @@ def test_methods_agree_and_are_deterministic(engine):
     assert a == b == c == d
 
 
+def test_run_floor_rescan_keeps_results(engine):
+    """Run floors (runs too short to matter are not tracked inside segments)
+    never change results: with every floor raised past the longest run the
+    library rescans the dense primes exactly and returns the same records."""
+    primes = oracle.next_primes(3_000_000_000, 3) + [1000003, 65537, 4294967291]
+    for method in (rabung_amd.METHOD_PLAIN, rabung_amd.METHOD_DENSE):
+        want = engine.profile_block(primes, method=method)
+        assert engine.last_timing()["rescans"] == 0
+        with rabung_amd.Engine(floor_boost=40) as eng:
+            assert eng.profile_block(primes, method=method) == want
+            assert eng.last_timing()["rescans"] == 1
+
+
 def test_tile_geometry_invariance():
     """Results must not depend on segment length or persistent grid size."""
     primes = [3, 5, 7, 11, 13, 101, 65537, 1000003] + oracle.next_primes(1 << 24, 8)
