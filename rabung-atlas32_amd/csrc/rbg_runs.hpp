@@ -27,6 +27,27 @@ __device__ __forceinline__ u32 shift_up_one(u32 v, u32 fill) {
     return (u32)__builtin_amdgcn_update_dpp((int)fill, (int)v, 0x138, 0xF, 0xF, false);
 }
 
+// Positions i of X with ones at i..i+k-1 (k >= 1): doubling up to s, the
+// largest power of two <= k, as straight-line code on wave-uniform
+// branches, then one shift by k - s.
+__device__ __forceinline__ u64 runs_of_ones(u64 X, u32 k) {
+    if (k >= 2) {
+        X &= X >> 1;
+        if (k >= 4) {
+            X &= X >> 2;
+            if (k >= 8) {
+                X &= X >> 4;
+                if (k >= 16) {
+                    X &= X >> 8;
+                    if (k >= 32) X &= X >> 16;
+                }
+            }
+        }
+    }
+    if (X == 0) return 0;
+    return X & (X >> (k - (1u << (31 - __builtin_clz(k)))));
+}
+
 // Longest interior run of one 64-position chunk.  N: same-as-left mask of the
 // valid positions; f < h are the first and last change bits.  Interior runs
 // start at the change bits in [f, h) and close before h; they start after
@@ -34,6 +55,9 @@ __device__ __forceinline__ u32 shift_up_one(u32 v, u32 fill) {
 __device__ __forceinline__ void interior_best(u64 N, u32 f, u32 h, u32 base, u32 &best,
                                               u32 &bstart) {
     if (best > 62) return;
+    // quick reject on the whole chunk (the interior mask only narrows N):
+    // once best is large, most chunks hold no best consecutive equal pairs
+    if (best != 0 && runs_of_ones(N, best) == 0) return;
     const u64 between = ((1ull << h) - 1ull) & ~((2ull << f) - 1ull);
     const u64 T = N & between;
     u64 X;
@@ -46,29 +70,10 @@ __device__ __forceinline__ void interior_best(u64 N, u32 f, u32 h, u32 base, u32
         X = T;
         l = 1;
     } else {
-        // X: positions i with ones at i..i+best-1 (a run of length >= best+1).
-        // Doubling up to s, the largest power of two <= best, as straight-line
-        // code on wave-uniform branches: a chunk without s consecutive ones
-        // (most of them, once best is large) leaves after these few steps
-        const u32 k = best;
-        X = T;
-        if (k >= 2) {
-            X &= X >> 1;
-            if (k >= 4) {
-                X &= X >> 2;
-                if (k >= 8) {
-                    X &= X >> 4;
-                    if (k >= 16) {
-                        X &= X >> 8;
-                        if (k >= 32) X &= X >> 16;
-                    }
-                }
-            }
-        }
+        // X: positions i with ones at i..i+best-1 (a run of length >= best+1)
+        X = runs_of_ones(T, best);
         if (X == 0) return;
-        X &= X >> (k - (1u << (31 - __builtin_clz(k))));
-        if (X == 0) return;
-        l = k;
+        l = best;
     }
     for (;;) {
         const u64 Y = X & (T >> l);
