@@ -102,6 +102,14 @@ typedef struct rbg_config {
                                 the floor are not tracked inside segments; a prime whose
                                 longest run stays below it is rescanned exactly); 0 for
                                 normal use.  A large value forces the rescan path */
+    uint32_t cover_div;      /* sparse engine: cover shape n, D = floor(2 sqrt(p/H) / n)
+                                (H = Q - 1), B the least radius certified for that D;
+                                larger n trades a wider table for fewer denominators
+                                (paper/GPU_METHOD_EN.tex:322-336).  2..8; 0 = 2, the
+                                reference's D = floor(sqrt(p/H)).  D grows back toward
+                                the n = 2 value where B would pass 2^20 */
+    uint32_t filter_chunk;   /* sparse filter: tasks one wave claims from its XCD's
+                                queue at a time, 1..256; 0 = 32 */
 } rbg_config;
 
 typedef struct rbg_ctx rbg_ctx;

@@ -73,7 +73,8 @@ extern "C" hipError_t rbg_sparse_filter(const SparsePrime *P, const SparseProf *
                                         const uint16_t *tab_lab, unsigned long long *best,
                                         uint32_t *failed, unsigned long long *stats,
                                         uint32_t *queue, int wide, uint32_t grid, uint32_t group,
-                                        uint32_t extend_limit, hipStream_t stream);
+                                        uint32_t chunk, uint32_t extend_limit,
+                                        hipStream_t stream);
 extern "C" size_t rbg_pt_bytes(void);
 extern "C" hipError_t rbg_pt_build(const uint32_t *d_small, uint32_t nsmall, uint64_t *bits,
                                    uint32_t *counts, uint32_t *prefix, uint32_t *d_total,
@@ -250,13 +251,41 @@ u64 isqrt64(u64 n) {
 // Certified clipped cover for threshold cap Q (GPU_METHOD_EN.tex:254-326;
 // reference vdw_clipped::from_denominator): D ~ sqrt(p/H), least B with
 // 4BD - HD^2 > 4p, all checked in integers.
-bool choose_cover(u64 p, u32 Q, u32 &B, u32 &D) {
+// Least certified radius for a given D (GPU_METHOD_EN.tex eq. Bchoice), or 0
+// when the branch condition H D^2 <= 4p fails.
+u64 cover_radius(u64 p, u64 H, u64 d) {
+    if (d < 1 || H * d * d > 4ull * p) return 0;
+    return (4ull * p + H * d * d) / (4ull * d) + 1;
+}
+
+// Cover shape n (paper/GPU_METHOD_EN.tex:322-336): D = floor(2 sqrt(p/H) / n),
+// so n = 2 is the reference's D = floor(sqrt(p/H)) and B D ~ p (1 + 1/n^2).
+// Where the least radius for that D would pass the table limit 2^20, D grows
+// to the least value whose radius fits (never past the n = 2 value).  The
+// returned pair is certified with exact integers.
+bool choose_cover(u64 p, u32 Q, u32 div, u32 &B, u32 &D) {
     const u64 H = Q - 1;
-    u64 d = isqrt64(p / H);
-    while (d > 1 && H * d * d > 4ull * p) --d;
+    const u64 n = div ? div : 2;
+    u64 dmax = isqrt64(p / H);  // n = 2
+    while (dmax > 1 && H * dmax * dmax > 4ull * p) --dmax;
+    u64 d = n <= 2 ? dmax : std::max<u64>(1, isqrt64(4ull * p / (H * n * n)));
+    if (d > dmax) d = dmax;
+    constexpr u64 kMaxB = 1u << 20;
+    if (d < dmax && cover_radius(p, H, d) > kMaxB) {
+        // radius(d) ~ p/d + H d/4 falls as d grows below 2 sqrt(p/H): start
+        // from the real root of H d^2 - 4 kMaxB d + 4p = 0 and step to the
+        // least integer that fits
+        const double bm = (double)kMaxB - 1.0;
+        const double disc = bm * bm - (double)H * (double)p;
+        u64 d0 = disc > 0 ? (u64)((2.0 * bm - 2.0 * std::sqrt(disc)) / (double)H) : dmax;
+        d0 = std::min(std::max<u64>(d0, d), dmax);
+        while (d0 > d && cover_radius(p, H, d0 - 1) && cover_radius(p, H, d0 - 1) <= kMaxB) --d0;
+        while (d0 < dmax && (!cover_radius(p, H, d0) || cover_radius(p, H, d0) > kMaxB)) ++d0;
+        d = d0;
+    }
     if (d < 1) return false;
-    const u64 b = (4ull * p + H * d * d) / (4ull * d) + 1;
-    if (b >= p || d >= p || b > (1u << 20) || H * d > 2 * b) return false;
+    const u64 b = cover_radius(p, H, d);
+    if (!b || b >= p || d >= p || b > kMaxB || H * d > 2 * b) return false;
     if (4ull * b * d - H * d * d <= 4ull * p) return false;
     if (2 * b + 2 >= (u64)p - 1) return false;  // table stays inside the first half
     B = (u32)b;
@@ -442,6 +471,10 @@ u32 xcd_group(const rbg_ctx *c) { return c->cfg.xcd_group ? c->cfg.xcd_group : 1
 u64 budget(const rbg_ctx *c) { return c->cfg.hbm_budget ? c->cfg.hbm_budget : (16ull << 30); }
 
 u32 cover_q(const rbg_ctx *c) { return c->cfg.cover_q ? c->cfg.cover_q : kCoverQ; }
+
+u32 cover_div(const rbg_ctx *c) { return c->cfg.cover_div ? c->cfg.cover_div : 2u; }
+
+u32 filter_chunk(const rbg_ctx *c) { return c->cfg.filter_chunk ? c->cfg.filter_chunk : 32u; }
 
 u32 extend_limit(const rbg_ctx *c) {
     return c->cfg.extend_limit ? c->cfg.extend_limit : kExtendLimit;
@@ -776,7 +809,7 @@ int sparse_plan(rbg_ctx *c, const u64 *primes, const std::vector<u32> &idx_in,
             const u64 p = primes[idx[k]];
             const bool wide = p >= (1ull << 32);
             u32 B = 0, D = 0;
-            it.sparse = p >= kSparseMin && choose_cover(p, cover_q(c), B, D);
+            it.sparse = p >= kSparseMin && choose_cover(p, cover_q(c), cover_div(c), B, D);
             it.ok = true;
             if (!it.sparse) continue;
             const MontConsts mc = mont_consts(p);
@@ -942,7 +975,9 @@ int sparse_run(rbg_ctx *c, rbg_record *recs, u32 *err, std::vector<u32> &failed_
     u32 *failed = (u32 *)c->sp_failed.ptr;
     // the filter's XCD-aware schedule needs a grid that is a multiple of 8
     const u32 grid = std::max(8u, ((u32)c->cus * blocks_per_cu(c) + 7u) / 8u * 8u);
-    if (std::max(dmax[0], dmax[1]) * (nf + 8ull * xcd_group(c)) > 0xFFFFFFFFull)
+    // a per-XCD counter ends at most one claim per wave past its items
+    if (std::max(dmax[0], dmax[1]) * (nf + 8ull * xcd_group(c)) +
+            4ull * grid * filter_chunk(c) > 0xFFFFFFFFull)
         return c->fail(RBG_ECAPACITY, "too many filter tasks");
     // one chain: clear flags and counters, tables and planes, seed, filter
     // (eagerly with an event per stage; in graph mode the whole chain is
@@ -977,7 +1012,8 @@ int sparse_run(rbg_ctx *c, rbg_record *recs, u32 *err, std::vector<u32> &failed_
                                        (const u16 *)c->sp_lab.ptr, best + f0, failed,
                                        (unsigned long long *)c->sp_stats.ptr,
                                        (u32 *)((char *)c->sp_stats.ptr + kQueueOffset) + 8 * wide,
-                                       wide, grid, xcd_group(c), extend_limit(c), s));
+                                       wide, grid, xcd_group(c), filter_chunk(c),
+                                       extend_limit(c), s));
         }
         return RBG_OK;
     });
@@ -1012,14 +1048,14 @@ int sparse_run(rbg_ctx *c, rbg_record *recs, u32 *err, std::vector<u32> &failed_
 
 // Device workspace one prime needs under `flags` (the planning rules of
 // sparse_plan and dense_run).
-u64 prime_bytes(u64 p, u32 flags, u32 tile_pos, u32 q) {
+u64 prime_bytes(u64 p, u32 flags, u32 tile_pos, u32 q, u32 div) {
     if (p < 3) return sizeof(rbg_record);
     const u32 mask = mask_of(p), m = lcm_of(mask), nr = (u32)__builtin_popcount(mask);
     const u64 vbytes = p >= (1ull << 32) ? 8 : 4;  // root-table and inverse values
     u64 bytes = sizeof(rbg_record) + (u64)m * (vbytes + 4) + 64;
     u32 B = 0, D = 0;
     if ((flags & RBG_METHOD_MASK) == RBG_METHOD_BATCHED && p >= kSparseMin &&
-        choose_cover(p, q, B, D)) {
+        choose_cover(p, q, div, B, D)) {
         bytes += 2ull * (B + 1) + vbytes * (D + 1) + 88;
         for (u32 r = 2; r <= 9; ++r)
             if (mask & (1u << (r - 2)))
@@ -1044,6 +1080,9 @@ int run_block(rbg_ctx *c, const u64 *h_primes, u32 n, u32 flags, rbg_record *d_o
     if (method > RBG_METHOD_DENSE) return c->fail(RBG_EINVAL, "unknown method");
     if (cover_q(c) < 3 || cover_q(c) > kMaxCoverQ)
         return c->fail(RBG_EINVAL, "cover_q outside 3..63");
+    if (c->cfg.cover_div == 1 || c->cfg.cover_div > 8)
+        return c->fail(RBG_EINVAL, "cover_div outside 2..8");
+    if (c->cfg.filter_chunk > 256) return c->fail(RBG_EINVAL, "filter_chunk outside 1..256");
     std::vector<u32> odd, dense, sparse, failed;
     for (u32 i = 0; i < n; ++i) {
         const u64 p = h_primes[i];
@@ -1126,7 +1165,7 @@ int run_batch(rbg_ctx *c, const u64 *primes, u32 n, u32 flags, rbg_record *out, 
         u32 last = first;
         u64 bytes = 0;
         while (last < n && last - first < kBatchChunk) {
-            const u64 b = prime_bytes(primes[last], flags, tile_pos, cover_q(c));
+            const u64 b = prime_bytes(primes[last], flags, tile_pos, cover_q(c), cover_div(c));
             if (last > first && bytes + b > cap) break;
             bytes += b;
             ++last;
@@ -1249,7 +1288,7 @@ int rbg_estimate_bytes_wide(const rbg_ctx *c, const uint64_t *primes, uint32_t n
     if (!c || (!primes && n) || !bytes) return RBG_EINVAL;
     const u32 tile_pos = (u32)rbg_waves_per_block() * tile_chunks(c) * 64u;
     u64 total = 0;
-    for (u32 i = 0; i < n; ++i) total += prime_bytes(primes[i], flags, tile_pos, cover_q(c));
+    for (u32 i = 0; i < n; ++i) total += prime_bytes(primes[i], flags, tile_pos, cover_q(c), cover_div(c));
     *bytes = total;
     return RBG_OK;
 }
@@ -1259,7 +1298,7 @@ int rbg_estimate_bytes(const rbg_ctx *c, const uint32_t *primes, uint32_t n, uin
     if (!c || (!primes && n) || !bytes) return RBG_EINVAL;
     const u32 tile_pos = (u32)rbg_waves_per_block() * tile_chunks(c) * 64u;
     u64 total = 0;
-    for (u32 i = 0; i < n; ++i) total += prime_bytes(primes[i], flags, tile_pos, cover_q(c));
+    for (u32 i = 0; i < n; ++i) total += prime_bytes(primes[i], flags, tile_pos, cover_q(c), cover_div(c));
     *bytes = total;
     return RBG_OK;
 }
@@ -1495,7 +1534,7 @@ int rbg_profile_block_noalloc(rbg_ctx *c, const uint32_t *d_primes, uint32_t n, 
     const u32 tile_pos = (u32)rbg_waves_per_block() * tile_chunks(c) * 64u;
     u64 need = 0;
     for (u32 i = 0; i < n; ++i)
-        need += prime_bytes(wp[i], flags & RBG_METHOD_MASK, tile_pos, cover_q(c));
+        need += prime_bytes(wp[i], flags & RBG_METHOD_MASK, tile_pos, cover_q(c), cover_div(c));
     if (need > cap) {
         char msg[160];
         snprintf(msg, sizeof msg, "block needs about %llu bytes of device workspace; "

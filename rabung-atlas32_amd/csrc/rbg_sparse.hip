@@ -672,11 +672,12 @@ __device__ __forceinline__ void drain_queue(const FilterCtx<D> &c, const u32 *co
         }
     }
 }
-// 64-bit window from the three words at its first bit (shift sh = bit & 31);
-// the filter's 96-bit windows are this plus one more funnel shift.
-__device__ __forceinline__ u64 window_of(u32 a, u32 b, u32 c, u32 sh) {
-    return ((u64)__builtin_amdgcn_alignbit(c, b, sh) << 32) | __builtin_amdgcn_alignbit(b, a, sh);
+// m | (a ^ b) in one V_BITOP3_B32 (truth table 0xF6 over the inputs m, a, b,
+// m the most significant index bit).
+__device__ __forceinline__ u32 or_xor(u32 m, u32 a, u32 b) {
+    return __builtin_amdgcn_bitop3_b32(m, a, b, 0xF6);
 }
+
 
 // Per class of a task: where its words lie and which of their bits are
 // anchors (wave-uniform).  Every span holds the same anchors of the class
@@ -726,8 +727,11 @@ __device__ __forceinline__ ClassWords class_words(const Spans &S, const TaskRows
 // survivors; the queue is screened and completed with all lanes at once
 // outside the word loop, at the end of the task or when it is full (the
 // lanes whose words did not fit keep them and the word loop stops handing
-// out words until the queue has been drained).  Stages past t-1 re-read
-// stage t-1, which keeps every address inside the row.
+// out words until the queue has been drained).  A step tests min(G, t-1)
+// distinct stages from min(j, t - min(G, t-1)), so a word's last step
+// re-tests stages it has already passed rather than read past stage t-1: every
+// address stays inside the row and the stage offsets of a step are
+// wave-uniform.
 constexpr u32 kRefill = 48;  // smallest cohort of idle lanes that takes new words
 
 template <class D, int NP>
@@ -771,6 +775,12 @@ __device__ void filter_task(const FilterCtx<D> &c, u32 sp, u32 phi, u32 *qb, u64
     const u32 *plb[NP];
 #pragma unroll
     for (int pb = 0; pb < NP; ++pb) plb[pb] = c.rows + (u32)pb * c.wpr;
+    // stage offsets of a step: Gt = min(G, t-1) distinct stages, the rest
+    // repeat the last one (wave-uniform)
+    const u32 Gt = min((u32)G, t - 1u);
+    u32 ug[G];
+#pragma unroll
+    for (int g = 0; g < G; ++g) ug[g] = min((u32)g, Gt - 1u) * dg;
     u32 next = 0, qn = 0;
     bool pend = false;  // a passing word that did not fit the queue
     u32 k0 = 0;
@@ -780,7 +790,10 @@ __device__ void filter_task(const FilterCtx<D> &c, u32 sp, u32 phi, u32 *qb, u64
         bool stall = false;
         bool have = false;
         u32 j = 1, sh = 0;
-        int io = 0, io_last = 0;  // row bit of stage j and of stage t-1
+        // row bit of stage j, and of stage t - Gt: a step tests Gt stages
+        // from min(j, t - Gt), so a word's last step re-tests a few stages
+        // instead of reading past stage t-1
+        int io = 0, io_min = 0;
         u32 braw[NP][NW];
 #pragma unroll
         for (int pb = 0; pb < NP; ++pb)
@@ -846,7 +859,8 @@ __device__ void filter_task(const FilterCtx<D> &c, u32 sp, u32 phi, u32 *qb, u64
                             io = (int)k0 - (int)dg;
                             // anchors are >= bmin, so stage t-1 stays in the
                             // row; bits below bmin only need an address in it
-                            io_last = max((int)k0 - (int)((t - 1u) * dg), (int)C.start);
+                            io_min = max((int)k0 - (int)((t - Gt) * dg),
+                                         (int)(C.start + (Gt - 1u) * dg));
                             sh = k0 & 31u;
                             const u32 wo = k0 >> 5;
 #pragma unroll
@@ -867,11 +881,12 @@ __device__ void filter_task(const FilterCtx<D> &c, u32 sp, u32 phi, u32 *qb, u64
             bool fin = false;
             if (have) {
                 // every load of the step is issued before the first use
+                const int ia = max(io, io_min);
                 u32 raw[G][NP][NW], shg[G];
 #pragma unroll
                 for (int g = 0; g < G; ++g) {
-                    const u32 ig = (u32)max(io - g * (int)dg, io_last);
-                    shg[g] = ig & 31u;
+                    const u32 ig = (u32)(ia - (int)ug[g]);
+                    shg[g] = ig;  // alignbit takes the low 5 bits
 #pragma unroll
                     for (int pb = 0; pb < NP; ++pb) {
                         const u32 *q = plb[pb] + (ig >> 5);
@@ -880,21 +895,24 @@ __device__ void filter_task(const FilterCtx<D> &c, u32 sp, u32 phi, u32 *qb, u64
                     }
                 }
                 __builtin_amdgcn_sched_barrier(0);
-                u64 mism = 0;
-                u32 mism_hi = 0;
+                // mismatches of anchors 0-31, 32-63 and 64-95, one 3-input
+                // bit operation per window dword
+                u32 m0 = 0, m1 = 0, m2 = 0;
 #pragma unroll
                 for (int pb = 0; pb < NP; ++pb) {
-                    const u64 base = window_of(braw[pb][0], braw[pb][1], braw[pb][2], sh);
+                    const u32 b0 = __builtin_amdgcn_alignbit(braw[pb][1], braw[pb][0], sh);
+                    const u32 b1 = __builtin_amdgcn_alignbit(braw[pb][2], braw[pb][1], sh);
+                    const u32 b2 = __builtin_amdgcn_alignbit(braw[pb][3], braw[pb][2], sh);
 #pragma unroll
-                    for (int g = 0; g < G; ++g)
-                        mism |= base ^ window_of(raw[g][pb][0], raw[g][pb][1], raw[g][pb][2], shg[g]);
-                    const u32 base_hi = __builtin_amdgcn_alignbit(braw[pb][3], braw[pb][2], sh);
-#pragma unroll
-                    for (int g = 0; g < G; ++g)
-                        mism_hi |= base_hi ^ __builtin_amdgcn_alignbit(raw[g][pb][3], raw[g][pb][2], shg[g]);
+                    for (int g = 0; g < G; ++g) {
+                        const u32 *w = raw[g][pb];
+                        m0 = or_xor(m0, b0, __builtin_amdgcn_alignbit(w[1], w[0], shg[g]));
+                        m1 = or_xor(m1, b1, __builtin_amdgcn_alignbit(w[2], w[1], shg[g]));
+                        m2 = or_xor(m2, b2, __builtin_amdgcn_alignbit(w[3], w[2], shg[g]));
+                    }
                 }
-                live &= ~mism;
-                live_hi &= ~mism_hi;
+                live &= ~(((u64)m1 << 32) | m0);
+                live_hi &= ~m2;
                 j += G;
                 io -= G * (int)dg;
                 if (!(live | live_hi)) {
@@ -964,16 +982,14 @@ __device__ void filter_task(const FilterCtx<D> &c, u32 sp, u32 phi, u32 *qb, u64
 
 constexpr u32 kFilterWaves = 4;  // waves per filter block (blockDim 256)
 
-constexpr u32 kChunk = 32;  // filter tasks claimed per queue access
-
-// Next task of the wave: the rest of its chunk, then the chunk claimed
-// during the chunk's last task (`claim` is valid in lane 0).
-__device__ __forceinline__ u32 next_task(u32 &k, u32 &chunk, u32 claim) {
-    if (++k == kChunk) {
-        chunk = __builtin_amdgcn_readfirstlane(claim);
+// Next task of the wave: the rest of its chunk of `chunk` tasks, then the
+// chunk claimed during the chunk's last task (`claim` is valid in lane 0).
+__device__ __forceinline__ u32 next_task(u32 &k, u32 &first, u32 claim, u32 chunk) {
+    if (++k == chunk) {
+        first = __builtin_amdgcn_readfirstlane(claim);
         k = 0;
     }
-    return chunk + k;
+    return first + k;
 }
 
 // Context of one (profile, denominator) task; false when the task has
@@ -1039,7 +1055,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void s
                                                  u32 *__restrict__ failed,
                                                  unsigned long long *__restrict__ stats,
                                                  u32 *__restrict__ queue, u32 kGroup,
-                                                 u32 ext_limit) {
+                                                 u32 chunk, u32 ext_limit) {
     // per-lane work counts: window comparisons in 32-anchor plane dwords,
     // candidate words, completed anchors (summed per block, three atomics
     // per block)
@@ -1066,7 +1082,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void s
     // then touch the planes of at most kGroup profiles at a time (they stay
     // in its L2), while each profile's denominators are still spread over
     // the group.  Tasks are taken from a per-XCD counter (a device work
-    // queue, zeroed before the launch) in chunks of kChunk consecutive
+    // queue, zeroed before the launch) in chunks of `chunk` consecutive
     // tasks, so the tasks in flight on an XCD stay within a short window
     // however many waves are resident or queued; the next chunk is claimed
     // while the last task of the current one runs.  A wave whose XCD has no
@@ -1080,10 +1096,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void s
     const u32 ngroups = (nprof_x + kGroup - 1) / kGroup;
     const u32 items = ngroups * kGroup * dmax;  // checked below 2^32 by the host
     u32 claim = 0;
-    if (lane == 0) claim = atomicAdd(&queue[xcd], kChunk);
-    u32 chunk = __builtin_amdgcn_readfirstlane(claim), k = 0;
-    for (u32 item = chunk; item < items; item = next_task(k, chunk, claim)) {
-        if (k == kChunk - 1u && lane == 0) claim = atomicAdd(&queue[xcd], kChunk);
+    if (lane == 0) claim = atomicAdd(&queue[xcd], chunk);
+    u32 first = __builtin_amdgcn_readfirstlane(claim), k = 0;
+    for (u32 item = first; item < items; item = next_task(k, first, claim, chunk)) {
+        if (k == chunk - 1u && lane == 0) claim = atomicAdd(&queue[xcd], chunk);
         const u32 g = item / (kGroup * dmax);
         const u32 rem = item - g * (kGroup * dmax);
         const u32 dd = __builtin_amdgcn_readfirstlane(1u + rem / kGroup);
@@ -1202,15 +1218,16 @@ extern "C" hipError_t rbg_sparse_filter(const SparsePrime *P, const SparseProf *
                                         const uint16_t *tab_lab, unsigned long long *best,
                                         uint32_t *failed, unsigned long long *stats,
                                         uint32_t *queue, int wide, uint32_t grid, uint32_t group,
-                                        uint32_t extend_limit, hipStream_t stream) {
+                                        uint32_t chunk, uint32_t extend_limit,
+                                        hipStream_t stream) {
     if (wide)
         hipLaunchKernelGGL(rbg::sp_filter<rbg::Dom33>, dim3(grid), dim3(256), 0, stream, P, prof,
                            nprof, dmax, planes, (const uint64_t *)inv, (const uint64_t *)tab_val,
-                           tab_lab, best, failed, stats, queue, group, extend_limit);
+                           tab_lab, best, failed, stats, queue, group, chunk, extend_limit);
     else
         hipLaunchKernelGGL(rbg::sp_filter<rbg::Dom32>, dim3(grid), dim3(256), 0, stream, P, prof,
                            nprof, dmax, planes, (const uint32_t *)inv, (const uint32_t *)tab_val,
-                           tab_lab, best, failed, stats, queue, group, extend_limit);
+                           tab_lab, best, failed, stats, queue, group, chunk, extend_limit);
     return hipGetLastError();
 }
 

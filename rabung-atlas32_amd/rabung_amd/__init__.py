@@ -65,7 +65,8 @@ class Config(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int32), ("tile_chunks", ctypes.c_uint32),
                 ("blocks_per_cu", ctypes.c_uint32), ("hbm_budget", ctypes.c_uint64),
                 ("xcd_group", ctypes.c_uint32), ("cover_q", ctypes.c_uint32),
-                ("extend_limit", ctypes.c_uint32), ("floor_boost", ctypes.c_uint32)]
+                ("extend_limit", ctypes.c_uint32), ("floor_boost", ctypes.c_uint32),
+                ("cover_div", ctypes.c_uint32), ("filter_chunk", ctypes.c_uint32)]
 
 
 EXPORTS = ["rbg_open", "rbg_profile_block", "rbg_profile_block_device",
@@ -258,10 +259,10 @@ class Engine:
     """One library context (device, stream, workspaces)."""
 
     def __init__(self, device=0, tile_chunks=0, blocks_per_cu=0, hbm_budget=0, xcd_group=0,
-                 cover_q=0, extend_limit=0, floor_boost=0):
+                 cover_q=0, extend_limit=0, floor_boost=0, cover_div=0, filter_chunk=0):
         lib = load()
         cfg = Config(device, tile_chunks, blocks_per_cu, hbm_budget, xcd_group, cover_q,
-                     extend_limit, floor_boost)
+                     extend_limit, floor_boost, cover_div, filter_chunk)
         self._ctx = lib.rbg_open(ctypes.byref(cfg))
         if not self._ctx:
             raise RabungError(2, "rbg_open failed (no HIP device %d?)" % device)

@@ -219,6 +219,37 @@ def test_cover_threshold_outside_range_refused():
             eng.profile_block([101])
     assert e.value.status == 1
 
+
+@pytest.mark.parametrize("div,chunk", [(3, 1), (4, 8), (8, 256)])
+def test_cover_shape_and_chunk_do_not_change_records(div, chunk):
+    """Cover shapes n = 3, 4, 8 (fewer denominators, wider tables; near 2^32
+    n = 8 hits the 2^20 table limit and D grows back) and task chunks of 1,
+    8 and 256 give the golden records of the top of block 49900 and the
+    default engine's records on seeded primes across the sparse domain."""
+    rows, a4, _ = golden_data.sample_block(49900)
+    tail = rows[-40:]
+    rng = random.Random(div * 1000 + chunk)
+    primes = sorted({rabung_amd.primes_from(int(2.0 ** rng.uniform(17, 32)), 1)[0]
+                     for _ in range(24)})
+    with rabung_amd.Engine() as eng:
+        ref = eng.profile_block(primes)
+    with rabung_amd.Engine(cover_div=div, filter_chunk=chunk) as eng:
+        got = eng.profile_block(primes)
+        top = eng.profile_block([r["p"] for r in tail])
+        t = eng.last_timing()
+    assert got == ref
+    assert codec.records_to_rows(top) == tail
+    assert [g["A4"] for g in top] == a4[-40:]
+    assert t["sparse_primes"] == 40 and t["fallbacks"] == 0
+
+
+@pytest.mark.parametrize("kw", [{"cover_div": 1}, {"cover_div": 9}, {"filter_chunk": 257}])
+def test_cover_shape_and_chunk_outside_range_refused(kw):
+    with rabung_amd.Engine(**kw) as eng:
+        with pytest.raises(rabung_amd.RabungError) as e:
+            eng.profile_block([101])
+    assert e.value.status == 1
+
 def test_methods_agree_and_are_deterministic(engine):
     primes = oracle.next_primes(3_000_000_000, 16) + [4294967291, 4273768801]
     a = engine.profile_block(primes, method=rabung_amd.METHOD_BATCHED)
