@@ -105,11 +105,11 @@ typedef struct rbg_config {
     uint32_t cover_div;      /* sparse engine: cover shape n, D = floor(2 sqrt(p/H) / n)
                                 (H = Q - 1), B the least radius certified for that D;
                                 larger n trades a wider table for fewer denominators
-                                (paper/GPU_METHOD_EN.tex:322-336).  2..8; 0 = 2, the
-                                reference's D = floor(sqrt(p/H)).  D grows back toward
-                                the n = 2 value where B would pass 2^20 */
+                                (paper/GPU_METHOD_EN.tex:322-336; n = 2 is the
+                                reference's D = floor(sqrt(p/H))).  2..8; 0 = 3.  Where B
+                                would pass 2^20 - 2, D grows back toward the n = 2 value */
     uint32_t filter_chunk;   /* sparse filter: tasks one wave claims from its XCD's
-                                queue at a time, 1..256; 0 = 32 */
+                                queue at a time, 1..256; 0 = 16 */
 } rbg_config;
 
 typedef struct rbg_ctx rbg_ctx;
@@ -166,7 +166,7 @@ int rbg_profile_block_device(rbg_ctx *ctx, const uint32_t *d_primes, uint32_t n,
  * per-device lock inside the process, so calls that overlap on one GPU wait
  * for each other (the call returns only when its work is complete).  Returns
  * RBG_ECAPACITY when the block needs more workspace than the array holds
- * (about 90 primes near 2^32 with the batched method), RBG_EINVAL for the
+ * (about 30 primes near 2^32 with the batched method), RBG_EINVAL for the
  * RBG_GRAPH flag; RBG_CERTIFY is ignored as in rbg_profile_block_device. */
 int rbg_profile_block_noalloc(rbg_ctx *ctx, const uint32_t *d_primes, uint32_t n,
                               uint32_t flags, rbg_record *d_out);

@@ -68,7 +68,8 @@ extern "C" hipError_t rbg_sparse_seed(const SparsePrime *P, SparseProf *prof, ui
                                       unsigned long long *best, uint32_t *failed,
                                       hipStream_t stream);
 extern "C" hipError_t rbg_sparse_filter(const SparsePrime *P, const SparseProf *prof,
-                                        uint32_t nprof, uint32_t dmax, const uint32_t *planes,
+                                        uint32_t nprof, uint32_t dmax, const uint32_t *tpre,
+                                        const uint32_t *planes,
                                         const void *inv, const void *tab_val,
                                         const uint16_t *tab_lab, unsigned long long *best,
                                         uint32_t *failed, unsigned long long *stats,
@@ -260,7 +261,7 @@ u64 cover_radius(u64 p, u64 H, u64 d) {
 
 // Cover shape n (paper/GPU_METHOD_EN.tex:322-336): D = floor(2 sqrt(p/H) / n),
 // so n = 2 is the reference's D = floor(sqrt(p/H)) and B D ~ p (1 + 1/n^2).
-// Where the least radius for that D would pass the table limit 2^20, D grows
+// Where the least radius for that D would pass the table limit, D grows
 // to the least value whose radius fits (never past the n = 2 value).  The
 // returned pair is certified with exact integers.
 bool choose_cover(u64 p, u32 Q, u32 div, u32 &B, u32 &D) {
@@ -270,7 +271,9 @@ bool choose_cover(u64 p, u32 Q, u32 div, u32 &B, u32 &D) {
     while (dmax > 1 && H * dmax * dmax > 4ull * p) --dmax;
     u64 d = n <= 2 ? dmax : std::max<u64>(1, isqrt64(4ull * p / (H * n * n)));
     if (d > dmax) d = dmax;
-    constexpr u64 kMaxB = 1u << 20;
+    // table radius limit: 4096 primes of at most 2^20 - 1 label entries each
+    // keep a block's label offsets below 2^32
+    constexpr u64 kMaxB = (1u << 20) - 2;
     if (d < dmax && cover_radius(p, H, d) > kMaxB) {
         // radius(d) ~ p/d + H d/4 falls as d grows below 2 sqrt(p/H): start
         // from the real root of H d^2 - 4 kMaxB d + 4p = 0 and step to the
@@ -372,12 +375,13 @@ struct rbg_ctx {
     DevBuf tab_lab, dense_lab;
     // sparse path
     DevBuf sp_primes, sp_profs, sp_val, sp_val64, sp_lab, sp_labels, sp_inv, sp_inv64, sp_planes,
-        sp_best, sp_failed, sp_a4, sp_stats, sp_segs;
+        sp_best, sp_failed, sp_a4, sp_stats, sp_segs, sp_tpre;
     std::vector<SparsePrime> h_sp;
     std::vector<SparseProf> h_prof;
     std::vector<u32> h_sval;
     std::vector<u64> h_sval64;
     std::vector<u16> h_slab;
+    std::vector<u32> h_tpre;  // filter task lists (first task of each profile per XCD)
     u32 nprof32 = 0;  // profiles of primes below 2^32 (they come first)
     // timing of the last call
     rbg_timing timing{};
@@ -396,7 +400,8 @@ struct rbg_ctx {
     std::vector<DevBuf *> buffers() {
         return {&descs, &tab_val, &tab_val64, &tab_code, &tab_lab, &dense_lab, &hdr, &runs,
                 &recs, &flags, &sp_primes, &sp_profs, &sp_val, &sp_val64, &sp_lab, &sp_labels, &sp_inv,
-                &sp_inv64, &sp_planes, &sp_best, &sp_failed, &sp_a4, &sp_stats, &sp_segs, &sel};
+                &sp_inv64, &sp_planes, &sp_best, &sp_failed, &sp_a4, &sp_stats, &sp_segs, &sp_tpre,
+                &sel};
     }
 
     int fail(int code, const std::string &msg) {
@@ -472,9 +477,9 @@ u64 budget(const rbg_ctx *c) { return c->cfg.hbm_budget ? c->cfg.hbm_budget : (1
 
 u32 cover_q(const rbg_ctx *c) { return c->cfg.cover_q ? c->cfg.cover_q : kCoverQ; }
 
-u32 cover_div(const rbg_ctx *c) { return c->cfg.cover_div ? c->cfg.cover_div : 2u; }
+u32 cover_div(const rbg_ctx *c) { return c->cfg.cover_div ? c->cfg.cover_div : 3u; }
 
-u32 filter_chunk(const rbg_ctx *c) { return c->cfg.filter_chunk ? c->cfg.filter_chunk : 32u; }
+u32 filter_chunk(const rbg_ctx *c) { return c->cfg.filter_chunk ? c->cfg.filter_chunk : 16u; }
 
 u32 extend_limit(const rbg_ctx *c) {
     return c->cfg.extend_limit ? c->cfg.extend_limit : kExtendLimit;
@@ -975,10 +980,34 @@ int sparse_run(rbg_ctx *c, rbg_record *recs, u32 *err, std::vector<u32> &failed_
     u32 *failed = (u32 *)c->sp_failed.ptr;
     // the filter's XCD-aware schedule needs a grid that is a multiple of 8
     const u32 grid = std::max(8u, ((u32)c->cus * blocks_per_cu(c) + 7u) / 8u * 8u);
+    // task lists of the XCDs, per filter launch (domain): XCD x owns the
+    // launch's profiles x, x+8, ...; its list holds their denominators 1..D
+    // back to back, and tpre holds, per XCD, the first task of each of its
+    // profiles followed by the list's length (nf + 16 entries in all)
+    const u32 nfd[2] = {c->nprof32, nf - c->nprof32};
+    std::vector<u32> &tpre = c->h_tpre;
+    tpre.clear();
+    tpre.reserve(nf + 16);
+    u64 most = 0;
+    for (int wide = 0; wide < 2; ++wide) {
+        const u32 f0 = wide ? c->nprof32 : 0u;
+        for (u32 x = 0; x < 8; ++x) {
+            u64 at = 0;
+            for (u32 k = x; k < nfd[wide]; k += 8) {
+                tpre.push_back((u32)at);
+                at += c->h_sp[c->h_prof[f0 + k].prime].D;
+            }
+            tpre.push_back((u32)at);
+            most = std::max(most, at);
+        }
+    }
     // a per-XCD counter ends at most one claim per wave past its items
-    if (std::max(dmax[0], dmax[1]) * (nf + 8ull * xcd_group(c)) +
+    if (std::max<u64>(std::max(dmax[0], dmax[1]) * (nf + 8ull * xcd_group(c)), most) +
             4ull * grid * filter_chunk(c) > 0xFFFFFFFFull)
         return c->fail(RBG_ECAPACITY, "too many filter tasks");
+    HIPCK(c, c->sp_tpre.ensure(tpre.size() * 4));
+    HIPCK(c, hipMemcpyAsync(c->sp_tpre.ptr, tpre.data(), tpre.size() * 4, hipMemcpyHostToDevice,
+                            s));
     // one chain: clear flags and counters, tables and planes, seed, filter
     // (eagerly with an event per stage; in graph mode the whole chain is
     // timed as the filter)
@@ -1001,11 +1030,12 @@ int sparse_run(rbg_ctx *c, rbg_record *recs, u32 *err, std::vector<u32> &failed_
         if (!c->graph) HIPCK(c, hipEventRecord(c->ev[kFilt0], s));
         // one filter launch per domain, each over its own profiles (the
         // profiles of the primes below 2^32 come first)
-        const u32 nfd[2] = {c->nprof32, nf - c->nprof32};
         for (int wide = 0; wide < 2; ++wide) {
             if (!nfd[wide]) continue;
             const u32 f0 = wide ? c->nprof32 : 0u;
-            HIPCK(c, rbg_sparse_filter(dP, dF + f0, nfd[wide], (u32)dmax[wide],
+            // the 32-bit launch's lists take nprof32 + 8 entries of tpre
+            const u32 *tp = (const u32 *)c->sp_tpre.ptr + (wide ? c->nprof32 + 8u : 0u);
+            HIPCK(c, rbg_sparse_filter(dP, dF + f0, nfd[wide], (u32)dmax[wide], tp,
                                        (const u32 *)c->sp_planes.ptr,
                                        wide ? c->sp_inv64.ptr : c->sp_inv.ptr,
                                        wide ? c->sp_val64.ptr : c->sp_val.ptr,
@@ -1064,6 +1094,29 @@ u64 prime_bytes(u64 p, u32 flags, u32 tile_pos, u32 q, u32 div) {
     }
     const u64 tiles = ((p - 1) / 2 + tile_pos - 1) / tile_pos;
     return bytes + tiles * (sizeof(TileHdr) + nr * sizeof(RunSum));
+}
+
+// Offsets one sparse prime adds to a call's label array (u16 entries), plane
+// array (units of 16 words) and plane-block grid; each has a 32-bit (blocks:
+// 31-bit) range per call.  Zero for a prime the dense scan takes.
+struct SparseExtent {
+    u64 labels = 0, words16 = 0, blocks = 0;
+};
+
+SparseExtent sparse_extent(u64 p, u32 flags, u32 q, u32 div) {
+    SparseExtent e;
+    u32 B = 0, D = 0;
+    if ((flags & RBG_METHOD_MASK) != RBG_METHOD_BATCHED || p < kSparseMin ||
+        !choose_cover(p, q, div, B, D))
+        return e;
+    const u32 mask = mask_of(p);
+    e.labels = B + 1ull;
+    for (u32 r = 2; r <= 9; ++r)
+        if (mask & (1u << (r - 2))) {
+            e.words16 += rbg::row_offset(rbg::kRows, B, 32u - (u32)__builtin_clz(r - 1)) / 16u;
+            for (u32 row = 0; row < rbg::kRows; ++row) e.blocks += rbg::row_blocks(row, B);
+        }
+    return e;
 }
 
 // Profiles a block of primes below `pmax` (2^32 for the 32-bit entry points,
@@ -1164,10 +1217,19 @@ int run_batch(rbg_ctx *c, const u64 *primes, u32 n, u32 flags, rbg_record *out, 
     while (first < n) {
         u32 last = first;
         u64 bytes = 0;
+        SparseExtent used;
         while (last < n && last - first < kBatchChunk) {
             const u64 b = prime_bytes(primes[last], flags, tile_pos, cover_q(c), cover_div(c));
-            if (last > first && bytes + b > cap) break;
+            const SparseExtent e = sparse_extent(primes[last], flags, cover_q(c), cover_div(c));
+            // a sub-batch also keeps the sparse engine's per-call offsets in range
+            const bool full = used.labels + e.labels > 0xFFFFFFFFull ||
+                              used.words16 + e.words16 > 0xFFFFFFFFull ||
+                              used.blocks + e.blocks > 0x7FFFFFFFull;
+            if (last > first && (bytes + b > cap || full)) break;
             bytes += b;
+            used.labels += e.labels;
+            used.words16 += e.words16;
+            used.blocks += e.blocks;
             ++last;
         }
         int rc = run_block(c, primes + first, last - first, flags, nullptr, out + first,

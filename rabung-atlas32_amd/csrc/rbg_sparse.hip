@@ -1047,7 +1047,8 @@ __device__ __forceinline__ bool task_setup(const SparsePrime *P, const SparsePro
 template <class D>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void sp_filter(const SparsePrime *__restrict__ P,
                                                  const SparseProf *__restrict__ prof, u32 nprof,
-                                                 u32 dmax, const u32 *__restrict__ planes,
+                                                 u32 dmax, const u32 *__restrict__ tpre,
+                                                 const u32 *__restrict__ planes,
                                                  const typename D::V *__restrict__ inv,
                                                  const typename D::V *__restrict__ tab_val,
                                                  const u16 *__restrict__ tab_lab,
@@ -1093,18 +1094,47 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void s
     for (u32 v = 0; v < kXcds; ++v) {
     const u32 xcd = (blockIdx.x + v) % kXcds;
     const u32 nprof_x = nprof > xcd ? (nprof - xcd + kXcds - 1) / kXcds : 0u;
+    // groups of one profile (the usual shape): the XCD's tasks are its
+    // profiles' denominators 1..D back to back, tp[l] the first task of its
+    // profile l and tp[nprof_x] their number, so no claimed task is empty;
+    // larger groups interleave the denominators of kGroup profiles up to
+    // the launch's largest D (tasks past a profile's own D return at once)
+    u32 off = xcd;  // the lists before this XCD's, one entry more than profiles each
+    for (u32 x = 0; x < xcd; ++x) off += nprof > x ? (nprof - x + kXcds - 1) / kXcds : 0u;
+    const u32 *tp = tpre + off;
     const u32 ngroups = (nprof_x + kGroup - 1) / kGroup;
-    const u32 items = ngroups * kGroup * dmax;  // checked below 2^32 by the host
+    // checked below 2^32 by the host
+    const u32 items = kGroup == 1 ? (nprof_x ? tp[nprof_x] : 0u) : ngroups * kGroup * dmax;
+    u32 cur = 0, lo = 0, hi = 0;  // profile of the list being walked and its tasks [lo, hi)
     u32 claim = 0;
     if (lane == 0) claim = atomicAdd(&queue[xcd], chunk);
     u32 first = __builtin_amdgcn_readfirstlane(claim), k = 0;
     for (u32 item = first; item < items; item = next_task(k, first, claim, chunk)) {
         if (k == chunk - 1u && lane == 0) claim = atomicAdd(&queue[xcd], chunk);
-        const u32 g = item / (kGroup * dmax);
-        const u32 rem = item - g * (kGroup * dmax);
-        const u32 dd = __builtin_amdgcn_readfirstlane(1u + rem / kGroup);
-        const u32 local = g * kGroup + rem % kGroup;
-        if (local >= nprof_x) continue;
+        u32 dd, local;
+        if (kGroup == 1) {
+            if (item >= hi) {
+                // the profile holding `item`: tasks come in increasing order,
+                // so it is at or after the current one (binary search)
+                u32 a = cur, b = nprof_x - 1u;
+                while (a < b) {
+                    const u32 m = (a + b) >> 1;
+                    if (tp[m + 1] > item) b = m;
+                    else a = m + 1u;
+                }
+                cur = a;
+                lo = tp[a];
+                hi = tp[a + 1];
+            }
+            dd = __builtin_amdgcn_readfirstlane(1u + item - lo);
+            local = cur;
+        } else {
+            const u32 g = item / (kGroup * dmax);
+            const u32 rem = item - g * (kGroup * dmax);
+            dd = __builtin_amdgcn_readfirstlane(1u + rem / kGroup);
+            local = g * kGroup + rem % kGroup;
+            if (local >= nprof_x) continue;
+        }
         const u32 kk = __builtin_amdgcn_readfirstlane(xcd + local * kXcds);
         FilterCtx<D> c;
         u32 sp = 0, phi = 0;
@@ -1213,7 +1243,8 @@ extern "C" hipError_t rbg_sparse_seed(const SparsePrime *P, SparseProf *prof, ui
 // profiles of primes below 2^32 (inv and tab_val hold u32 values), 1 for the
 // wider primes (u64 values).
 extern "C" hipError_t rbg_sparse_filter(const SparsePrime *P, const SparseProf *prof,
-                                        uint32_t nprof, uint32_t dmax, const uint32_t *planes,
+                                        uint32_t nprof, uint32_t dmax, const uint32_t *tpre,
+                                        const uint32_t *planes,
                                         const void *inv, const void *tab_val,
                                         const uint16_t *tab_lab, unsigned long long *best,
                                         uint32_t *failed, unsigned long long *stats,
@@ -1222,11 +1253,11 @@ extern "C" hipError_t rbg_sparse_filter(const SparsePrime *P, const SparseProf *
                                         hipStream_t stream) {
     if (wide)
         hipLaunchKernelGGL(rbg::sp_filter<rbg::Dom33>, dim3(grid), dim3(256), 0, stream, P, prof,
-                           nprof, dmax, planes, (const uint64_t *)inv, (const uint64_t *)tab_val,
+                           nprof, dmax, tpre, planes, (const uint64_t *)inv, (const uint64_t *)tab_val,
                            tab_lab, best, failed, stats, queue, group, chunk, extend_limit);
     else
         hipLaunchKernelGGL(rbg::sp_filter<rbg::Dom32>, dim3(grid), dim3(256), 0, stream, P, prof,
-                           nprof, dmax, planes, (const uint32_t *)inv, (const uint32_t *)tab_val,
+                           nprof, dmax, tpre, planes, (const uint32_t *)inv, (const uint32_t *)tab_val,
                            tab_lab, best, failed, stats, queue, group, chunk, extend_limit);
     return hipGetLastError();
 }

@@ -411,6 +411,24 @@ def test_batch_across_blocks(engine):
     assert [rabung_amd.record_to_dict(raw[i]) for i in range(len(flat))] == got
 
 
+def test_batch_of_top_primes_splits_by_table_offsets():
+    """About 8000 primes just below 2^32 hold more than 2^32 label entries
+    (B ~ 562k each), so a batch with room in its HBM budget must still cut
+    them into calls whose table offsets fit 32 bits.  The top 4096 are sample
+    block 49900; the primes below them must equal one block call."""
+    rows, a4, _ = golden_data.sample_block(49900)
+    lo = rows[0]["p"]
+    below = [p for p in oracle.next_primes(lo - 88000, 4400) if p < lo]
+    assert 3800 < len(below) <= 4096
+    flat = below + [r["p"] for r in rows]
+    with rabung_amd.Engine(hbm_budget=64 << 30) as eng:
+        got = eng.profile_batch(flat)
+        assert eng.last_timing()["sparse_primes"] == len(flat)
+        assert as_rows(got[len(below):]) == rows
+        assert [g["A4"] for g in got[len(below):]] == a4
+        assert got[:len(below)] == eng.profile_block(below)
+
+
 @pytest.mark.parametrize("method", [rabung_amd.METHOD_BATCHED, rabung_amd.METHOD_PLAIN,
                                     rabung_amd.METHOD_DENSE])
 def test_graph_capture_replays(method):

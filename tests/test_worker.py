@@ -129,7 +129,7 @@ def test_recompute_sample_block(args, primes, frame):
 
 @pytest.mark.gpu
 def test_profile_capacity_refusal_replays_in_halves():
-    """A workspace budget (8 MB) below what a 64-prime batch near 2^32 needs
+    """A workspace budget (12 MB) below what a 64-prime batch near 2^32 needs
     makes the library refuse with RBG_ECAPACITY; the worker replays the
     batch in halves (the reference's replay rule, GPU_CONFIG.json:164-170)
     and its output equals an unconstrained run's."""
@@ -137,7 +137,7 @@ def test_profile_capacity_refusal_replays_in_halves():
     batch = rows[-64:]
     text = "0 %s\nQUIT\n" % " ".join(str(r["p"]) for r in batch)
     free = run("profile", text)
-    tight = run("profile", text, "--hbm-budget", "8000000")
+    tight = run("profile", text, "--hbm-budget", "12000000")
     assert free.returncode == 0 and tight.returncode == 0, tight.stderr
     assert b"capacity refusal of 64 primes" in tight.stderr
     assert b"capacity refusal" not in free.stderr
