@@ -312,6 +312,23 @@ def golden_check(primes, records):
     return checked
 
 
+def memory_path(timing, kern_ms, info):
+    """The filter against the unit that bounds it: every wave-level 4-dword
+    load moves 64 lanes x 16 bytes through a CU's vector-memory path (TA/TD,
+    whatever the active lanes), which delivers 64 bytes per clock per CU
+    (profiles/valu_and_load_rates.md: 16-17 cycles per such load)."""
+    loads = timing.get("filter_loads", 0)
+    if not loads:
+        return None
+    bytes_moved = loads * 64 * 16
+    peak = info["cus"] * 64 * info["clock_khz"] * 1e3 / 1e12  # TB/s
+    achieved = bytes_moved / (kern_ms / 1000.0) / 1e12
+    return {"bound": "vector-memory path (TA/TD)", "wave_loads": loads,
+            "bytes_per_launch": bytes_moved, "achieved": round(achieved, 3),
+            "peak": round(peak, 3), "unit": "TB/s", "frac": round(achieved / peak, 4),
+            "peak_basis": "%d CUs x 64 B x %.0f MHz" % (info["cus"], info["clock_khz"] / 1e3)}
+
+
 def host_threads():
     """Host cores this process may run on (the box's CPU share)."""
     try:
@@ -571,6 +588,12 @@ def main():
                               "montgomery_products_per_s": round(ops / OPS_PER_PRODUCT / scan_s, 0),
                               "frac_of_measured_product_rate": round(
                                   ops / OPS_PER_PRODUCT / scan_s / probe["montgomery_products_per_s"], 4),
+                              "positions_per_s": round(ptimings[-1]["positions"] / scan_s, 0),
+                              # the same time priced at one power per position
+                              # (the scan without label reuse): the work the
+                              # labels replaced counted as done
+                              "frac_one_power_per_position": round(
+                                  int32_ops(primes) / scan_s / 1e12 / peak_tops, 4),
                               "probe": probe}}
 
     # dominant kernel of the batched path: the sparse anchor filter (or the
@@ -599,7 +622,8 @@ def main():
                                  "a4_host": round(timing["a4_host_ms"], 3),
                                  "a4_wait": round(timing["a4_wait_ms"], 3),
                                  "call_total": round(timing["total_ms"], 3)},
-                    "peak_basis": peak_basis, "probe": probe}
+                    "peak_basis": peak_basis, "probe": probe,
+                    "memory_path": memory_path(timing, kern_ms, info)}
     else:
         ops = int32_ops(primes)
         kern_ms = sum(t["dense_scan_ms"] for t in timings) / len(timings)

@@ -236,6 +236,10 @@ typedef struct rbg_timing {
     double a4_wait_ms;       /* of that, time the call waited after the device was done */
     uint64_t rescans;        /* dense scans repeated with every run tracked (a longest
                                 run below its floor; expected never above small primes) */
+    uint64_t filter_loads;   /* wave-level 4-dword load instructions of the filter's word
+                                loop (stage and base windows; each moves 64 x 16 bytes
+                                through a CU's vector-memory path whatever the active
+                                lanes) */
 } rbg_timing;
 
 int rbg_last_timing(const rbg_ctx *ctx, rbg_timing *out);

@@ -337,7 +337,7 @@ constexpr u32 kCoverQ = 48;           // default threshold cap of the cover
 constexpr u32 kMaxCoverQ = 63;        // the screen counts up to q - t = 31 (t = ceil(q/2))
 constexpr u32 kExtendLimit = 1u << 22;  // default extension steps of one passing anchor
 constexpr u32 kBatchChunk = 16384;    // primes per launch chain of a batch
-// filter statistics (3 x u64) and the filter's per-XCD task counters (8 x u32
+// filter statistics (4 x u64) and the filter's per-XCD task counters (8 x u32
 // per launch: primes below 2^32, wider primes)
 constexpr size_t kQueueOffset = 32, kStatsBytes = kQueueOffset + 16 * 4;
 
@@ -1051,7 +1051,7 @@ int sparse_run(rbg_ctx *c, rbg_record *recs, u32 *err, std::vector<u32> &failed_
     HIPCK(c, hipEventRecord(c->ev[kFilt1], s));
     c->timing.launches += 6;
     std::vector<u32> h_failed(np, 0);
-    u64 h_stats[3] = {0, 0, 0};
+    u64 h_stats[4] = {0, 0, 0, 0};
     HIPCK(c, hipMemcpyAsync(h_failed.data(), failed, np * 4, hipMemcpyDeviceToHost, s));
     HIPCK(c, hipMemcpyAsync(h_stats, c->sp_stats.ptr, sizeof h_stats, hipMemcpyDeviceToHost, s));
     HIPCK(c, hipStreamSynchronize(s));
@@ -1073,6 +1073,7 @@ int sparse_run(rbg_ctx *c, rbg_record *recs, u32 *err, std::vector<u32> &failed_
     c->timing.filter_windows = h_stats[0];
     c->timing.filter_words = h_stats[1];
     c->timing.filter_completions = h_stats[2];
+    c->timing.filter_loads = h_stats[3];
     return RBG_OK;
 }
 
@@ -1252,6 +1253,7 @@ int run_batch(rbg_ctx *c, const u64 *primes, u32 n, u32 flags, rbg_record *out, 
         sum.filter_windows += t.filter_windows;
         sum.filter_words += t.filter_words;
         sum.filter_completions += t.filter_completions;
+        sum.filter_loads += t.filter_loads;
         sum.a4_host_ms += t.a4_host_ms;
         sum.a4_wait_ms += t.a4_wait_ms;
         sum.rescans += t.rescans;

@@ -736,7 +736,7 @@ constexpr u32 kRefill = 48;  // smallest cohort of idle lanes that takes new wor
 
 template <class D, int NP>
 __device__ void filter_task(const FilterCtx<D> &c, u32 sp, u32 phi, u32 *qb, u64 *qs, u32 *cw,
-                            u32 &n_windows, u32 &n_words, u32 &n_done) {
+                            u32 &n_windows, u32 &n_words, u32 &n_done, u32 &n_loads) {
     constexpr u32 WB = kWordBits;
     constexpr u32 NW = 4;  // dwords per window load
     constexpr int G = NP == 1 ? 6 : NP == 2 ? 3 : NP == 3 ? 2 : 1;  // stages per step (register budget)
@@ -872,12 +872,14 @@ __device__ void filter_task(const FilterCtx<D> &c, u32 sp, u32 phi, u32 *qb, u64
                     }
                 }
                 next += (u32)__builtin_popcountll(idle);
+                n_loads += (u32)NP;  // the new words' base windows
             }
             const u64 busy = ballot64(have);
             if (!busy) {
                 if (stall || next >= total) break;
                 continue;
             }
+            n_loads += (u32)(G * NP);  // the step's window loads, one per stage and plane
             bool fin = false;
             if (have) {
                 // every load of the step is issued before the first use
@@ -1057,21 +1059,23 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void s
                                                  unsigned long long *__restrict__ stats,
                                                  u32 *__restrict__ queue, u32 kGroup,
                                                  u32 chunk, u32 ext_limit) {
-    // per-lane work counts: window comparisons in 32-anchor plane dwords,
-    // candidate words, completed anchors (summed per block, three atomics
-    // per block)
-    __shared__ unsigned long long s_stats[3];
+    // work counts: window comparisons in 32-anchor plane dwords, candidate
+    // words, completed anchors (per lane) and the word loop's wave-level
+    // load instructions (per wave), summed per block (four atomics per
+    // block)
+    __shared__ unsigned long long s_stats[4];
     // per-wave queue of passing words: start b0 and surviving anchors
     __shared__ u32 s_qb[kFilterWaves][kQueue];
     __shared__ u64 s_qs[kFilterWaves][kQueue];
     // per-wave word parameters of the current task's classes (16 words)
     __shared__ __attribute__((aligned(16))) u32 s_cw[kFilterWaves][16];
-    if (threadIdx.x < 3) s_stats[threadIdx.x] = 0;
+    if (threadIdx.x < 4) s_stats[threadIdx.x] = 0;
     __syncthreads();
     // u32 per lane, added to the block's totals before they can overflow: a
     // task adds less than 2^28 to n_windows and n_done, and n_words stays
     // below n_windows (every word takes at least one step)
     u32 n_windows = 0, n_words = 0, n_done = 0;
+    u32 n_loads = 0;  // wave-level load instructions of the word loop (wave-uniform)
     const u32 wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     u32 *qb = s_qb[wave];
     u64 *qs = s_qs[wave];
@@ -1143,24 +1147,30 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void s
             continue;
         const SparseProf *fg = c.f;
         switch (fg->np) {
-        case 1: filter_task<D, 1>(c, sp, phi, qb, qs, cw, n_windows, n_words, n_done); break;
-        case 2: filter_task<D, 2>(c, sp, phi, qb, qs, cw, n_windows, n_words, n_done); break;
-        case 3: filter_task<D, 3>(c, sp, phi, qb, qs, cw, n_windows, n_words, n_done); break;
-        default: filter_task<D, 4>(c, sp, phi, qb, qs, cw, n_windows, n_words, n_done); break;
+        case 1: filter_task<D, 1>(c, sp, phi, qb, qs, cw, n_windows, n_words, n_done,
+                                  n_loads); break;
+        case 2: filter_task<D, 2>(c, sp, phi, qb, qs, cw, n_windows, n_words, n_done,
+                                  n_loads); break;
+        case 3: filter_task<D, 3>(c, sp, phi, qb, qs, cw, n_windows, n_words, n_done,
+                                  n_loads); break;
+        default: filter_task<D, 4>(c, sp, phi, qb, qs, cw, n_windows, n_words, n_done,
+                                  n_loads); break;
         }
         if ((n_windows | n_done) >= 0x80000000u) {
             atomicAdd(&s_stats[0], (unsigned long long)n_windows);
             atomicAdd(&s_stats[1], (unsigned long long)n_words);
             atomicAdd(&s_stats[2], (unsigned long long)n_done);
-            n_windows = n_words = n_done = 0;
+            if (lane == 0) atomicAdd(&s_stats[3], (unsigned long long)n_loads);
+            n_windows = n_words = n_done = n_loads = 0;
         }
     }
     }
     atomicAdd(&s_stats[0], (unsigned long long)n_windows);
     atomicAdd(&s_stats[1], (unsigned long long)n_words);
     atomicAdd(&s_stats[2], (unsigned long long)n_done);
+    if (lane == 0) atomicAdd(&s_stats[3], (unsigned long long)n_loads);
     __syncthreads();
-    if (threadIdx.x < 3) atomicAdd(&stats[threadIdx.x], s_stats[threadIdx.x]);
+    if (threadIdx.x < 4) atomicAdd(&stats[threadIdx.x], s_stats[threadIdx.x]);
 }
 
 // One thread per prime: the output contract from the reduced keys.

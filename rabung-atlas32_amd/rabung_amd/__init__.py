@@ -58,7 +58,8 @@ class Timing(ctypes.Structure):
                 ("products", ctypes.c_uint64), ("launches", ctypes.c_uint64),
                 ("filter_windows", ctypes.c_uint64), ("filter_words", ctypes.c_uint64),
                 ("filter_completions", ctypes.c_uint64), ("a4_host_ms", ctypes.c_double),
-                ("a4_wait_ms", ctypes.c_double), ("rescans", ctypes.c_uint64)]
+                ("a4_wait_ms", ctypes.c_double), ("rescans", ctypes.c_uint64),
+                ("filter_loads", ctypes.c_uint64)]
 
 
 class Config(ctypes.Structure):
