@@ -110,6 +110,15 @@ typedef struct rbg_config {
                                 would pass 2^20 - 2, D grows back toward the n = 2 value */
     uint32_t filter_chunk;   /* sparse filter: tasks one wave claims from its XCD's
                                 queue at a time, 1..256; 0 = 16 */
+    uint32_t spec_off;       /* sparse filter: nonzero starts every profile from its
+                                seed threshold only; 0 also applies the speculative
+                                threshold (the largest L with at least 14 runs of
+                                length >= L expected over the half interval of a random
+                                r-colouring), and a profile whose filter finds no run
+                                reaching it is recomputed by the dense scan */
+    uint32_t spec_boost;     /* added to every speculative threshold; 0 for normal use.
+                                A large value makes every profile miss and forces the
+                                dense recomputation */
 } rbg_config;
 
 typedef struct rbg_ctx rbg_ctx;
@@ -237,9 +246,11 @@ typedef struct rbg_timing {
     uint64_t rescans;        /* dense scans repeated with every run tracked (a longest
                                 run below its floor; expected never above small primes) */
     uint64_t filter_loads;   /* wave-level 4-dword load instructions of the filter's word
-                                loop (stage and base windows; each moves 64 x 16 bytes
-                                through a CU's vector-memory path whatever the active
-                                lanes) */
+                                loop for the primes below 2^32 (stage and base windows;
+                                each moves 64 x 16 bytes through a CU's vector-memory
+                                path whatever the active lanes); 0 for wider primes */
+    uint64_t spec_misses;    /* profiles whose longest run stayed below their speculative
+                                threshold (their primes are among the fallbacks) */
 } rbg_timing;
 
 int rbg_last_timing(const rbg_ctx *ctx, rbg_timing *out);

@@ -481,6 +481,19 @@ u32 cover_div(const rbg_ctx *c) { return c->cfg.cover_div ? c->cfg.cover_div : 3
 
 u32 filter_chunk(const rbg_ctx *c) { return c->cfg.filter_chunk ? c->cfg.filter_chunk : 16u; }
 
+// Speculative threshold of order r for p: the largest L whose expected
+// number of runs of length >= L over the half interval of a random
+// r-colouring is at least 14, (p-1)/2 (1 - 1/r) r^-(L-1) >= 14 (so a
+// longest run below L has probability about e^-14; 0 of the 35,676
+// sparse-domain profiles of the committed vectors falls below it).  0 when
+// no L >= 2 qualifies.
+u32 spec_threshold(u64 p, u32 r) {
+    double v = (double)((p - 1) / 2) * (1.0 - 1.0 / r) / 14.0;
+    u32 L = 1;
+    for (; v >= r; v /= r) ++L;
+    return L >= 2 ? L : 0u;
+}
+
 u32 extend_limit(const rbg_ctx *c) {
     return c->cfg.extend_limit ? c->cfg.extend_limit : kExtendLimit;
 }
@@ -890,6 +903,10 @@ int sparse_plan(rbg_ctx *c, const u64 *primes, const std::vector<u32> &idx_in,
                 f.plane_off = (u32)words;  // units of 16 words
                 f.sigma0 = (((p - 1) / 2) % r) == 0;
                 f.pblk = (u32)pblk;
+                if (!c->cfg.spec_off) {
+                    const u32 q0 = spec_threshold(p, r);
+                    f.qspec = q0 ? std::min(s.Q, q0 + c->cfg.spec_boost) : 0u;
+                }
                 words += rbg::row_offset(rbg::kRows, B, f.np) / 16u;
                 for (u32 row = 0; row < rbg::kRows; ++row) pblk += rbg::row_blocks(row, B);
                 c->h_prof.push_back(f);
@@ -1052,9 +1069,21 @@ int sparse_run(rbg_ctx *c, rbg_record *recs, u32 *err, std::vector<u32> &failed_
     c->timing.launches += 6;
     std::vector<u32> h_failed(np, 0);
     u64 h_stats[4] = {0, 0, 0, 0};
+    std::vector<u64> h_best(nf, 0);
     HIPCK(c, hipMemcpyAsync(h_failed.data(), failed, np * 4, hipMemcpyDeviceToHost, s));
     HIPCK(c, hipMemcpyAsync(h_stats, c->sp_stats.ptr, sizeof h_stats, hipMemcpyDeviceToHost, s));
+    HIPCK(c, hipMemcpyAsync(h_best.data(), best, nf * 8, hipMemcpyDeviceToHost, s));
     HIPCK(c, hipStreamSynchronize(s));
+    // a profile whose longest run found stays below its speculative
+    // threshold may hold longer runs below it that the filter skipped: its
+    // prime is recomputed like a refused one
+    for (u32 k = 0; k < nf; ++k) {
+        const SparseProf &f = c->h_prof[k];
+        if (f.qspec && (u32)(h_best[k] >> 32) < f.qspec) {
+            h_failed[f.prime] |= 8u;
+            ++c->timing.spec_misses;
+        }
+    }
     const auto wait0 = std::chrono::steady_clock::now();
     a4_thread.join();
     c->timing.a4_wait_ms +=
@@ -1254,6 +1283,7 @@ int run_batch(rbg_ctx *c, const u64 *primes, u32 n, u32 flags, rbg_record *out, 
         sum.filter_words += t.filter_words;
         sum.filter_completions += t.filter_completions;
         sum.filter_loads += t.filter_loads;
+        sum.spec_misses += t.spec_misses;
         sum.a4_host_ms += t.a4_host_ms;
         sum.a4_wait_ms += t.a4_wait_ms;
         sum.rescans += t.rescans;

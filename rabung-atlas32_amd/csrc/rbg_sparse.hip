@@ -872,14 +872,16 @@ __device__ void filter_task(const FilterCtx<D> &c, u32 sp, u32 phi, u32 *qb, u64
                     }
                 }
                 next += (u32)__builtin_popcountll(idle);
-                n_loads += (u32)NP;  // the new words' base windows
+                // the new words' base windows (counted for the 32-bit filter
+                // only: the wide one has no registers to spare)
+                if constexpr (!D::wide) n_loads += (u32)NP;
             }
             const u64 busy = ballot64(have);
             if (!busy) {
                 if (stall || next >= total) break;
                 continue;
             }
-            n_loads += (u32)(G * NP);  // the step's window loads, one per stage and plane
+            if constexpr (!D::wide) n_loads += (u32)(G * NP);  // one per stage and plane
             bool fin = false;
             if (have) {
                 // every load of the step is issued before the first use
@@ -1005,7 +1007,9 @@ __device__ __forceinline__ bool task_setup(const SparsePrime *P, const SparsePro
     const SparsePrime *sg = P + fg->prime;
     if (dd > sg->D) return false;
     const u64 key = __hip_atomic_load(best_slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const u32 q = min((u32)(key >> 32), sg->Q);
+    // the running best, or the speculative threshold when it is higher (a
+    // profile whose filter finds no run reaching it is recomputed)
+    const u32 q = min(max((u32)(key >> 32), fg->qspec), sg->Q);
     if (q < 3u) return false;
     u32 t = max(3u, (q + 1u) / 2u);
     if (t > q) t = q;

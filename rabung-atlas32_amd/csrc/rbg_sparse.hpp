@@ -32,6 +32,7 @@ struct SparseProf {
     u32 prefix;     // initial run of the identity colour (seed kernel)
     u32 sigma0;     // r | (p-1)/2
     u32 pblk;       // first plane block
+    u32 qspec;      // speculative threshold (0: none), at most Q
 };
 
 // Colour rows of one profile.  Row 0 is the full signed table: bit i of

@@ -59,7 +59,7 @@ class Timing(ctypes.Structure):
                 ("filter_windows", ctypes.c_uint64), ("filter_words", ctypes.c_uint64),
                 ("filter_completions", ctypes.c_uint64), ("a4_host_ms", ctypes.c_double),
                 ("a4_wait_ms", ctypes.c_double), ("rescans", ctypes.c_uint64),
-                ("filter_loads", ctypes.c_uint64)]
+                ("filter_loads", ctypes.c_uint64), ("spec_misses", ctypes.c_uint64)]
 
 
 class Config(ctypes.Structure):
@@ -67,7 +67,8 @@ class Config(ctypes.Structure):
                 ("blocks_per_cu", ctypes.c_uint32), ("hbm_budget", ctypes.c_uint64),
                 ("xcd_group", ctypes.c_uint32), ("cover_q", ctypes.c_uint32),
                 ("extend_limit", ctypes.c_uint32), ("floor_boost", ctypes.c_uint32),
-                ("cover_div", ctypes.c_uint32), ("filter_chunk", ctypes.c_uint32)]
+                ("cover_div", ctypes.c_uint32), ("filter_chunk", ctypes.c_uint32),
+                ("spec_off", ctypes.c_uint32), ("spec_boost", ctypes.c_uint32)]
 
 
 EXPORTS = ["rbg_open", "rbg_profile_block", "rbg_profile_block_device",
@@ -260,10 +261,11 @@ class Engine:
     """One library context (device, stream, workspaces)."""
 
     def __init__(self, device=0, tile_chunks=0, blocks_per_cu=0, hbm_budget=0, xcd_group=0,
-                 cover_q=0, extend_limit=0, floor_boost=0, cover_div=0, filter_chunk=0):
+                 cover_q=0, extend_limit=0, floor_boost=0, cover_div=0, filter_chunk=0,
+                 spec_off=0, spec_boost=0):
         lib = load()
         cfg = Config(device, tile_chunks, blocks_per_cu, hbm_budget, xcd_group, cover_q,
-                     extend_limit, floor_boost, cover_div, filter_chunk)
+                     extend_limit, floor_boost, cover_div, filter_chunk, spec_off, spec_boost)
         self._ctx = lib.rbg_open(ctypes.byref(cfg))
         if not self._ctx:
             raise RabungError(2, "rbg_open failed (no HIP device %d?)" % device)

@@ -195,6 +195,32 @@ def test_forced_refusal_falls_back_to_dense():
         assert got[j] == rows[i] and forced[j]["A4"] == a4[i]
 
 
+def test_speculative_threshold_misses_fall_back():
+    """A speculative threshold above every profile's longest run (spec_boost
+    40 lifts it to the cap Q) makes every profile miss: each prime is
+    recomputed by the dense scan, and the records equal the default
+    engine's, the golden values and the records without speculation."""
+    rows, a4, _ = golden_data.sample_block(49900)
+    pick = sorted(random.Random(14).sample(range(len(rows)), 6))
+    primes = [rows[i]["p"] for i in pick] + oracle.next_primes(1 << 24, 6) + \
+        oracle.next_primes(70001, 4)
+    with rabung_amd.Engine() as eng:
+        ref = eng.profile_block(primes)
+        t = eng.last_timing()
+        assert t["spec_misses"] == 0 and t["fallbacks"] == 0
+    for j, i in enumerate(pick):
+        assert as_rows([ref[j]])[0] == rows[i] and ref[j]["A4"] == a4[i]
+    with rabung_amd.Engine(spec_boost=40) as eng:
+        got = eng.profile_block(primes)
+        t = eng.last_timing()
+    assert got == ref
+    assert t["spec_misses"] == t["sparse_profiles"] > 0
+    assert t["fallbacks"] == t["sparse_primes"] == len(primes)
+    with rabung_amd.Engine(spec_off=1) as eng:
+        assert eng.profile_block(primes) == ref
+        assert eng.last_timing()["spec_misses"] == 0
+
+
 @pytest.mark.parametrize("q", [12, 24, 63])
 def test_cover_threshold_does_not_change_records(q):
     """Any certified cover gives the same exact profiles: threshold caps Q of
