@@ -371,6 +371,19 @@ int rbg_encode_a4_frame(const rbg_record *recs, uint32_t n, uint64_t lo, uint64_
  * DATA_FORMAT.md): 32-byte SHA-256 digests. */
 int rbg_science_hash(const rbg_record *recs, uint32_t n, uint8_t out[32]);
 int rbg_prime_hash(const rbg_record *recs, uint32_t n, uint8_t out[32]);
+/* SHA-256 of the block's A4 values as u32 LE (the values hash of an A4 frame,
+ * docs/DATA_FORMAT.md:43-47). */
+int rbg_values_hash(const rbg_record *recs, uint32_t n, uint8_t out[32]);
+
+/* One step of the atlas hash chains (docs/DATA_FORMAT.md:63-69, walked by
+ * scripts/verify_atlas.py:79-81): scan = SHA256(prev || prime_hash ||
+ * profile_hash || u64 lo || u64 hi), A4 the same with values_hash before
+ * the interval.  out may alias prev.  Chains start from 32 zero bytes. */
+int rbg_chain_scan(const uint8_t prev[32], const uint8_t prime_hash[32],
+                   const uint8_t profile_hash[32], uint64_t lo, uint64_t hi, uint8_t out[32]);
+int rbg_chain_a4(const uint8_t prev[32], const uint8_t prime_hash[32],
+                 const uint8_t profile_hash[32], const uint8_t values_hash[32], uint64_t lo,
+                 uint64_t hi, uint8_t out[32]);
 
 /* ---- reference bridge (baseline_bridge.cu:7,21-27) ---------------------- */
 

@@ -226,3 +226,43 @@ extern "C" int rbg_prime_hash(const rbg_record *recs, uint32_t n, uint8_t out[32
     if ((n && !recs) || !out) return RBG_EINVAL;
     return prime_digest(recs, n, out) ? RBG_OK : RBG_EINVAL;
 }
+
+extern "C" int rbg_values_hash(const rbg_record *recs, uint32_t n, uint8_t out[32]) {
+    if ((n && !recs) || !out) return RBG_EINVAL;
+    std::vector<u8> raw;
+    raw.reserve(4ull * n);
+    for (u32 i = 0; i < n; ++i) put_le(raw, recs[i].A4, 4);
+    Sha256 h;
+    h.update(raw.data(), raw.size());
+    return h.final(out) ? RBG_OK : RBG_EINVAL;
+}
+
+namespace {
+// SHA-256 over the given 32-byte digests followed by u64 lo, u64 hi (LE).
+int chain_step(const u8 *const *parts, int nparts, u64 lo, u64 hi, u8 out[32]) {
+    std::vector<u8> msg;
+    for (int i = 0; i < nparts; ++i) {
+        if (!parts[i]) return RBG_EINVAL;
+        msg.insert(msg.end(), parts[i], parts[i] + 32);
+    }
+    put_le(msg, lo, 8);
+    put_le(msg, hi, 8);
+    Sha256 h;
+    h.update(msg.data(), msg.size());
+    return out && h.final(out) ? RBG_OK : RBG_EINVAL;
+}
+}  // namespace
+
+extern "C" int rbg_chain_scan(const uint8_t prev[32], const uint8_t prime_hash[32],
+                              const uint8_t profile_hash[32], uint64_t lo, uint64_t hi,
+                              uint8_t out[32]) {
+    const u8 *parts[3] = {prev, prime_hash, profile_hash};
+    return chain_step(parts, 3, lo, hi, out);
+}
+
+extern "C" int rbg_chain_a4(const uint8_t prev[32], const uint8_t prime_hash[32],
+                            const uint8_t profile_hash[32], const uint8_t values_hash[32],
+                            uint64_t lo, uint64_t hi, uint8_t out[32]) {
+    const u8 *parts[4] = {prev, prime_hash, profile_hash, values_hash};
+    return chain_step(parts, 4, lo, hi, out);
+}

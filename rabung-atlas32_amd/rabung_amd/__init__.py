@@ -76,7 +76,8 @@ EXPORTS = ["rbg_open", "rbg_profile_block", "rbg_profile_block_device",
            "rbg_estimate_bytes", "rbg_autotune", "rbg_last_error",
            "rbg_last_timing", "rbg_device_info", "rbg_close", "rbg_mask", "rbg_is_prime",
            "rbg_version", "rbg_a4_host", "rbg_primes_from", "rbg_encode_scan_frame",
-           "rbg_encode_a4_frame", "rbg_science_hash", "rbg_prime_hash", "rbg_a4_certificate", "rbg_microbench",
+           "rbg_encode_a4_frame", "rbg_science_hash", "rbg_prime_hash", "rbg_values_hash",
+           "rbg_chain_scan", "rbg_chain_a4", "rbg_a4_certificate", "rbg_microbench",
            "rbg_profile_block_wide", "rbg_profile_batch_wide", "rbg_estimate_bytes_wide",
            "rbg_a4_host_wide", "rbg_is_prime_wide", "rbg_mask_wide", "rbg_primes_from_wide",
            "rbg_check_wide", "rbg_certify_records", "rbg_prime_table", "rbg_prime_rank",
@@ -155,9 +156,15 @@ def load():
         fn.argtypes += [ctypes.c_int] if name == "rbg_encode_scan_frame" else []
         fn.argtypes += [ctypes.c_int, ctypes.c_void_p, P(ctypes.c_size_t)]
         fn.restype = ctypes.c_int
-    for name in ("rbg_science_hash", "rbg_prime_hash"):
+    for name in ("rbg_science_hash", "rbg_prime_hash", "rbg_values_hash"):
         getattr(lib, name).argtypes = [P(Record), ctypes.c_uint32, ctypes.c_char_p]
         getattr(lib, name).restype = ctypes.c_int
+    lib.rbg_chain_scan.argtypes = [ctypes.c_char_p] * 3 + [ctypes.c_uint64] * 2 + \
+        [ctypes.c_char_p]
+    lib.rbg_chain_scan.restype = ctypes.c_int
+    lib.rbg_chain_a4.argtypes = [ctypes.c_char_p] * 4 + [ctypes.c_uint64] * 2 + \
+        [ctypes.c_char_p]
+    lib.rbg_chain_a4.restype = ctypes.c_int
     lib.rbg_microbench.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_uint32,
                                    P(ctypes.c_double)]
     lib.rbg_microbench.restype = ctypes.c_int
@@ -254,6 +261,42 @@ def science_hash(records):
     rc = load().rbg_science_hash(_records_array(records), len(records), out)
     if rc:
         raise RabungError(rc, "science hash")
+    return out.raw
+
+
+def prime_hash(records):
+    out = ctypes.create_string_buffer(32)
+    rc = load().rbg_prime_hash(_records_array(records), len(records), out)
+    if rc:
+        raise RabungError(rc, "prime hash")
+    return out.raw
+
+
+def values_hash(records):
+    out = ctypes.create_string_buffer(32)
+    rc = load().rbg_values_hash(_records_array(records), len(records), out)
+    if rc:
+        raise RabungError(rc, "values hash")
+    return out.raw
+
+
+def chain_scan(previous, prime_digest, profile_digest, lo, hi):
+    """One step of the scan chain through the C ABI (rbg_chain_scan)."""
+    out = ctypes.create_string_buffer(32)
+    rc = load().rbg_chain_scan(bytes(previous), bytes(prime_digest), bytes(profile_digest),
+                               lo, hi, out)
+    if rc:
+        raise RabungError(rc, "scan chain")
+    return out.raw
+
+
+def chain_a4(previous, prime_digest, profile_digest, values_digest, lo, hi):
+    """One step of the A4 chain through the C ABI (rbg_chain_a4)."""
+    out = ctypes.create_string_buffer(32)
+    rc = load().rbg_chain_a4(bytes(previous), bytes(prime_digest), bytes(profile_digest),
+                             bytes(values_digest), lo, hi, out)
+    if rc:
+        raise RabungError(rc, "A4 chain")
     return out.raw
 
 

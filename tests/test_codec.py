@@ -5,6 +5,7 @@ truncated, extended or bit-flipped frames raise) and the sample checks of
 scripts/verify_atlas.py:18-45 (prime/science/values hashes, A4 range).
 """
 import hashlib
+import struct
 import re
 
 import pytest
@@ -105,6 +106,38 @@ def test_library_encoder_byte_identical(block):
     assert rabung_amd.science_hash(recs).hex() == entry["science_hash"]
     nowit = rabung_amd.encode_scan_frame(recs, entry["lo"], entry["hi"], False, 6)
     assert nowit == codec.encode_scan_frame(rows, entry["lo"], entry["hi"], False, 6)
+
+
+def test_library_chains_and_values_hash():
+    """The C-ABI commitments of both sample blocks equal the manifest's prime,
+    science and values hashes, and the C-ABI chain steps (rbg_chain_scan,
+    rbg_chain_a4) equal the Python codec's over block 0 then block 49900
+    and in the other order (which gives different chains)."""
+    import rabung_amd
+    z = bytes(32)
+    steps = []
+    for block in (0, 49900):
+        rows, values, entry = golden_data.sample_block(block)
+        recs = _records(rows, values)
+        ph, sh = rabung_amd.prime_hash(recs), rabung_amd.science_hash(recs)
+        vh = rabung_amd.values_hash(recs)
+        assert (ph.hex(), sh.hex(), vh.hex()) == (entry["prime_hash"], entry["science_hash"],
+                                                  entry["values_hash"])
+        assert vh == hashlib.sha256(struct.pack("<%dI" % len(values), *values)).digest()
+        steps.append((ph, sh, vh, entry["lo"], entry["hi"]))
+    for order in (steps, steps[::-1]):
+        sc_c = sc_py = ac_c = ac_py = z
+        for ph, sh, vh, lo, hi in order:
+            sc_c = rabung_amd.chain_scan(sc_c, ph, sh, lo, hi)
+            sc_py = codec.chain_scan(sc_py, ph, sh, lo, hi)
+            ac_c = rabung_amd.chain_a4(ac_c, ph, sh, vh, lo, hi)
+            ac_py = codec.chain_a4(ac_py, ph, sh, vh, lo, hi)
+        assert sc_c == sc_py and ac_c == ac_py and sc_c != ac_c
+    forward = rabung_amd.chain_scan(rabung_amd.chain_scan(z, *steps[0][:2], *steps[0][3:]),
+                                    *steps[1][:2], *steps[1][3:])
+    backward = rabung_amd.chain_scan(rabung_amd.chain_scan(z, *steps[1][:2], *steps[1][3:]),
+                                     *steps[0][:2], *steps[0][3:])
+    assert forward != backward
 
 
 def test_library_encoder_refuses_bad_records():
