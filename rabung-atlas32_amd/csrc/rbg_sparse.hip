@@ -120,6 +120,16 @@ __global__ __launch_bounds__(256) void sp_labels(const SparsePrime *__restrict__
         labels_of<Dom32>(d, tab_val, tab_lab, labels, err, (u32 *)s_val, s_lab);
 }
 
+// x mod r for x < 2^20 and 2 <= r < 2^12 (labels, sums of a few labels,
+// colours) by one high multiply: with g = floor((2^32-1)/r) + 1 the
+// quotient floor(x g / 2^32) is exact because x g / 2^32 exceeds x / r by
+// less than x / 2^32 < 1 / r.
+struct SmallMod {
+    u32 r, g;
+    __device__ __forceinline__ explicit SmallMod(u32 r_) : r(r_), g(0xFFFFFFFFu / r_ + 1u) {}
+    __device__ __forceinline__ u32 operator()(u32 x) const { return x - r * __umulhi(x, g); }
+};
+
 // Second pass: y = 2^a 3^b x with x coprime to 6 and y >= 4 not coprime to
 // 6 gets j(y) = (a j(2) + b j(3) + j(x)) mod m (j(1) = 0).
 __global__ __launch_bounds__(256) void sp_labels_rest(const SparsePrime *__restrict__ P,
@@ -128,6 +138,7 @@ __global__ __launch_bounds__(256) void sp_labels_rest(const SparsePrime *__restr
     const SparsePrime d = P[pi];
     const u16 *lab = labels + d.lab_off;
     const u32 j2 = lab[2], j3 = lab[3];
+    const SmallMod mod_m(d.m);
     const u32 y0 = 1u + (blockIdx.x - d.lblk) * (256u * kLabelsPerThread);
     for (u32 i = 0; i < kLabelsPerThread; ++i) {
         const u32 y = y0 + i * 256u + threadIdx.x;
@@ -139,7 +150,8 @@ __global__ __launch_bounds__(256) void sp_labels_rest(const SparsePrime *__restr
             x /= 3u;
             ++b;
         }
-        labels[(u64)d.lab_off + y] = (u16)((a * j2 + b * j3 + lab[x]) % d.m);
+        // a <= 20, b <= 12: the sum stays below 33 m < 2^17
+        labels[(u64)d.lab_off + y] = (u16)mod_m(a * j2 + b * j3 + lab[x]);
     }
 }
 
@@ -163,23 +175,62 @@ __global__ __launch_bounds__(256) void sp_inverses(const SparsePrime *__restrict
 
 constexpr u32 kPlanePairsPerWave = kPlanePairsPerBlock / 4;  // 64: one per lane
 
-// x mod r for x < 2^16 and 2 <= r < 2^12 (labels, label + label of -1, and
-// their colours) by one high multiply: with g = floor((2^32-1)/r) + 1 the
-// quotient floor(x g / 2^32) is exact because x g / 2^32 exceeds x / r by
-// less than x / 2^32 < 1 / r.
-struct SmallMod {
-    u32 r, g;
-    __device__ __forceinline__ explicit SmallMod(u32 r_) : r(r_), g(0xFFFFFFFFu / r_ + 1u) {}
-    __device__ __forceinline__ u32 operator()(u32 x) const { return x - r * __umulhi(x, g); }
-};
-
 // Full colour row (row 0) of one order: bit k of plane b is bit b of the
 // colour of the numerator y = k - B, cl(-y) = cl(y) + cl(-1), and 0 for
 // numerators outside [-B, B] and for y = 0.  One wave builds 64 bits (two
-// words) of every plane per step; lane q keeps the words of step q, and
-// after 64 steps the lanes store them together (one 512-byte store per plane
-// word).  The kernel is written for any row (g, rho) but launched blocks of
-// the decimated rows return at once: sp_rows copies their bits out of row 0.
+// words) of each of the NP planes per step; lane q keeps the words of step
+// q, and after 64 steps the lanes store them together (one 512-byte store
+// per plane word).  |y| <= B + 64 < 2^21, so positions are 32-bit.
+template <int NP>
+__device__ __forceinline__ void row0_planes(const SparsePrime &s, const SparseProf &f, u32 first,
+                                            u32 nq, u32 pairs, u32 wpr, const u16 *lab,
+                                            u32 *base) {
+    const u32 lane = threadIdx.x & 63u;
+    u64 mine[NP];  // this lane's step: words of planes 0..NP-1
+#pragma unroll
+    for (int b = 0; b < NP; ++b) mine[b] = 0;
+    const SmallMod mod_r(f.r), mod_m(s.m);
+    const int B = (int)s.B;
+    // eight steps at a time: the eight label loads first (every lane loads an
+    // in-range label, position 1 for the positions outside the table), then
+    // the colours and ballots of the steps that exist
+    for (u32 q0 = 0; q0 < nq; q0 += 8u) {
+        u32 jv[8];
+        int yv[8];
+#pragma unroll
+        for (u32 u = 0; u < 8u; ++u) {
+            yv[u] = (int)((first + q0 + u) * 64u + lane) - B;
+            const int a = yv[u] < 0 ? -yv[u] : yv[u];
+            jv[u] = lab[(yv[u] != 0 && a <= B) ? a : 1];
+        }
+#pragma unroll
+        for (u32 u = 0; u < 8u; ++u) {
+            if (q0 + u >= nq) break;
+            const int y = yv[u];
+            const int a = y < 0 ? -y : y;
+            const bool in = y != 0 && a <= B;
+            const u32 c = !in ? 0u : y > 0 ? mod_r(jv[u]) : mod_r(mod_m(jv[u] + s.jneg));
+#pragma unroll
+            for (int b = 0; b < NP; ++b) {
+                const u64 bits = ballot64((c >> b) & 1u);
+                if (lane == q0 + u) mine[b] = bits;
+            }
+        }
+    }
+    const u32 pair = first + lane;
+    if (pair < pairs) {
+#pragma unroll
+        for (int b = 0; b < NP; ++b) {
+            u32 *pl = base + (u64)b * wpr;
+            pl[2 * pair] = (u32)mine[b];
+            pl[2 * pair + 1] = (u32)(mine[b] >> 32);
+        }
+    }
+}
+
+// Row 0 of every profile.  The kernel's grid covers every row's blocks (the
+// plane-block map of the profile records), but the blocks of the decimated
+// rows return at once: sp_rows copies their bits out of row 0.
 __global__ __launch_bounds__(256) void sp_planes(const SparsePrime *__restrict__ P, u32 nprimes,
                                                  const SparseProf *__restrict__ prof, u32 nprof,
                                                  const u16 *__restrict__ labels,
@@ -192,54 +243,20 @@ __global__ __launch_bounds__(256) void sp_planes(const SparsePrime *__restrict__
     u32 k = s.prof_begin;
     while (k + 1 < s.prof_begin + s.nprof && prof[k + 1].pblk <= blockIdx.x) ++k;
     const SparseProf f = prof[k];
-    u32 local = blockIdx.x - f.pblk, row = 0;
-    while (row + 1 < kRows && local >= row_blocks(row, s.B)) local -= row_blocks(row++, s.B);
-    row = __builtin_amdgcn_readfirstlane(row);
-    if (row != 0) return;  // the decimated rows come from row 0 (sp_rows)
-    const u32 g = row_g(row), wpr = row_words(g, s.B);
-    const int64_t rho = row_rho(row, s.B);
-    const u32 lane = threadIdx.x & 63u;
+    const u32 local = blockIdx.x - f.pblk;
+    if (local >= row_blocks(0, s.B)) return;  // a block of a decimated row
+    const u32 wpr = row_words(1, s.B);
     const u32 wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const u32 pairs = wpr / 2;
     const u32 first = (local * 4u + wave) * kPlanePairsPerWave;
-    u64 mine[4] = {0, 0, 0, 0};  // this lane's step: words of planes 0..np-1
-    const SmallMod mod_r(f.r), mod_m(s.m);
     const u32 nq = first < pairs ? min(kPlanePairsPerWave, pairs - first) : 0u;
     const u16 *lab = labels + s.lab_off;
-    // eight steps at a time: the eight label loads first (every lane loads an
-    // in-range label, position 1 for the positions outside the table), then
-    // the colours and ballots of the steps that exist
-    for (u32 q0 = 0; q0 < nq; q0 += 8u) {
-        u32 jv[8];
-        int64_t yv[8];
-#pragma unroll
-        for (u32 u = 0; u < 8u; ++u) {
-            yv[u] = rho + (int64_t)g * ((int64_t)(first + q0 + u) * 64 + lane) - (int64_t)s.B;
-            const u32 a = (u32)(yv[u] < 0 ? -yv[u] : yv[u]);
-            jv[u] = lab[(yv[u] != 0 && a <= s.B) ? a : 1u];
-        }
-#pragma unroll
-        for (u32 u = 0; u < 8u; ++u) {
-            if (q0 + u >= nq) break;
-            const int64_t y = yv[u];
-            const u32 a = (u32)(y < 0 ? -y : y);
-            const bool in = y != 0 && a <= s.B;
-            const u32 c = !in ? 0u : y > 0 ? mod_r(jv[u]) : mod_r(mod_m(jv[u] + s.jneg));
-#pragma unroll
-            for (u32 b = 0; b < 4; ++b) {
-                const u64 bits = ballot64((c >> b) & 1u);
-                if (lane == q0 + u) mine[b] = bits;
-            }
-        }
-    }
-    const u32 pair = first + lane;
-    if (pair < pairs) {
-        u32 *base = planes + (u64)f.plane_off * 16u + row_offset(row, s.B, f.np);
-        for (u32 b = 0; b < f.np; ++b) {
-            u32 *pl = base + (u64)b * wpr;
-            pl[2 * pair] = (u32)mine[b];
-            pl[2 * pair + 1] = (u32)(mine[b] >> 32);
-        }
+    u32 *base = planes + (u64)f.plane_off * 16u;
+    switch (f.np) {
+    case 1: row0_planes<1>(s, f, first, nq, pairs, wpr, lab, base); break;
+    case 2: row0_planes<2>(s, f, first, nq, pairs, wpr, lab, base); break;
+    case 3: row0_planes<3>(s, f, first, nq, pairs, wpr, lab, base); break;
+    default: row0_planes<4>(s, f, first, nq, pairs, wpr, lab, base); break;
     }
 }
 
@@ -325,20 +342,33 @@ __global__ __launch_bounds__(256) void sp_seed_scan(const SparsePrime *__restric
     order_init(st);
     u32 carry = 0, fcol = 0, lcol = 0;
     const u32 nchunks = (len + 63u) >> 6;
-    for (u32 c = 0; c < nchunks; ++c) {
-        const u32 base = c << 6;
-        const u32 valid = min(64u, len - base);
-        const u64 vmask = valid == 64u ? ~0ull : ((1ull << valid) - 1ull);
-        const u32 y = first + base + lane;
-        const u32 col = y <= s.B ? mod_r(labels[(u64)s.lab_off + y]) : 0u;
-        const u32 left = shift_up_one(col, carry);
-        carry = __builtin_amdgcn_readlane(col, 63);
-        if (c == 0) fcol = __builtin_amdgcn_readfirstlane(col);
-        lcol = __builtin_amdgcn_readlane(col, valid - 1);
-        u64 C = ballot64(col != left);
-        if (c == 0) C |= 1ull;
-        C &= vmask;
-        order_update(st, C, ~C & vmask, base);
+    const u16 *lab = labels + s.lab_off;
+    for (u32 c0 = 0; c0 < nchunks; c0 += 4u) {
+        // the labels of four chunks in flight before the first is used
+        u32 jv[4];
+#pragma unroll
+        for (u32 i = 0; i < 4u; ++i) {
+            const u32 y = first + ((c0 + i) << 6) + lane;
+            jv[i] = lab[y <= s.B ? y : 0u];
+        }
+#pragma unroll
+        for (u32 i = 0; i < 4u; ++i) {
+            const u32 c = c0 + i;
+            if (c >= nchunks) break;
+            const u32 base = c << 6;
+            const u32 valid = min(64u, len - base);
+            const u64 vmask = valid == 64u ? ~0ull : ((1ull << valid) - 1ull);
+            const u32 y = first + base + lane;
+            const u32 col = y <= s.B ? mod_r(jv[i]) : 0u;
+            const u32 left = shift_up_one(col, carry);
+            carry = __builtin_amdgcn_readlane(col, 63);
+            if (c == 0) fcol = __builtin_amdgcn_readfirstlane(col);
+            lcol = __builtin_amdgcn_readlane(col, valid - 1);
+            u64 C = ballot64(col != left);
+            if (c == 0) C |= 1ull;
+            C &= vmask;
+            order_update(st, C, ~C & vmask, base);
+        }
     }
     const u32 suffix = order_finish(st, len);
     if (lane == 0) {
