@@ -38,12 +38,17 @@ using rbg::u16;
 using rbg::u32;
 using rbg::u64;
 
-extern "C" hipError_t rbg_launch_scan_reuse(const PrimeDesc *descs, uint32_t pi, uint32_t ntiles,
-                                            uint32_t chunks, const uint32_t *tab_val,
-                                            const uint16_t *tab_lab, uint16_t *lab,
-                                            uint32_t lab_from, uint32_t first_phase, uint32_t j2,
-                                            uint32_t j3, TileHdr *hdr, RunSum *runs, uint32_t *err,
-                                            uint32_t *launches, hipStream_t stream);
+extern "C" hipError_t rbg_launch_label_seed(const PrimeDesc *descs, const rbg::ReusePiece *pieces,
+                                            uint32_t npieces, uint32_t nblocks,
+                                            const uint32_t *tab_val, const uint16_t *tab_lab,
+                                            uint16_t *lab, uint32_t lab_from, uint32_t *err,
+                                            hipStream_t stream);
+extern "C" hipError_t rbg_launch_scan_reuse(const PrimeDesc *descs, const rbg::ReusePiece *pieces,
+                                            uint32_t npieces, uint32_t nblocks, uint32_t chunks,
+                                            const uint32_t *tab_val, const uint16_t *tab_lab,
+                                            uint16_t *lab, uint32_t lab_from, TileHdr *hdr,
+                                            RunSum *runs, uint32_t *err, hipStream_t stream);
+extern "C" uint32_t rbg_seed_per_block(void);
 extern "C" hipError_t rbg_launch_scan(const PrimeDesc *descs, uint32_t nprimes, uint32_t tile_first,
                                       uint32_t tile_end, uint32_t chunks, const void *tab_val,
                                       const uint16_t *tab_code, TileHdr *hdr, RunSum *runs,
@@ -371,8 +376,15 @@ struct rbg_ctx {
         u32 chunks, reuse, j2, j3;
     };
     std::vector<DenseLaunch> h_launch;
+    // batched label-reuse launches of the plain method: pieces (one per
+    // prime and launch) and the launches over them
+    struct ReuseLaunch {
+        u32 first, count, blocks, seed;  // pieces [first, first+count); seed launch or phase
+    };
+    std::vector<rbg::ReusePiece> h_pieces;
+    std::vector<ReuseLaunch> h_reuse;
     bool exact_runs = false;  // dense scans with run floor 1 (the rescan)
-    DevBuf tab_lab, dense_lab;
+    DevBuf tab_lab, dense_lab, dense_pieces;
     // sparse path
     DevBuf sp_primes, sp_profs, sp_val, sp_val64, sp_lab, sp_labels, sp_inv, sp_inv64, sp_planes,
         sp_best, sp_failed, sp_a4, sp_stats, sp_segs, sp_tpre;
@@ -399,7 +411,8 @@ struct rbg_ctx {
     // every device buffer of the context
     std::vector<DevBuf *> buffers() {
         return {&descs, &tab_val, &tab_val64, &tab_code, &tab_lab, &dense_lab, &hdr, &runs,
-                &recs, &flags, &sp_primes, &sp_profs, &sp_val, &sp_val64, &sp_lab, &sp_labels, &sp_inv,
+                &recs, &flags, &dense_pieces, &sp_primes, &sp_profs, &sp_val, &sp_val64, &sp_lab,
+                &sp_labels, &sp_inv,
                 &sp_inv64, &sp_planes, &sp_best, &sp_failed, &sp_a4, &sp_stats, &sp_segs, &sp_tpre,
                 &sel};
     }
@@ -709,12 +722,72 @@ int dense_run(rbg_ctx *c, const u64 *primes, const std::vector<u32> &idx_in, boo
         for (rbg_ctx::DenseLaunch &L : c->h_launch) L.reuse = 0;
         lab_bytes = 0;
     }
+    // the label-reuse primes run in groups whose label arrays fit side by
+    // side in the budget the other workspaces leave (and below 2^32
+    // entries): per group one seed launch and one launch per phase over all
+    // its primes, so that a phase fills the chip however few tiles each
+    // prime has in it.  Phase windows of tiles [0, 256), [256, 512),
+    // [512, 1024), ...: a position x of a phase reads x/2 and x/3, which lie
+    // in the seed or an earlier phase of its own prime.
+    c->h_pieces.clear();
+    c->h_reuse.clear();
+    const u32 reuse_pos = waves * reuse_chunks * 64u;  // positions per tile
+    const u32 lab_from = kReuseFirstPhase * reuse_pos / 2u;
+    u64 lab_entries_max = 0;
+    if (lab_bytes) {
+        const u64 cap = std::min<u64>((budget(c) - bytes) / 2, 0xFFFFFFFFull);
+        const u32 seed_blocks = (lab_from + rbg_seed_per_block() - 1) / rbg_seed_per_block();
+        std::vector<u32> group;
+        std::vector<u64> offs;
+        u64 entries = 0;
+        auto flush = [&]() {
+            if (group.empty()) return;
+            rbg_ctx::ReuseLaunch S{(u32)c->h_pieces.size(), 0, 0, 1};
+            for (size_t g = 0; g < group.size(); ++g) {
+                const rbg_ctx::DenseLaunch &L = c->h_launch[group[g]];
+                c->h_pieces.push_back({S.blocks, group[g], 0, L.j2, L.j3, 0, offs[g]});
+                S.blocks += seed_blocks;
+                ++S.count;
+            }
+            c->h_reuse.push_back(S);
+            for (u32 lo = 0, hi = kReuseFirstPhase;; lo = hi, hi *= 2u) {
+                rbg_ctx::ReuseLaunch P{(u32)c->h_pieces.size(), 0, 0, 0};
+                for (size_t g = 0; g < group.size(); ++g) {
+                    const u32 nt = c->h_descs[group[g]].ntiles;
+                    if (nt <= lo) continue;
+                    const rbg_ctx::DenseLaunch &L = c->h_launch[group[g]];
+                    c->h_pieces.push_back({P.blocks, group[g], lo, L.j2, L.j3, 0, offs[g]});
+                    P.blocks += std::min(hi, nt) - lo;
+                    ++P.count;
+                }
+                if (!P.count) break;
+                c->h_reuse.push_back(P);
+            }
+            lab_entries_max = std::max(lab_entries_max, entries);
+            group.clear();
+            offs.clear();
+            entries = 0;
+        };
+        for (u32 i = 0; i < np; ++i) {
+            if (!c->h_launch[i].reuse) continue;
+            const u64 e = (u64)c->h_descs[i].nh / 2 + 1;
+            if (!group.empty() && entries + e > cap) flush();
+            group.push_back(i);
+            offs.push_back(entries);
+            entries += e;
+        }
+        flush();
+        lab_bytes = lab_entries_max * 2;
+    }
     HIPCK(c, c->descs.ensure(np * sizeof(PrimeDesc)));
     HIPCK(c, c->tab_val.ensure(c->h_val.size() * 4));
     HIPCK(c, c->tab_val64.ensure(c->h_val64.size() * 8));
     HIPCK(c, c->tab_code.ensure(c->h_code.size() * 2));
     HIPCK(c, c->tab_lab.ensure(c->h_lab.size() * 2));
-    if (lab_bytes) HIPCK(c, c->dense_lab.ensure(lab_bytes));
+    if (lab_bytes) {
+        HIPCK(c, c->dense_lab.ensure(lab_bytes));
+        HIPCK(c, c->dense_pieces.ensure(c->h_pieces.size() * sizeof(rbg::ReusePiece)));
+    }
     HIPCK(c, c->hdr.ensure(tiles * sizeof(TileHdr)));
     HIPCK(c, c->runs.ensure(runs * sizeof(RunSum)));
     hipStream_t s = c->stream;
@@ -730,6 +803,10 @@ int dense_run(rbg_ctx *c, const u64 *primes, const std::vector<u32> &idx_in, boo
                             hipMemcpyHostToDevice, s));
     HIPCK(c, hipMemcpyAsync(c->tab_lab.ptr, c->h_lab.data(), c->h_lab.size() * 2,
                             hipMemcpyHostToDevice, s));
+    if (!c->h_pieces.empty())
+        HIPCK(c, hipMemcpyAsync(c->dense_pieces.ptr, c->h_pieces.data(),
+                                c->h_pieces.size() * sizeof(rbg::ReusePiece),
+                                hipMemcpyHostToDevice, s));
     // tile counters of the two persistent launches (flags[0], flags[2])
     u32 *counter = (u32 *)c->flags.ptr, *counter64 = (u32 *)c->flags.ptr + 2;
     const PrimeDesc *dd = (const PrimeDesc *)c->descs.ptr;
@@ -740,23 +817,27 @@ int dense_run(rbg_ctx *c, const u64 *primes, const std::vector<u32> &idx_in, boo
         HIPCK(c, rbg_fill_u32(counter, 1, 0u, s));
         HIPCK(c, rbg_fill_u32(counter64, 1, 0u, s));
         if (plain) {
+            const rbg::ReusePiece *pieces = (const rbg::ReusePiece *)c->dense_pieces.ptr;
+            for (const rbg_ctx::ReuseLaunch &L : c->h_reuse) {
+                if (L.seed)
+                    HIPCK(c, rbg_launch_label_seed(dd, pieces + L.first, L.count, L.blocks,
+                                                   (const u32 *)c->tab_val.ptr,
+                                                   (const u16 *)c->tab_lab.ptr,
+                                                   (u16 *)c->dense_lab.ptr, lab_from, err, s));
+                else
+                    HIPCK(c, rbg_launch_scan_reuse(dd, pieces + L.first, L.count, L.blocks,
+                                                   reuse_chunks, (const u32 *)c->tab_val.ptr,
+                                                   (const u16 *)c->tab_lab.ptr,
+                                                   (u16 *)c->dense_lab.ptr, lab_from,
+                                                   (TileHdr *)c->hdr.ptr, (RunSum *)c->runs.ptr,
+                                                   err, s));
+                ++c->timing.launches;
+            }
             for (u32 i = 0; i < np; ++i) {
                 const PrimeDesc &d = c->h_descs[i];
                 const rbg_ctx::DenseLaunch &L = c->h_launch[i];
                 const int wide = (int)(d.hi & 1u);
-                if (L.reuse) {
-                    const u32 lab_from = kReuseFirstPhase * waves * L.chunks * 64u / 2u;
-                    u32 launches = 0;
-                    HIPCK(c, rbg_launch_scan_reuse(dd, i, d.ntiles, L.chunks,
-                                                   (const u32 *)c->tab_val.ptr,
-                                                   (const u16 *)c->tab_lab.ptr,
-                                                   (u16 *)c->dense_lab.ptr, lab_from,
-                                                   kReuseFirstPhase, L.j2, L.j3,
-                                                   (TileHdr *)c->hdr.ptr, (RunSum *)c->runs.ptr,
-                                                   err, &launches, s));
-                    c->timing.launches += launches;
-                    continue;
-                }
+                if (L.reuse) continue;  // scanned by the batched phases above
                 HIPCK(c, rbg_launch_scan(dd, np, d.tile_begin, d.tile_begin + d.ntiles, L.chunks,
                                          wide ? c->tab_val64.ptr : c->tab_val.ptr,
                                          (const u16 *)c->tab_code.ptr, (TileHdr *)c->hdr.ptr,

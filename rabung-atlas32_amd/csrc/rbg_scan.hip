@@ -406,16 +406,33 @@ __global__ __launch_bounds__(kWavesPerBlock * 64) void scan_tiles(
     }
 }
 
-// Label seed of the reuse phases: lab[x] = j(x) for 1 <= x <= n by direct
-// powers (the positions below the first phase's reads).
-__global__ __launch_bounds__(256) void label_seed(const PrimeDesc *__restrict__ descs, u32 pi,
-                                                  const u32 *__restrict__ tab_val,
+// The piece of a batched launch that block b belongs to (pieces sorted by
+// block0, the first at 0).
+__device__ __forceinline__ u32 piece_of(const ReusePiece *pieces, u32 npieces, u32 b) {
+    u32 lo = 0, hi = npieces;
+    while (hi - lo > 1) {
+        const u32 mid = (lo + hi) >> 1;
+        if (pieces[mid].block0 <= b) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+constexpr u32 kSeedPerThread = 16;  // seed positions per thread
+
+// Label seed of the reuse phases, for several primes at once: lab[x] = j(x)
+// for 1 <= x <= n by direct powers (the positions below the first phase's
+// reads), kSeedPerThread positions per thread.
+__global__ __launch_bounds__(256) void label_seed(const PrimeDesc *__restrict__ descs,
+                                                  const ReusePiece *__restrict__ pieces,
+                                                  u32 npieces, const u32 *__restrict__ tab_val,
                                                   const u16 *__restrict__ tab_lab,
-                                                  u16 *__restrict__ lab, u32 n,
+                                                  u16 *__restrict__ lab_base, u32 n,
                                                   u32 *__restrict__ err) {
     __shared__ u32 s_val[kMaxM];
     __shared__ u16 s_lab[kMaxM];
-    const PrimeDesc d = descs[pi];
+    const ReusePiece pc = pieces[__builtin_amdgcn_readfirstlane(piece_of(pieces, npieces, blockIdx.x))];
+    const PrimeDesc d = descs[pc.pi];
+    u16 *lab = lab_base + pc.lab_off;
     for (u32 j = threadIdx.x; j < d.m; j += blockDim.x) {
         s_val[j] = tab_val[d.tab_off + j];
         s_lab[j] = tab_lab[d.tab_off + j];
@@ -423,7 +440,10 @@ __global__ __launch_bounds__(256) void label_seed(const PrimeDesc *__restrict__ 
     __syncthreads();
     const PrimeConsts<Dom32> k = consts_of(d, Dom32{});
     u32 bad = 0;
-    for (u32 x = 1u + blockIdx.x * blockDim.x + threadIdx.x; x <= n; x += gridDim.x * blockDim.x) {
+    const u32 x0 = 1u + (blockIdx.x - pc.block0) * (256u * kSeedPerThread) + threadIdx.x;
+    for (u32 i = 0; i < kSeedPerThread; ++i) {
+        const u32 x = x0 + i * 256u;
+        if (x > n) break;
         const u32 y = k.dom.pow(k.dom.mul(x, k.r2), d.e, k.one);
         const u32 lo = table_index(s_val, d.m, y);
         bad |= s_val[lo] != y;
@@ -432,28 +452,30 @@ __global__ __launch_bounds__(256) void label_seed(const PrimeDesc *__restrict__ 
     if (bad) atomicOr(err, 1u);
 }
 
-// One phase of the label-reuse scan of prime pi: tiles tile_lo + blockIdx.x
-// (prime-local tile numbers, one tile per workgroup).  lab holds the labels
-// of positions 1 .. lab_from (seed) and of every position of the earlier
-// phases up to lab_to = nh / 2; this phase adds its own up to lab_to.
+// One phase of the label-reuse scan for several primes at once: block b of
+// piece (prime pi) scans its prime's tile tile_lo + b - block0 (one tile per
+// workgroup).  The prime's labels hold positions 1 .. lab_from (seed) and
+// every position of the earlier phases up to lab_to = nh / 2; this phase
+// adds its own up to lab_to.
 __global__ __launch_bounds__(kWavesPerBlock * 64) void scan_reuse(
-    const PrimeDesc *__restrict__ descs, u32 pi, u32 tile_lo, u32 chunks,
-    const u32 *__restrict__ tab_val, const u16 *__restrict__ tab_lab, u16 *lab, u32 lab_from,
-    u32 j2, u32 j3, TileHdr *__restrict__ hdr, RunSum *__restrict__ runs, u32 *__restrict__ err) {
+    const PrimeDesc *__restrict__ descs, const ReusePiece *__restrict__ pieces, u32 npieces,
+    u32 chunks, const u32 *__restrict__ tab_val, const u16 *__restrict__ tab_lab, u16 *lab_base,
+    u32 lab_from, TileHdr *__restrict__ hdr, RunSum *__restrict__ runs, u32 *__restrict__ err) {
     __shared__ u32 s_val[kMaxM];
     __shared__ u16 s_lab[kMaxM];   // sorted root-table index -> label j
     __shared__ u16 s_code[kMaxM];  // label j -> colour code
     __shared__ u16 s_jp[kWavesPerBlock][kReuseChunks * 64];
     __shared__ TileWaves s_tw;
     const u32 wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const PrimeDesc d = descs[pi];
+    const ReusePiece pc = pieces[__builtin_amdgcn_readfirstlane(piece_of(pieces, npieces, blockIdx.x))];
+    const PrimeDesc d = descs[pc.pi];
     for (u32 j = threadIdx.x; j < d.m; j += blockDim.x) {
         s_val[j] = tab_val[d.tab_off + j];
         s_lab[j] = tab_lab[d.tab_off + j];
         s_code[j] = (u16)colour_code(j, d.m);
     }
     __syncthreads();
-    const u32 lt = tile_lo + blockIdx.x;
+    const u32 lt = pc.tile_lo + (blockIdx.x - pc.block0);
     const u32 seg_positions = chunks * 64u;
     const u64 seg = (u64)lt * kWavesPerBlock + wave;
     const u64 seg_start = 1ull + seg * seg_positions;
@@ -461,7 +483,8 @@ __global__ __launch_bounds__(kWavesPerBlock * 64) void scan_reuse(
         seg_start > d.nh ? 0u : (u32)min((u64)seg_positions, (u64)d.nh - seg_start + 1ull);
     u32 codes = 0;
     wave_scan_reuse(d, consts_of(d, Dom32{}), (u32)seg_start, seg_len, s_val, s_lab, s_code,
-                    s_jp[wave], lab, lab_from, d.nh / 2u, j2, j3, s_tw.run[wave], codes, err);
+                    s_jp[wave], lab_base + pc.lab_off, lab_from, d.nh / 2u, pc.j2, pc.j3,
+                    s_tw.run[wave], codes, err);
     write_tile(s_tw, wave, d, lt, d.tile_begin + lt, (u32)seg_start, seg_len, codes, hdr, runs);
 }
 
@@ -588,30 +611,35 @@ extern "C" hipError_t rbg_launch_scan(const rbg::PrimeDesc *descs, uint32_t npri
 // labels 1 .. lab_from, then phases of doubling tile ranges, each phase
 // reading only labels of positions below its first tile.  first_phase = the
 // tiles of the first phase (2 lab_from >= first_phase * tile positions).
-extern "C" hipError_t rbg_launch_scan_reuse(const rbg::PrimeDesc *descs, uint32_t pi,
-                                            uint32_t ntiles, uint32_t chunks,
-                                            const uint32_t *tab_val, const uint16_t *tab_lab,
-                                            uint16_t *lab, uint32_t lab_from,
-                                            uint32_t first_phase, uint32_t j2, uint32_t j3,
-                                            rbg::TileHdr *hdr, rbg::RunSum *runs, uint32_t *err,
-                                            uint32_t *launches, hipStream_t stream) {
-    if (chunks % rbg::kReuseChunks || !first_phase ||
-        2ull * lab_from < (uint64_t)first_phase * rbg::kWavesPerBlock * chunks * 64u)
-        return hipErrorInvalidValue;
-    const uint32_t seed_blocks = (lab_from + 255u) / 256u < 2048u ? (lab_from + 255u) / 256u : 2048u;
-    if (lab_from)
-        hipLaunchKernelGGL(rbg::label_seed, dim3(seed_blocks), dim3(256), 0, stream, descs, pi,
-                           tab_val, tab_lab, lab, lab_from, err);
-    *launches = lab_from ? 1u : 0u;
-    for (uint32_t lo = 0, hi = first_phase < ntiles ? first_phase : ntiles; lo < ntiles;
-         lo = hi, hi = 2u * hi < ntiles ? 2u * hi : ntiles) {
-        hipLaunchKernelGGL(rbg::scan_reuse, dim3(hi - lo), dim3(rbg::kWavesPerBlock * 64), 0,
-                           stream, descs, pi, lo, chunks, tab_val, tab_lab, lab, lab_from, j2, j3,
-                           hdr, runs, err);
-        ++*launches;
-    }
+// Batched label-reuse launches (plain method): the seed of every piece's
+// prime (pieces[0 .. npieces), nblocks blocks of kSeedPerThread x 256
+// positions), or one phase (one block per tile).
+extern "C" hipError_t rbg_launch_label_seed(const rbg::PrimeDesc *descs,
+                                            const rbg::ReusePiece *pieces, uint32_t npieces,
+                                            uint32_t nblocks, const uint32_t *tab_val,
+                                            const uint16_t *tab_lab, uint16_t *lab,
+                                            uint32_t lab_from, uint32_t *err,
+                                            hipStream_t stream) {
+    hipLaunchKernelGGL(rbg::label_seed, dim3(nblocks), dim3(256), 0, stream, descs, pieces,
+                       npieces, tab_val, tab_lab, lab, lab_from, err);
     return hipGetLastError();
 }
+
+extern "C" hipError_t rbg_launch_scan_reuse(const rbg::PrimeDesc *descs,
+                                            const rbg::ReusePiece *pieces, uint32_t npieces,
+                                            uint32_t nblocks, uint32_t chunks,
+                                            const uint32_t *tab_val, const uint16_t *tab_lab,
+                                            uint16_t *lab, uint32_t lab_from, rbg::TileHdr *hdr,
+                                            rbg::RunSum *runs, uint32_t *err,
+                                            hipStream_t stream) {
+    if (chunks % rbg::kReuseChunks) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(rbg::scan_reuse, dim3(nblocks), dim3(rbg::kWavesPerBlock * 64), 0, stream,
+                       descs, pieces, npieces, chunks, tab_val, tab_lab, lab, lab_from, hdr, runs,
+                       err);
+    return hipGetLastError();
+}
+
+extern "C" uint32_t rbg_seed_per_block(void) { return 256u * rbg::kSeedPerThread; }
 
 extern "C" hipError_t rbg_launch_merge(const rbg::PrimeDesc *descs, uint32_t nprimes,
                                        const rbg::TileHdr *hdr, const rbg::RunSum *runs,

@@ -46,6 +46,15 @@ RBG_HD u32 colour_code(u32 j, u32 m) {
 }
 
 // Per-prime constants, one record per prime of the block.
+// One prime's part of a batched label-reuse launch (plain method): blocks
+// [block0, next piece's block0) of the launch work for prime pi, the scan
+// tiles from tile_lo (seed launches: positions from 1), its labels at
+// lab + lab_off, labels j2 = j(2), j3 = j(3).
+struct ReusePiece {
+    u32 block0, pi, tile_lo, j2, j3, pad;
+    u64 lab_off;
+};
+
 struct PrimeDesc {
     u32 p;          // the prime
     u32 nh;         // (p-1)/2, length of the half interval scanned
