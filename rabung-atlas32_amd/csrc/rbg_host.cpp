@@ -618,8 +618,11 @@ u32 run_floor(u64 nh, u32 r) {
 }
 
 // Label-reuse scan (plain method, primes below 2^32): tiles of the first
-// phase, whose positions read the labels seeded by direct powers.
-constexpr u32 kReuseFirstPhase = 256;
+// phase, whose positions read the labels seeded by direct powers.  Batched
+// phases fill the chip from small windows on, so the seed can be short
+// (block 2^24: 0.287 s with 256 first-phase tiles, 0.264 / 0.253 / 0.247 s
+// with 128 / 64 / 32; the top primes do not change).
+constexpr u32 kReuseFirstPhase = 32;
 constexpr u32 kReuseChunkUnit = 12;  // segments start at 1 mod 6, passes of 12 chunks
 
 // Label j of v (x^e = z^j) from a prime's sorted root table.
@@ -726,9 +729,9 @@ int dense_run(rbg_ctx *c, const u64 *primes, const std::vector<u32> &idx_in, boo
     // side in the budget the other workspaces leave (and below 2^32
     // entries): per group one seed launch and one launch per phase over all
     // its primes, so that a phase fills the chip however few tiles each
-    // prime has in it.  Phase windows of tiles [0, 256), [256, 512),
-    // [512, 1024), ...: a position x of a phase reads x/2 and x/3, which lie
-    // in the seed or an earlier phase of its own prime.
+    // prime has in it.  Phase windows of tiles [0, f), [f, 2f), [2f, 4f),
+    // ... (f = kReuseFirstPhase): a position x of a phase reads x/2 and x/3,
+    // which lie in the seed or an earlier phase of its own prime.
     c->h_pieces.clear();
     c->h_reuse.clear();
     const u32 reuse_pos = waves * reuse_chunks * 64u;  // positions per tile
