@@ -987,6 +987,10 @@ int sparse_plan(rbg_ctx *c, const u64 *primes, const std::vector<u32> &idx_in,
                 f.plane_off = (u32)words;  // units of 16 words
                 f.sigma0 = (((p - 1) / 2) % r) == 0;
                 f.pblk = (u32)pblk;
+                f.B = s.B;
+                f.D = s.D;
+                f.Q = s.Q;
+                f.inv_off = s.inv_off;
                 if (!c->cfg.spec_off) {
                     const u32 q0 = spec_threshold(p, r);
                     f.qspec = q0 ? std::min(s.Q, q0 + c->cfg.spec_boost) : 0u;

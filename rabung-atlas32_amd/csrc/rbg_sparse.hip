@@ -1035,11 +1035,11 @@ __device__ __forceinline__ bool task_setup(const SparsePrime *P, const SparsePro
                                            unsigned long long *best_slot, u32 *failed,
                                            u32 ext_limit, FilterCtx<D> &c, u32 &sp, u32 &phi) {
     const SparsePrime *sg = P + fg->prime;
-    if (dd > sg->D) return false;
+    if (dd > fg->D) return false;
     const u64 key = __hip_atomic_load(best_slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     // the running best, or the speculative threshold when it is higher (a
     // profile whose filter finds no run reaching it is recomputed)
-    const u32 q = min(max((u32)(key >> 32), fg->qspec), sg->Q);
+    const u32 q = min(max((u32)(key >> 32), fg->qspec), fg->Q);
     if (q < 3u) return false;
     u32 t = max(3u, (q + 1u) / 2u);
     if (t > q) t = q;
@@ -1048,7 +1048,7 @@ __device__ __forceinline__ bool task_setup(const SparsePrime *P, const SparsePro
     c.s = sg;
     c.f = fg;
     // rows of g = gcd(d, 6): the full row for g = 1, else the class rows
-    const u32 B = sg->B, np = fg->np;
+    const u32 B = fg->B, np = fg->np;
     const u32 wg = ((dd & 1u) ? 1u : 2u) * (dd % 3u ? 1u : 3u);
     const u32 row0 = first_row_of(wg);
     const u32 *prof_rows = planes + (u64)fg->plane_off * 16u;
@@ -1069,7 +1069,7 @@ __device__ __forceinline__ bool task_setup(const SparsePrime *P, const SparsePro
     c.best = best_slot;
     c.failed = failed;
     c.d = dd;
-    c.dinv = inv[(u64)sg->inv_off + dd];
+    c.dinv = inv[(u64)fg->inv_off + dd];
     c.t = t;
     c.q = q;
     c.B = B;

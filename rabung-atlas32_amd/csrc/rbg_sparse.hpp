@@ -33,6 +33,9 @@ struct SparseProf {
     u32 sigma0;     // r | (p-1)/2
     u32 pblk;       // first plane block
     u32 qspec;      // speculative threshold (0: none), at most Q
+    // copies of the prime's cover, threshold cap and inverse offset, so that
+    // a filter task's setup needs no second dependent load
+    u32 B, D, Q, inv_off;
 };
 
 // Colour rows of one profile.  Row 0 is the full signed table: bit i of
