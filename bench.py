@@ -23,7 +23,8 @@ METRIC = "exact (p,r) profiles/sec near 2^32, plain vs batched; p50/p99 query la
 HBM_PEAK_GBS = 8000.0
 # sp_filter's unit of work: one 32-anchor dword of a plane window compared
 # with the base window (one funnel shift, one xor, one or, and a share of the
-# address arithmetic)
+# address arithmetic), counted only for the stages a word needed (through the
+# stage that killed its last anchor), not for the rest of its last step
 OPS_PER_WINDOW_DWORD = 4
 # scan_tiles' unit of work: one Montgomery product (a 32x32->64 multiply, a
 # low and a high 32-bit multiply, a subtract with borrow, a select, an add)

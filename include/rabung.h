@@ -238,7 +238,9 @@ typedef struct rbg_timing {
     uint64_t positions;      /* half-interval positions of the dense scan */
     uint64_t products;       /* Montgomery products of the dense scan */
     uint64_t launches;       /* kernel launches of the dominant stages */
-    uint64_t filter_windows; /* plane windows compared by the filter, in 32-anchor dwords */
+    uint64_t filter_windows; /* plane windows the filter needed, in 32-anchor dwords: per
+                                word the stages through the one that killed its last
+                                anchor (all t - 1 for a passing word), plus the screen's */
     uint64_t filter_words;   /* 64- or 96-anchor words the filter examined */
     uint64_t filter_completions; /* surviving anchors extended to their runs */
     double a4_host_ms;       /* elliptic A4 counts on host threads (beside the device) */

@@ -702,10 +702,10 @@ __device__ __forceinline__ void drain_queue(const FilterCtx<D> &c, const u32 *co
         }
     }
 }
-// m | (a ^ b) in one V_BITOP3_B32 (truth table 0xF6 over the inputs m, a, b,
-// m the most significant index bit).
-__device__ __forceinline__ u32 or_xor(u32 m, u32 a, u32 b) {
-    return __builtin_amdgcn_bitop3_b32(m, a, b, 0xF6);
+// r & ~(a ^ b) in one V_BITOP3_B32 (truth table 0x90 over the inputs r, a, b,
+// r the most significant index bit).
+__device__ __forceinline__ u32 and_xnor(u32 r, u32 a, u32 b) {
+    return __builtin_amdgcn_bitop3_b32(r, a, b, 0x90);
 }
 
 
@@ -761,7 +761,9 @@ __device__ __forceinline__ ClassWords class_words(const Spans &S, const TaskRows
 // distinct stages from min(j, t - min(G, t-1)), so a word's last step
 // re-tests stages it has already passed rather than read past stage t-1: every
 // address stays inside the row and the stage offsets of a step are
-// wave-uniform.
+// wave-uniform.  The work count n_windows takes, per word, the stages through
+// the one that killed its last anchor (t - 1 for a passing word), not the
+// stages its last step loaded after that.
 constexpr u32 kRefill = 48;  // smallest cohort of idle lanes that takes new words
 
 template <class D, int NP>
@@ -812,6 +814,7 @@ __device__ void filter_task(const FilterCtx<D> &c, u32 sp, u32 phi, u32 *qb, u64
 #pragma unroll
     for (int g = 0; g < G; ++g) ug[g] = min((u32)g, Gt - 1u) * dg;
     u32 next = 0, qn = 0;
+    u32 used = 0;  // stages the lane's last word needed (counted when it takes the next)
     bool pend = false;  // a passing word that did not fit the queue
     u32 k0 = 0;
     u64 live = 0;    // surviving anchors 0 .. 63 of the word
@@ -836,10 +839,12 @@ __device__ void filter_task(const FilterCtx<D> &c, u32 sp, u32 phi, u32 *qb, u64
             const u64 idle = ballot64(!have && !pend);
             const u32 nidle = (u32)__builtin_popcountll(idle);
             if (!stall && idle && next < total && (nidle >= kRefill || nidle == 64u)) {
+                bool took = false;
                 if (!have && !pend) {
                     // the stages this lane's last word took (t - 1 when it
                     // passed) are counted now, once per word
-                    n_windows += (u32)NP * (WB / 32u) * min(j - 1u, t - 1u);
+                    n_windows += (u32)NP * (WB / 32u) * used;
+                    used = 0;
                     j = 1;
                     const u32 my = next + lanes_below(idle);
                     if (my < total) {
@@ -897,11 +902,12 @@ __device__ void filter_task(const FilterCtx<D> &c, u32 sp, u32 phi, u32 *qb, u64
                             for (int pb = 0; pb < NP; ++pb)
 #pragma unroll
                                 for (u32 k = 0; k < NW; ++k) braw[pb][k] = plb[pb][wo + k];
-                            ++n_words;
+                            took = true;
                         }
                     }
                 }
                 next += (u32)__builtin_popcountll(idle);
+                n_words += (u32)__builtin_popcountll(ballot64(took));  // wave-uniform
                 // the new words' base windows (counted for the 32-bit filter
                 // only: the wide one has no registers to spare)
                 if constexpr (!D::wide) n_loads += (u32)NP;
@@ -929,31 +935,45 @@ __device__ void filter_task(const FilterCtx<D> &c, u32 sp, u32 phi, u32 *qb, u64
                     }
                 }
                 __builtin_amdgcn_sched_barrier(0);
-                // mismatches of anchors 0-31, 32-63 and 64-95, one 3-input
-                // bit operation per window dword
-                u32 m0 = 0, m1 = 0, m2 = 0;
+                // live anchors 0-31, 32-63 and 64-95 cleared where a window
+                // dword differs from the base, one 3-input bit operation per
+                // window dword, stage by stage; `alive` counts the stages
+                // after which the word still had a live anchor, so a word that
+                // dies in this step is known to have needed exactly the
+                // stages through the one that killed it
+                u32 bw[NP][3];
 #pragma unroll
                 for (int pb = 0; pb < NP; ++pb) {
-                    const u32 b0 = __builtin_amdgcn_alignbit(braw[pb][1], braw[pb][0], sh);
-                    const u32 b1 = __builtin_amdgcn_alignbit(braw[pb][2], braw[pb][1], sh);
-                    const u32 b2 = __builtin_amdgcn_alignbit(braw[pb][3], braw[pb][2], sh);
-#pragma unroll
-                    for (int g = 0; g < G; ++g) {
-                        const u32 *w = raw[g][pb];
-                        m0 = or_xor(m0, b0, __builtin_amdgcn_alignbit(w[1], w[0], shg[g]));
-                        m1 = or_xor(m1, b1, __builtin_amdgcn_alignbit(w[2], w[1], shg[g]));
-                        m2 = or_xor(m2, b2, __builtin_amdgcn_alignbit(w[3], w[2], shg[g]));
-                    }
+                    bw[pb][0] = __builtin_amdgcn_alignbit(braw[pb][1], braw[pb][0], sh);
+                    bw[pb][1] = __builtin_amdgcn_alignbit(braw[pb][2], braw[pb][1], sh);
+                    bw[pb][2] = __builtin_amdgcn_alignbit(braw[pb][3], braw[pb][2], sh);
                 }
-                live &= ~(((u64)m1 << 32) | m0);
-                live_hi &= ~m2;
+                u32 r0 = (u32)live, r1 = (u32)(live >> 32), r2 = live_hi, alive = 0;
+#pragma unroll
+                for (int g = 0; g < G; ++g) {
+#pragma unroll
+                    for (int pb = 0; pb < NP; ++pb) {
+                        const u32 *w = raw[g][pb];
+                        r0 = and_xnor(r0, bw[pb][0], __builtin_amdgcn_alignbit(w[1], w[0], shg[g]));
+                        r1 = and_xnor(r1, bw[pb][1], __builtin_amdgcn_alignbit(w[2], w[1], shg[g]));
+                        r2 = and_xnor(r2, bw[pb][2], __builtin_amdgcn_alignbit(w[3], w[2], shg[g]));
+                    }
+                    if (g + 1 < G) alive += (r0 | r1 | r2) != 0u ? 1u : 0u;
+                }
+                live = ((u64)r1 << 32) | r0;
+                live_hi = r2;
+                // first stage of this step: j, or t - Gt on a word's last
+                // (clamped) step, whose earlier stages repeat passed ones
+                const u32 s0 = io < io_min ? t - Gt : j;
                 j += G;
                 io -= G * (int)dg;
                 if (!(live | live_hi)) {
                     have = false;
+                    used = s0 + alive;  // stages 1 .. s0 + alive
                 } else if (j >= t) {
                     have = false;
                     fin = true;
+                    used = t - 1u;
                 }
             }
             // a passing word takes one queue entry per half with survivors:
@@ -986,7 +1006,8 @@ __device__ void filter_task(const FilterCtx<D> &c, u32 sp, u32 phi, u32 *qb, u64
         }
         // no word in flight: count the last words' stages, screen and
         // complete the queue, then requeue the lanes that were held back
-        n_windows += (u32)NP * (WB / 32u) * min(j - 1u, t - 1u);
+        n_windows += (u32)NP * (WB / 32u) * used;
+        used = 0;
         wave_lds_fence();
         if (qn) drain_queue<D, NP>(c, plb, qn, qb, qs, lane, n_windows, n_done);
         wave_lds_fence();
@@ -1105,10 +1126,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void s
     __shared__ __attribute__((aligned(16))) u32 s_cw[kFilterWaves][16];
     if (threadIdx.x < 4) s_stats[threadIdx.x] = 0;
     __syncthreads();
-    // u32 per lane, added to the block's totals before they can overflow: a
-    // task adds less than 2^28 to n_windows and n_done, and n_words stays
-    // below n_windows (every word takes at least one step)
-    u32 n_windows = 0, n_words = 0, n_done = 0;
+    // u32 counts, added to the block's totals before they can overflow: a
+    // task adds less than 2^28 to a lane's n_windows and n_done and less
+    // than 2^23 to its wave's n_words
+    u32 n_windows = 0, n_done = 0;  // per lane
+    u32 n_words = 0;  // words taken by the wave (wave-uniform)
     u32 n_loads = 0;  // wave-level load instructions of the word loop (wave-uniform)
     const u32 wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     u32 *qb = s_qb[wave];
@@ -1190,19 +1212,23 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void s
         default: filter_task<D, 4>(c, sp, phi, qb, qs, cw, n_windows, n_words, n_done,
                                   n_loads); break;
         }
-        if ((n_windows | n_done) >= 0x80000000u) {
+        if ((n_windows | n_done | n_words) >= 0x80000000u) {
             atomicAdd(&s_stats[0], (unsigned long long)n_windows);
-            atomicAdd(&s_stats[1], (unsigned long long)n_words);
             atomicAdd(&s_stats[2], (unsigned long long)n_done);
-            if (lane == 0) atomicAdd(&s_stats[3], (unsigned long long)n_loads);
+            if (lane == 0) {
+                atomicAdd(&s_stats[1], (unsigned long long)n_words);
+                atomicAdd(&s_stats[3], (unsigned long long)n_loads);
+            }
             n_windows = n_words = n_done = n_loads = 0;
         }
     }
     }
     atomicAdd(&s_stats[0], (unsigned long long)n_windows);
-    atomicAdd(&s_stats[1], (unsigned long long)n_words);
     atomicAdd(&s_stats[2], (unsigned long long)n_done);
-    if (lane == 0) atomicAdd(&s_stats[3], (unsigned long long)n_loads);
+    if (lane == 0) {
+        atomicAdd(&s_stats[1], (unsigned long long)n_words);
+        atomicAdd(&s_stats[3], (unsigned long long)n_loads);
+    }
     __syncthreads();
     if (threadIdx.x < 4) atomicAdd(&stats[threadIdx.x], s_stats[threadIdx.x]);
 }
