@@ -1083,8 +1083,13 @@ int sparse_run(rbg_ctx *c, rbg_record *recs, u32 *err, std::vector<u32> &failed_
     SparseProf *dF = (SparseProf *)c->sp_profs.ptr;
     unsigned long long *best = (unsigned long long *)c->sp_best.ptr;
     u32 *failed = (u32 *)c->sp_failed.ptr;
-    // the filter's XCD-aware schedule needs a grid that is a multiple of 8
+    // the filter's XCD-aware schedule needs a grid that is a multiple of 8;
+    // the 33-bit filter is compiled for 5 resident blocks per CU (its
+    // register budget), so its grid stops there (blocks past the resident
+    // ones would only start once the queues are empty)
     const u32 grid = std::max(8u, ((u32)c->cus * blocks_per_cu(c) + 7u) / 8u * 8u);
+    const u32 grid_wide =
+        std::max(8u, ((u32)c->cus * std::min(blocks_per_cu(c), 5u) + 7u) / 8u * 8u);
     // task lists of the XCDs, per filter launch (domain): XCD x owns the
     // launch's profiles x, x+8, ...; its list holds their denominators 1..D
     // back to back, and tpre holds, per XCD, the first task of each of its
@@ -1147,7 +1152,8 @@ int sparse_run(rbg_ctx *c, rbg_record *recs, u32 *err, std::vector<u32> &failed_
                                        (const u16 *)c->sp_lab.ptr, best + f0, failed,
                                        (unsigned long long *)c->sp_stats.ptr,
                                        (u32 *)((char *)c->sp_stats.ptr + kQueueOffset) + 8 * wide,
-                                       wide, grid, xcd_group(c), filter_chunk(c),
+                                       wide, wide ? grid_wide : grid, xcd_group(c),
+                                       filter_chunk(c),
                                        extend_limit(c), s));
         }
         return RBG_OK;

@@ -1101,8 +1101,16 @@ __device__ __forceinline__ bool task_setup(const SparsePrime *P, const SparsePro
 
 // D = Dom32 for the profiles of primes below 2^32, Dom33 for the wider
 // primes (one launch per domain, each over its own range of profiles).
+// Waves per SIMD the filter is compiled for: 6 (at most 80 VGPRs) for the
+// 32-bit domain; the 33-bit one keeps 64-bit residues and its completion
+// beside the word loop and would spill at 80 (a scratch reload per step and
+// a wait on the base loads at refill), so it takes 5 waves and 88 VGPRs
+// (filter of the 256 largest primes below 2^33: 80.2 -> 70.6 ms).
+template <class D> struct FilterWaves { static constexpr int value = 6; };
+template <> struct FilterWaves<Dom33> { static constexpr int value = 5; };
+
 template <class D>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void sp_filter(const SparsePrime *__restrict__ P,
+__global__ __launch_bounds__(256, FilterWaves<D>::value) void sp_filter(const SparsePrime *__restrict__ P,
                                                  const SparseProf *__restrict__ prof, u32 nprof,
                                                  u32 dmax, const u32 *__restrict__ tpre,
                                                  const u32 *__restrict__ planes,
