@@ -48,13 +48,44 @@ __device__ __forceinline__ u64 runs_of_ones(u64 X, u32 k) {
     return X & (X >> (k - (1u << (31 - __builtin_clz(k)))));
 }
 
+// Prefix maximum over the wave's lanes (lane i gets the maximum of lanes
+// 0..i): four in-row shifts and two row broadcasts, one V_MAX_U32 with DPP
+// each.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ u32 max_dpp(u32 v) {
+    return max(v, (u32)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, ROWS, 0xF, true));
+}
+__device__ __forceinline__ u32 prefix_max64(u32 v) {
+    v = max_dpp<0x111, 0xF>(v);  // row_shr:1
+    v = max_dpp<0x112, 0xF>(v);  // row_shr:2
+    v = max_dpp<0x114, 0xF>(v);  // row_shr:4
+    v = max_dpp<0x118, 0xF>(v);  // row_shr:8
+    v = max_dpp<0x142, 0xA>(v);  // row_bcast:15 into rows 1 and 3
+    v = max_dpp<0x143, 0xC>(v);  // row_bcast:31 into rows 2 and 3
+    return v;
+}
+
+// Whether the same-as-left mask N of the wave holds k >= 1 consecutive ones,
+// computed on the vector unit: zp is lane + 1 on the lanes whose bit of N is
+// clear and 0 elsewhere, so after the prefix maximum lane i holds one past
+// the last clear bit at or below i, and i + 1 - zp ones end at lane i.  The
+// same answer as runs_of_ones(N, k) != 0, for a kernel whose scalar unit is
+// the busier one.
+__device__ __forceinline__ bool has_ones_run(u32 zp, u32 k) {
+    const u32 lane1 = (threadIdx.x & 63u) + 1u;
+    return __builtin_amdgcn_ballot_w64(prefix_max64(zp) + k <= lane1) != 0ull;
+}
+
 // Longest interior run of one 64-position chunk.  N: same-as-left mask of the
 // valid positions; f < h are the first and last change bits.  Interior runs
 // start at the change bits in [f, h) and close before h; they start after
 // every run already seen, so only a run strictly longer than `best` matters.
+// `cand` false: the chunk holds no `kq` consecutive ones of N for some
+// 1 <= kq <= best, found beforehand (no run can beat best).
 __device__ __forceinline__ void interior_best(u64 N, u32 f, u32 h, u32 base, u32 &best,
-                                              u32 &bstart) {
+                                              u32 &bstart, bool cand = true) {
     if (best > 62) return;
+    if (!cand) return;
     // quick reject on the whole chunk (the interior mask only narrows N):
     // once best is large, most chunks hold no best consecutive equal pairs
     if (best != 0 && runs_of_ones(N, best) == 0) return;
@@ -106,7 +137,9 @@ __device__ __forceinline__ void order_init(OrderState &s, u32 floor = 1) {
 
 // Advance the state over one chunk at relative offset `base` with change mask
 // C and same mask N (both restricted to the valid positions).
-__device__ __forceinline__ void order_update(OrderState &s, u64 C, u64 N, u32 base) {
+// cand: as for interior_best (true when the caller has not tested the chunk).
+__device__ __forceinline__ void order_update(OrderState &s, u64 C, u64 N, u32 base,
+                                             bool cand = true) {
     if (C == 0) return;
     const u32 f = (u32)__builtin_ctzll(C);
     const u32 closed = base + f - s.cur;
@@ -120,7 +153,7 @@ __device__ __forceinline__ void order_update(OrderState &s, u64 C, u64 N, u32 ba
         const u32 h = 63u - (u32)__builtin_clzll(rest);
         if (s.prefix == 0)  // the head run closes at the second change (first chunk)
             s.prefix = base + (u32)__builtin_ctzll(rest) - s.cur;
-        interior_best(N, f, h, base, s.best, s.bstart);
+        interior_best(N, f, h, base, s.best, s.bstart, cand);
         s.cur = base + h;
     }
 }

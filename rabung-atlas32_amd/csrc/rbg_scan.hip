@@ -79,6 +79,8 @@ struct WaveRuns {
     OrderState st[8];
     u32 a4;
     u64 prevN;
+    u32 a4v;         // per-lane A4 counts (the vector-unit form of runs_chunk)
+    u32 pn62, pn63;  // r = 2 same-as-left bits of the previous chunk's lanes 62, 63
     u32 carry_code, first_code, last_code;
 };
 
@@ -87,16 +89,30 @@ __device__ __forceinline__ void runs_init(WaveRuns &w, const PrimeDesc &d) {
     for (int k = 0; k < 8; ++k) order_init(w.st[k], run_floor(d, (u32)k));
     w.a4 = 0;
     w.prevN = 0;
+    w.a4v = 0;
+    w.pn62 = w.pn63 = 0;
     w.carry_code = w.first_code = w.last_code = 0;
 }
 
 // Chunk c (positions base .. base+valid-1 of the segment) with this lane's
 // colour code: one DPP shift for the left neighbour, one ballot per order.
+// kVecReject: the test that lets most chunks skip the interior-run search
+// (no `best` consecutive equal neighbours) runs on the vector unit (a prefix
+// maximum over the lanes, nine VALU instructions) instead of the scalar
+// unit's doubling (about 30 scalar instructions), and so does the A4 count
+// (three consecutive equal neighbours of r = 2, per lane, summed at the end
+// of the segment); for the label-reuse scan, whose scalar unit is the busier
+// one.
+template <bool kVecReject = false>
 __device__ __forceinline__ void runs_chunk(WaveRuns &w, u32 code, u32 c, u32 base, u32 valid,
                                            u32 mask) {
     const u64 vmask = valid == 64u ? ~0ull : ((1ull << valid) - 1ull);
     const u32 left = shift_up_one(code, w.carry_code);
     const u32 diff = code ^ left;
+    const u32 lane = threadIdx.x & 63u;
+    // lanes whose bit of N is clear whatever the colours: past the valid
+    // positions, and position 0 of the segment (a forced change)
+    const u32 zfix = (lane >= valid || (c == 0 && lane == 0)) ? lane + 1u : 0u;
     if (c == 0) w.first_code = __builtin_amdgcn_readfirstlane(code);
     w.carry_code = __builtin_amdgcn_readlane(code, 63);
     w.last_code = __builtin_amdgcn_readlane(code, valid - 1);
@@ -104,24 +120,48 @@ __device__ __forceinline__ void runs_chunk(WaveRuns &w, u32 code, u32 c, u32 bas
     for (int k = 0; k < 8; ++k) {
         const u32 r = (u32)k + 2u;
         if (!(mask & (1u << k))) continue;
-        u64 C = ballot64((diff & plane_bits(r)) != 0u);
+        const bool chg = (diff & plane_bits(r)) != 0u;
+        u64 C = ballot64(chg);
         if (c == 0) C |= 1ull;
         C &= vmask;
         const u64 N = ~C & vmask;
-        if (r == 2) {
-            u64 W = N & ((N << 1) | (w.prevN >> 63)) & ((N << 2) | (w.prevN >> 62));
-            w.a4 += (u32)__builtin_popcountll(W);
-            w.prevN = N;
+        bool cand = true;
+        if constexpr (kVecReject) {
+            const u32 b = w.st[k].best;
+            if (b != 0u && b <= 62u) cand = has_ones_run(chg ? lane + 1u : zfix, b);
         }
-        order_update(w.st[k], C, N, base);
+        if (r == 2) {
+            if constexpr (kVecReject) {
+                // lane i: same-as-left at i, i-1 and i-2 (the previous
+                // chunk's lanes 63 and 62 before lane 0)
+                const u32 nb = (chg || zfix) ? 0u : 1u;
+                const u32 s1 = shift_up_one(nb, w.pn63);
+                const u32 s2 = shift_up_one(s1, w.pn62);
+                w.a4v += nb & s1 & s2;
+                w.pn63 = __builtin_amdgcn_readlane(nb, 63);
+                w.pn62 = __builtin_amdgcn_readlane(nb, 62);
+            } else {
+                u64 W = N & ((N << 1) | (w.prevN >> 63)) & ((N << 2) | (w.prevN >> 62));
+                w.a4 += (u32)__builtin_popcountll(W);
+                w.prevN = N;
+            }
+        }
+        order_update(w.st[k], C, N, base, cand);
     }
 }
 
 // Per-order summaries of the segment (positions relative to its start)
 // into out[r-2]; codes_out = first code | last code << 16.
+template <bool kVecReject = false>
 __device__ __forceinline__ void runs_finish(WaveRuns &w, u32 seg_len, u32 mask, RunSum *out,
                                             u32 &codes_out) {
     const u32 lane = threadIdx.x & 63u;
+    if constexpr (kVecReject) {
+        u32 a4 = w.a4v;
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) a4 += (u32)__shfl_xor((int)a4, o);
+        w.a4 = a4;
+    }
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
         if (!(mask & (1u << k))) continue;
@@ -270,12 +310,12 @@ __device__ __forceinline__ void wave_scan_reuse(const PrimeDesc &d, const PrimeC
             const u32 j = s_jp[(c << 6) + lane];
             const u32 x = seg_start + base + lane;
             if (lane < valid && x > lab_from && x <= lab_to) lab[x] = (u16)j;
-            runs_chunk(w, s_code[j], cc, base, valid, mask);
+            runs_chunk<true>(w, s_code[j], cc, base, valid, mask);
         }
         wave_lds_fence();  // the next pass rewrites s_jp
     }
     if (__builtin_amdgcn_ballot_w64(bad != 0)) atomicOr(err, 1u);
-    runs_finish(w, seg_len, mask, out, codes_out);
+    runs_finish<true>(w, seg_len, mask, out, codes_out);
 }
 
 constexpr int kWavesPerBlock = 4;
