@@ -364,10 +364,17 @@ __global__ __launch_bounds__(256) void sp_seed_scan(const SparsePrime *__restric
             carry = __builtin_amdgcn_readlane(col, 63);
             if (c == 0) fcol = __builtin_amdgcn_readfirstlane(col);
             lcol = __builtin_amdgcn_readlane(col, valid - 1);
-            u64 C = ballot64(col != left);
+            const bool chg = col != left;
+            u64 C = ballot64(chg);
             if (c == 0) C |= 1ull;
             C &= vmask;
-            order_update(st, C, ~C & vmask, base);
+            // the interior-run search's rejection test on the vector unit
+            // (the scalar unit carries the rest of the run bookkeeping)
+            bool cand = true;
+            if (st.best != 0u && st.best <= 62u)
+                cand = has_ones_run(chg || lane >= valid || (c == 0 && lane == 0) ? lane + 1u : 0u,
+                                    st.best);
+            order_update(st, C, ~C & vmask, base, cand);
         }
     }
     const u32 suffix = order_finish(st, len);
