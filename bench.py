@@ -442,9 +442,11 @@ def microbench():
     names = ["add_or_u32", "mul_lo_or_u32", "mul_hi_or_u32", "alignbit_b32", "montgomery_product",
              "add_u32", "mul_lo_u32"]
     rates = {n: round(rabung_amd.microbench(k) / 1e12, 3) for k, n in enumerate(names)}
-    layout = {4: "lanes_4B_apart", 8: "lanes_8B_apart", 4160: "line_per_lane"}
+    layout = {4: "lanes_4B_apart", 8: "lanes_8B_apart", 12: "lanes_12B_apart",
+              4160: "line_per_lane"}
     loads = {"%d_dword%s_%s" % (w, "s" if w > 1 else "", layout[st]):
-             round(rabung_amd.microbench(10 + w, st), 2) for st in (4, 8, 4160) for w in (1, 2, 3, 4)}
+             round(rabung_amd.microbench(10 + w, st), 2)
+             for st in (4, 8, 12, 4160) for w in (1, 2, 3, 4)}
     print(json.dumps({"valu_tops": rates, "cu_cycles_per_wave_load": loads}), flush=True)
     return 0
 

@@ -292,7 +292,8 @@ int rbg_a4_certificate(uint32_t p, uint32_t *a4, uint64_t *attempts, char *out, 
  * Montgomery products, plain add and plain v_mul_lo_u32 (8 independent
  * chains per thread, 8 waves per SIMD); kind 11-14: CU cycles per wave-wide
  * load of 1-4 dwords per lane with lanes `param` = 4 (consecutive dwords),
- * 8 (one contiguous span) or 4160 (one line each) bytes apart, from an
+ * 8 or 12 (one contiguous span; 12 is the anchor filter's word stride) or
+ * 4160 (one line each) bytes apart, from an
  * L2-resident buffer, 8 independent loads in flight per wave (issue cost,
  * not latency). */
 int rbg_microbench(int device, int kind, uint32_t param, double *out);

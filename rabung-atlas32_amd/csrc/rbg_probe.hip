@@ -171,7 +171,8 @@ extern "C" int rbg_microbench(int device, int kind, uint32_t param, double *out)
         }
         if (ms <= 0) rc = RBG_EDEVICE;
         else *out = (double)blocks * 256.0 * iters * rbg::kChains / (ms * 1e-3);
-    } else if (kind >= 11 && kind <= 14 && (param == 4u || param == 8u || param == 4160u)) {
+    } else if (kind >= 11 && kind <= 14 &&
+               (param == 4u || param == 8u || param == 12u || param == 4160u)) {
         const uint32_t bytes = 1u << 20, iters = 2000;
         uint32_t *buf = nullptr;
         if (hipMalloc(&buf, bytes + 64) != hipSuccess || hipMemset(buf, 1, bytes + 64) != hipSuccess) {
