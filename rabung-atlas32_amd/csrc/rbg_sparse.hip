@@ -393,18 +393,46 @@ __global__ __launch_bounds__(256) void sp_seed_scan(const SparsePrime *__restric
     }
 }
 
-// One thread per profile: ordered merge of its segment summaries; the key of
-// the longest seed run (partial end run included) and the prefix L.
-__global__ void sp_seed_merge(const SparsePrime *__restrict__ P, SparseProf *__restrict__ prof,
-                              u32 nprof, const Agg *__restrict__ segs, u32 segs_per_prof,
-                              unsigned long long *__restrict__ best, u32 *__restrict__ failed) {
-    const u32 k = blockIdx.x * blockDim.x + threadIdx.x;
+// One wave per profile: ordered merge of its segment summaries (each lane
+// merges a contiguous range of segments, then the lanes' results are merged
+// pairwise in lane order, six rounds); the key of the longest seed run
+// (partial end run included) and the prefix L.
+__device__ __forceinline__ Agg shfl_down_agg(const Agg &a, u32 off) {
+    Agg o;
+    o.first_x = (u32)__shfl_down((int)a.first_x, off);
+    o.len = (u32)__shfl_down((int)a.len, off);
+    o.fcode = (u32)__shfl_down((int)a.fcode, off);
+    o.lcode = (u32)__shfl_down((int)a.lcode, off);
+    o.prefix = (u32)__shfl_down((int)a.prefix, off);
+    o.suffix = (u32)__shfl_down((int)a.suffix, off);
+    o.best = (u32)__shfl_down((int)a.best, off);
+    o.bstart = (u32)__shfl_down((int)a.bstart, off);
+    o.a4 = (u32)__shfl_down((int)a.a4, off);
+    return o;
+}
+
+__global__ __launch_bounds__(256) void sp_seed_merge(const SparsePrime *__restrict__ P,
+                                                     SparseProf *__restrict__ prof, u32 nprof,
+                                                     const Agg *__restrict__ segs,
+                                                     u32 segs_per_prof,
+                                                     unsigned long long *__restrict__ best,
+                                                     u32 *__restrict__ failed) {
+    const u32 k = __builtin_amdgcn_readfirstlane(blockIdx.x * 4u + (threadIdx.x >> 6));
     if (k >= nprof) return;
-    const SparseProf f = prof[k];
-    const SparsePrime s = P[f.prime];
+    const u32 lane = threadIdx.x & 63u;
+    const u32 per = (segs_per_prof + 63u) / 64u;
+    const u32 i0 = min(lane * per, segs_per_prof), i1 = min(i0 + per, segs_per_prof);
     Agg acc;
     acc.len = 0;
-    for (u32 i = 0; i < segs_per_prof; ++i) acc = merge(acc, segs[(u64)k * segs_per_prof + i], ~0u);
+    for (u32 i = i0; i < i1; ++i) acc = merge(acc, segs[(u64)k * segs_per_prof + i], ~0u);
+#pragma unroll
+    for (u32 off = 1; off < 64u; off <<= 1) {
+        const Agg right = shfl_down_agg(acc, off);
+        if ((lane & (2u * off - 1u)) == 0) acc = merge(acc, right, ~0u);
+    }
+    if (lane != 0) return;
+    const SparseProf f = prof[k];
+    const SparsePrime s = P[f.prime];
     // runs inside 1..B < p/2 have reflected start = start
     best[k] = ((unsigned long long)acc.best << 32) | (0xFFFFFFFFu - acc.bstart);
     const bool whole = acc.prefix == s.B;  // the identity run may continue past B
@@ -1319,7 +1347,7 @@ extern "C" hipError_t rbg_sparse_seed(const SparsePrime *P, SparseProf *prof, ui
     const uint32_t waves = nprof * segs_per_prof;
     hipLaunchKernelGGL(rbg::sp_seed_scan, dim3((waves + 3) / 4), dim3(256), 0, stream, P, prof,
                        nprof, labels, (rbg::Agg *)segs, segs_per_prof);
-    hipLaunchKernelGGL(rbg::sp_seed_merge, dim3((nprof + 63) / 64), dim3(64), 0, stream, P, prof,
+    hipLaunchKernelGGL(rbg::sp_seed_merge, dim3((nprof + 3) / 4), dim3(256), 0, stream, P, prof,
                        nprof, (const rbg::Agg *)segs, segs_per_prof, best, failed);
     return hipGetLastError();
 }
