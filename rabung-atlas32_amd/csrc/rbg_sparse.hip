@@ -206,10 +206,20 @@ __device__ __forceinline__ void row0_planes(const SparsePrime &s, const SparsePr
 #pragma unroll
         for (u32 u = 0; u < 8u; ++u) {
             if (q0 + u >= nq) break;
-            const int y = yv[u];
-            const int a = y < 0 ? -y : y;
-            const bool in = y != 0 && a <= B;
-            const u32 c = !in ? 0u : y > 0 ? mod_r(jv[u]) : mod_r(mod_m(jv[u] + s.jneg));
+            // the step's 64 numerators: all positive and in the table, all
+            // negative and in it (wave-uniform tests), or neither
+            const int ylo = (int)((first + q0 + u) * 64u) - B;
+            u32 c;
+            if (ylo > 0 && ylo + 63 <= B) {
+                c = mod_r(jv[u]);
+            } else if (ylo + 63 < 0 && ylo >= -B) {
+                c = mod_r(mod_m(jv[u] + s.jneg));
+            } else {
+                const int y = yv[u];
+                const int a = y < 0 ? -y : y;
+                const bool in = y != 0 && a <= B;
+                c = !in ? 0u : y > 0 ? mod_r(jv[u]) : mod_r(mod_m(jv[u] + s.jneg));
+            }
 #pragma unroll
             for (int b = 0; b < NP; ++b) {
                 const u64 bits = ballot64((c >> b) & 1u);
