@@ -109,7 +109,10 @@ typedef struct rbg_config {
                                 reference's D = floor(sqrt(p/H))).  2..8; 0 = 3.  Where B
                                 would pass 2^20 - 2, D grows back toward the n = 2 value */
     uint32_t filter_chunk;   /* sparse filter: tasks one wave claims from its XCD's
-                                queue at a time, 1..256; 0 = 16 */
+                                queue at a time, 1..256; 0 = 16 for launches of more
+                                than 10^4 tasks per wave, else by each profile's
+                                cover radius B (4 for B >= 400000, 8 for B >= 100000,
+                                else 16) */
     uint32_t spec_off;       /* sparse filter: nonzero starts every profile from its
                                 seed threshold only; 0 also applies the speculative
                                 threshold (the largest L with at least 14 runs of

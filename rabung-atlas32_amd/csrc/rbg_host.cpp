@@ -492,7 +492,8 @@ u32 cover_q(const rbg_ctx *c) { return c->cfg.cover_q ? c->cfg.cover_q : kCoverQ
 
 u32 cover_div(const rbg_ctx *c) { return c->cfg.cover_div ? c->cfg.cover_div : 3u; }
 
-u32 filter_chunk(const rbg_ctx *c) { return c->cfg.filter_chunk ? c->cfg.filter_chunk : 16u; }
+// 0: chosen per launch (sparse_run)
+u32 filter_chunk(const rbg_ctx *c) { return c->cfg.filter_chunk; }
 
 // Speculative threshold of order r for p: the largest L whose expected
 // number of runs of length >= L over the half interval of a random
@@ -1098,7 +1099,7 @@ int sparse_run(rbg_ctx *c, rbg_record *recs, u32 *err, std::vector<u32> &failed_
     std::vector<u32> &tpre = c->h_tpre;
     tpre.clear();
     tpre.reserve(nf + 16);
-    u64 most = 0;
+    u64 most = 0, tasks = 0;
     for (int wide = 0; wide < 2; ++wide) {
         const u32 f0 = wide ? c->nprof32 : 0u;
         for (u32 x = 0; x < 8; ++x) {
@@ -1109,11 +1110,20 @@ int sparse_run(rbg_ctx *c, rbg_record *recs, u32 *err, std::vector<u32> &failed_
             }
             tpre.push_back((u32)at);
             most = std::max(most, at);
+            tasks += at;
         }
     }
+    // queue claims: the configured size, else 16 tasks for launches with more
+    // than 10^4 tasks per wave (the end of the launch is then a small share
+    // and large claims amortise the atomic: batch64 3.59 s of filter against
+    // 3.73 with claims sized per profile) and sized by each profile's cover
+    // radius in the kernel below that (top-256 40.7 -> 39.6 ms, 128-prime
+    // batches at 0.257e9 3.95 -> 3.57 ms and at 4.237e9 23.3 -> 20.8 ms)
+    const u32 chunk = filter_chunk(c) ? filter_chunk(c)
+                      : tasks > 10000ull * grid * 4u ? 16u : 0u;
     // a per-XCD counter ends at most one claim per wave past its items
     if (std::max<u64>(std::max(dmax[0], dmax[1]) * (nf + 8ull * xcd_group(c)), most) +
-            4ull * grid * filter_chunk(c) > 0xFFFFFFFFull)
+            4ull * grid * (chunk ? chunk : 16u) > 0xFFFFFFFFull)
         return c->fail(RBG_ECAPACITY, "too many filter tasks");
     HIPCK(c, c->sp_tpre.ensure(tpre.size() * 4));
     HIPCK(c, hipMemcpyAsync(c->sp_tpre.ptr, tpre.data(), tpre.size() * 4, hipMemcpyHostToDevice,
@@ -1153,7 +1163,7 @@ int sparse_run(rbg_ctx *c, rbg_record *recs, u32 *err, std::vector<u32> &failed_
                                        (unsigned long long *)c->sp_stats.ptr,
                                        (u32 *)((char *)c->sp_stats.ptr + kQueueOffset) + 8 * wide,
                                        wide, wide ? grid_wide : grid, xcd_group(c),
-                                       filter_chunk(c),
+                                       chunk,
                                        extend_limit(c), s));
         }
         return RBG_OK;

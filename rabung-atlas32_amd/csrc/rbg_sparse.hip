@@ -1082,14 +1082,26 @@ __device__ void filter_task(const FilterCtx<D> &c, u32 sp, u32 phi, u32 *qb, u64
 
 constexpr u32 kFilterWaves = 4;  // waves per filter block (blockDim 256)
 
-// Next task of the wave: the rest of its chunk of `chunk` tasks, then the
-// chunk claimed during the chunk's last task (`claim` is valid in lane 0).
-__device__ __forceinline__ u32 next_task(u32 &k, u32 &first, u32 claim, u32 chunk) {
-    if (++k == chunk) {
+// Next task of the wave: the rest of its chunk of `csz` tasks, then the
+// chunk of `nsz` tasks claimed during the chunk's last task (`claim` is
+// valid in lane 0).
+__device__ __forceinline__ u32 next_task(u32 &k, u32 &first, u32 claim, u32 &csz, u32 nsz) {
+    if (++k == csz) {
         first = __builtin_amdgcn_readfirstlane(claim);
+        csz = nsz;
         k = 0;
     }
     return first + k;
+}
+
+// Tasks per queue claim: `chunk` when set, else by the profile's cover
+// radius B (about the words a task holds, so the time a claim's round trip
+// has to hide behind): long tasks near 2^32 share their XCD's queue best in
+// small claims (4: more waves on the same few profiles' rows), short ones
+// in larger claims that amortise the atomic (16).  Any size gives the same
+// tasks and records.
+__device__ __forceinline__ u32 claim_size(u32 chunk, u32 B) {
+    return chunk ? chunk : B >= 400000u ? 4u : B >= 100000u ? 8u : 16u;
 }
 
 // Context of one (profile, denominator) task; false when the task has
@@ -1204,6 +1216,7 @@ __global__ __launch_bounds__(256, FilterWaves<D>::value) void sp_filter(const Sp
     // (their planes then come through its own L2), so no XCD idles while
     // another still has work.
     const u32 lane = threadIdx.x & 63u;
+    u32 lastB = 0;  // cover radius of the wave's last task (sizes its next claim)
     for (u32 v = 0; v < kXcds; ++v) {
     const u32 xcd = (blockIdx.x + v) % kXcds;
     const u32 nprof_x = nprof > xcd ? (nprof - xcd + kXcds - 1) / kXcds : 0u;
@@ -1220,10 +1233,14 @@ __global__ __launch_bounds__(256, FilterWaves<D>::value) void sp_filter(const Sp
     const u32 items = kGroup == 1 ? (nprof_x ? tp[nprof_x] : 0u) : ngroups * kGroup * dmax;
     u32 cur = 0, lo = 0, hi = 0;  // profile of the list being walked and its tasks [lo, hi)
     u32 claim = 0;
-    if (lane == 0) claim = atomicAdd(&queue[xcd], chunk);
+    u32 csz = claim_size(chunk, lastB), nsz = csz;
+    if (lane == 0) claim = atomicAdd(&queue[xcd], csz);
     u32 first = __builtin_amdgcn_readfirstlane(claim), k = 0;
-    for (u32 item = first; item < items; item = next_task(k, first, claim, chunk)) {
-        if (k == chunk - 1u && lane == 0) claim = atomicAdd(&queue[xcd], chunk);
+    for (u32 item = first; item < items; item = next_task(k, first, claim, csz, nsz)) {
+        if (k == csz - 1u) {
+            nsz = claim_size(chunk, lastB);
+            if (lane == 0) claim = atomicAdd(&queue[xcd], nsz);
+        }
         u32 dd, local;
         if (kGroup == 1) {
             if (item >= hi) {
@@ -1254,6 +1271,7 @@ __global__ __launch_bounds__(256, FilterWaves<D>::value) void sp_filter(const Sp
         if (!task_setup<D>(P, prof + kk, dd, planes, inv, tab_val, tab_lab, &best[kk], failed,
                            ext_limit, c, sp, phi))
             continue;
+        lastB = c.B;
         const SparseProf *fg = c.f;
         switch (fg->np) {
         case 1: filter_task<D, 1>(c, sp, phi, qb, qs, cw, n_windows, n_words, n_done,
