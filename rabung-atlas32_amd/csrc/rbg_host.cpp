@@ -903,7 +903,9 @@ int sparse_plan(rbg_ctx *c, const u64 *primes, const std::vector<u32> &idx_in,
         std::vector<u16> lab;
     };
     std::vector<Item> items(n);
-    const u32 nt = n < 128 ? 1u : std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    // a prime's plan takes a few microseconds (its root of order m), while
+    // starting and joining a thread costs tens: threads only for large blocks
+    const u32 nt = std::max(1u, std::min({16u, std::thread::hardware_concurrency(), n / 256u}));
     std::vector<Part> parts(nt);
     auto build = [&](u32 t) {
         Part &pt = parts[t];
