@@ -568,8 +568,8 @@ def main():
         ops = ptimings[-1]["products"] * OPS_PER_PRODUCT
         scan_s = sum(t["dense_scan_ms"] for t in ptimings) / len(ptimings) / 1000.0
         plain = {"method": "dense Montgomery scan with label reuse (powers for the positions "
-                           "coprime to 6, labels of the others from x/2 and x/3), phase "
-                           "launches per prime",
+                           "coprime to 6, labels of the others from x/2 and x/3), one "
+                           "launch per phase over groups of primes",
                  "positions_per_step": ptimings[-1]["positions"],
                  "products_per_step": ptimings[-1]["products"],
                  "profiles_per_s": round(profiles / (pdt / psteps), 3),
